@@ -1,0 +1,134 @@
+/*
+ * titus_hip.h -- C-ABI of the MI355X (gfx950) DLRA continuous-slowing-down
+ * transport step.
+ *
+ * The reference (CSMMLab/TITUS, Julia) drives this path through a mutable
+ * solver object:
+ *
+ *   SolverCSD(settings)                                 code/SolverCSD.jl:63-146
+ *   SolveFirstCollisionSourceDLR(obj)                   code/SolverCSD.jl:891-1120
+ *   SolveFirstCollisionSourceDLRBUGRejection(obj)       code/SolverCSD.jl:2011-2374
+ *
+ * Each entry point below replaces one piece of that interface.  All arrays
+ * are FP64 and column-major (Julia layout), so a Julia `ccall` passes its
+ * arrays without copies.  Spatial cells are numbered c = i*ny + j (0-based),
+ * the reference's vectorIndex (code/utils.jl:98-100) minus one.
+ *
+ * Conventions: every function returns 0 on success or a negative TITUS_E_*
+ * code; titus_last_error() then describes the failure (thread-local).  No
+ * exception crosses the ABI.  A context owns one HIP stream and device
+ * buffers; it is not shared between threads; separate contexts are
+ * independent.  The library copies every input at titus_create; the caller
+ * keeps ownership of its buffers.
+ */
+#ifndef TITUS_HIP_H
+#define TITUS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TITUS_ABI_VERSION 1
+
+typedef struct titus_ctx titus_ctx;
+
+enum {
+  TITUS_OK = 0,
+  TITUS_E_ARG = -1,        /* invalid argument or inconsistent sizes */
+  TITUS_E_HIP = -2,        /* a HIP runtime call failed */
+  TITUS_E_NONFINITE = -3,  /* a non-finite value appeared in the state */
+  TITUS_E_RANK = -4,       /* rank bookkeeping failed (cap, completion) */
+  TITUS_E_STATE = -5,      /* call out of order (e.g. step past the grid) */
+  TITUS_E_NODEVICE = -6    /* no usable GPU */
+};
+
+enum { TITUS_INTEGRATOR_FIXED = 0, TITUS_INTEGRATOR_BUG = 1 };
+enum { TITUS_PRECISION_FP64 = 0, TITUS_PRECISION_MIXED = 1 };
+
+/* Problem description: the operands SolverCSD(settings) builds plus the
+ * ordinate reduction and initial factors of the Solve* functions
+ * (code/SolverCSD.jl:63-146, 904-961, 2025-2082). */
+typedef struct titus_problem_v1 {
+  int32_t version;            /* TITUS_ABI_VERSION */
+  int32_t nx, ny;             /* NCellsX, NCellsY */
+  double dx, dy;              /* settings.dx, settings.dy */
+  const double* density;      /* C = nx*ny, densityVec (SolverCSD.jl:143) */
+
+  int32_t nm;                 /* moments carried */
+  const int32_t* degree;      /* nm: Legendre degree l of each moment */
+  const double* Ax;           /* nm x nm  pn.Ax    (PNSystem.jl:142-194) */
+  const double* Az;           /* nm x nm  pn.Az */
+  const double* absAx;        /* nm x nm  AbsAx    (SolverCSD.jl:83-90) */
+  const double* absAz;        /* nm x nm  AbsAz    (SolverCSD.jl:92-97) */
+
+  int32_t nq;                 /* reduced ordinates */
+  const double* omega;        /* nq x 3   qReduced (SolverCSD.jl:922) */
+  const double* M;            /* nm x nq  MReduced (SolverCSD.jl:923) */
+
+  int32_t ne;                 /* energy grid points, csd.eTrafo */
+  double dE;                  /* eTrafo[2]-eTrafo[1] (SolverCSD.jl:960) */
+  const double* stopping;     /* ne, csd.S */
+  int32_t nsigma;             /* nPN+1 */
+  const double* sigma;        /* ne x nsigma, row-major: sigma_l(eGrid[n]) */
+
+  int32_t nb;                 /* ghost cells */
+  const int32_t* bcells;      /* nb distinct ghost cells (SolverCSD.jl:108-131) */
+  const double* bc_base;      /* nb x nq: PsiBeam without its energy factor */
+  const double* bc_scale;     /* ne: energy factor of PsiBeam at eGrid[n] */
+  int32_t nrestore;           /* cells whose ordinate-0 flux survives the sweep */
+  const int32_t* restore_cells;  /* (SolverCSD.jl:1002,1005) */
+  int32_t ncand;              /* canonical-completion candidates (interior cells) */
+  const int32_t* cand_cells;
+
+  int32_t integrator;         /* TITUS_INTEGRATOR_* */
+  int32_t precision;          /* TITUS_PRECISION_* */
+  int32_t r;                  /* initial rank */
+  int32_t rank_cap;           /* settings.r for the adaptive integrator */
+  double theta, theta_index;  /* settings.ϑ, settings.ϑIndex */
+  double rank_tol;            /* relative singular-value floor of the bases */
+  const double* X0;           /* C x r  (NULL: svd!(zeros) start, ghost rows 0) */
+  const double* S0;           /* r x r  (NULL: zeros) */
+  const double* W0;           /* nm x r (NULL: identity columns) */
+} titus_problem_v1;
+
+/* SolverCSD(settings): validate, copy operands to device `device`. */
+int titus_create(struct titus_ctx** out, const titus_problem_v1* problem, int32_t device);
+
+/* One iteration `for n = ...` of the energy loop (SolverCSD.jl:972-1114 /
+ * 2102-2369).  Steps run in order 1..ne-1; `step` must be the next one. */
+int titus_step(struct titus_ctx* ctx, int32_t step);
+
+/* Run the remaining steps up to and including `last_step` (<= 0: to the end
+ * of the grid as the reference's loop does). */
+int titus_solve(struct titus_ctx* ctx, int32_t last_step);
+
+/* Number of steps taken so far and the current rank. */
+int titus_progress(const struct titus_ctx* ctx, int32_t* steps_done, int32_t* rank);
+
+/* Copy the state out (NULL pointers are skipped):
+ *   dose C; X C x rank; S rank x rank; W nm x rank; psi C x nq (cell-major
+ *   columns, Mat2Vec layout). */
+int titus_get_state(struct titus_ctx* ctx, double* dose, double* X, double* S, double* W,
+                    double* psi);
+
+/* Rank before each step taken (`rankInTime`, SolverCSD.jl:2109) and the
+ * streaming eta indicator (SolverCSD.jl:2220); n entries each at most. */
+int titus_get_history(const struct titus_ctx* ctx, int32_t* ranks, double* etas, int32_t n);
+
+/* Device time of the library's kernels since the last reset, measured with
+ * HIP events on the context's stream: total milliseconds and launch count for
+ * kernel `name` ("sweep", "stream_K", "galerkin", "gram", "gemm", ...). */
+int titus_set_timing(struct titus_ctx* ctx, int32_t enabled);
+int titus_kernel_time(const struct titus_ctx* ctx, const char* name, double* total_ms,
+                      int64_t* launches);
+
+void titus_destroy(struct titus_ctx* ctx);
+const char* titus_last_error(void);
+int32_t titus_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TITUS_HIP_H */

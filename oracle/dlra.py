@@ -1,0 +1,432 @@
+"""FP64 CPU oracle of the DLRA CSD energy loop -- TEST INFRASTRUCTURE ONLY.
+
+This module is the checker.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` may import it; the product path
+(``titus_amd``) never does.
+
+It restates, in numpy with T = Float64 throughout:
+
+* ``solve_fixed``  -- ``SolveFirstCollisionSourceDLR``   (``code/SolverCSD.jl:891-1120``)
+* ``solve_bug``    -- ``SolveFirstCollisionSourceDLRBUGRejection`` with the
+                      rejection branches off (``code/SolverCSD.jl:2011-2374``)
+* ``truncate``     -- ``truncate!``                       (``code/SolverCSD.jl:3369-3399``)
+* ``upwind_flux``  -- ``solveFluxUpwind!``                (``code/SolverCSD.jl:388-463``)
+* ``stencils``     -- ``Stencils`` applied matrix-free    (``code/stencils.jl:12-130``)
+* ``solve_dense``  -- the same splitting on the full C x Nm moment matrix
+                      (the collided part of ``SolveFirstCollisionSource``,
+                      ``SolverCSD.jl:465-586``, with the DLR step's conventions)
+
+Deliberate, documented choices where the reference is not deterministic:
+
+* ``svd!(K).U`` and ``qr([X K]).Q`` only matter through the subspace they span
+  when K has full numerical rank.  Where it does not, LAPACK / numpy fill the
+  remaining columns with rounding-determined directions.  Here the basis is
+  the range of K (singular values above ``RANK_TOL`` times the scale) completed
+  by canonical vectors (interior cells for spatial bases, moments for angular
+  bases, coefficient indices for the truncation SVD) orthogonalised against
+  it -- which is what LAPACK returns for an exactly zero matrix.
+* ``truncate!`` keeps singular values below ``RANK_TOL * sigma_1`` as exact
+  zeros and completes their singular vectors the same way.
+
+Parity status of each function is listed in the module docstrings of
+``tests/test_oracle_*.py``.
+"""
+import numpy as np
+
+RANK_TOL = 1e-7
+
+
+# ---------------------------------------------------------------------------
+# spatial operators
+# ---------------------------------------------------------------------------
+def stencils(U, rho, nx, ny, dx, dy):
+    """(L1x U, L1y U, L2x U, L2y U) for U of shape C x m (``stencils.jl:26-128``).
+
+    Interior rows only (i in 1..nx-2, j in 1..ny-2, 0-based); ghost rows are 0.
+    (L1x u)_c = (u_c/rho_c - u_{i-1}/(2 rho_{i-1}) - u_{i+1}/(2 rho_{i+1}))/dx
+    (L2x u)_c = (u_{i+1}/rho_{i+1} - u_{i-1}/rho_{i-1})/(2 dx), same along y.
+    """
+    m = U.shape[1]
+    V = (U / rho[:, None]).reshape(nx, ny, m)
+    out = [np.zeros((nx, ny, m)) for _ in range(4)]
+    c = V[1:-1, 1:-1]
+    w, e = V[:-2, 1:-1], V[2:, 1:-1]
+    s, n = V[1:-1, :-2], V[1:-1, 2:]
+    out[0][1:-1, 1:-1] = (c - 0.5 * w - 0.5 * e) / dx
+    out[1][1:-1, 1:-1] = (c - 0.5 * s - 0.5 * n) / dy
+    out[2][1:-1, 1:-1] = (e - w) / (2 * dx)
+    out[3][1:-1, 1:-1] = (n - s) / (2 * dy)
+    return [o.reshape(nx * ny, m) for o in out]
+
+
+def upwind_flux(psi, rho, omega, nx, ny, dx, dy):
+    """First-order upwind flux of psi/rho per ordinate (``SolverCSD.jl:388-463``).
+
+    psi is C x Q; returns C x Q with zeros on ghost cells.
+    """
+    Q = psi.shape[1]
+    V = (psi / rho[:, None]).reshape(nx, ny, Q)
+    F = np.zeros((nx, ny, Q))
+    ox, oz = omega[:, 0], omega[:, 2]
+    c = V[1:-1, 1:-1]
+    dxp = c - V[:-2, 1:-1]          # Omega_x >= 0: east = i, west = i-1
+    dxm = V[2:, 1:-1] - c           # Omega_x < 0: east = i+1, west = i
+    dyp = c - V[1:-1, :-2]
+    dym = V[1:-1, 2:] - c
+    fx = np.where(ox >= 0, dxp, dxm) * (ox / dx)
+    fy = np.where(oz >= 0, dyp, dym) * (oz / dy)
+    F[1:-1, 1:-1] = fx + fy
+    return F.reshape(nx * ny, Q)
+
+
+# ---------------------------------------------------------------------------
+# bases
+# ---------------------------------------------------------------------------
+def range_basis(A, ncols, candidates, against=None, scale=None, tol=None, greedy=False):
+    """Orthonormal basis of range(A) (minus span(against)), completed canonically.
+
+    Singular values <= tol * scale count as zero (scale defaults to sigma_1).
+    Missing columns are canonical vectors e_c orthogonalised against everything
+    kept so far.  ``greedy=False`` (spatial bases, many candidates): take the
+    candidates in order, skipping one whose residual norm^2 is <= 1/2.
+    ``greedy=True`` (small bases): take the candidate of largest residual,
+    lowest index on ties.  Returns an m x ncols orthonormal matrix.
+    """
+    if tol is None:
+        tol = RANK_TOL
+    m = A.shape[0]
+    if against is not None:
+        for _ in range(2):
+            A = A - against @ (against.T @ A)
+    U, sv, _ = np.linalg.svd(A, full_matrices=False)
+    if scale is None:
+        scale = sv[0] if sv.size else 0.0
+    p = int(np.count_nonzero(sv > tol * scale)) if scale > 0 else 0
+    p = min(p, ncols)
+    B = U[:, :p]
+    if against is not None:
+        base = np.hstack([against, B])
+    else:
+        base = B
+    cols = [B]
+    need = ncols - p
+    cand = list(candidates)
+    if greedy:
+        while need:
+            R = np.eye(m)[:, cand]
+            for _ in range(2):
+                R = R - base @ (base.T @ R)
+            nr = np.einsum("ij,ij->j", R, R)
+            t = int(np.argmax(nr))
+            if nr[t] <= 1e-8:
+                break
+            v = (R[:, t] / np.sqrt(nr[t]))[:, None]
+            cols.append(v)
+            base = np.hstack([base, v])
+            need -= 1
+    for c in ([] if greedy else cand):
+        if need == 0:
+            break
+        v = np.zeros(m)
+        v[c] = 1.0
+        for _ in range(2):
+            v = v - base @ (base.T @ v)
+        nv = np.linalg.norm(v)
+        if nv * nv > 0.5:
+            v = (v / nv)[:, None]
+            cols.append(v)
+            base = np.hstack([base, v])
+            need -= 1
+    if need:
+        raise RuntimeError("canonical completion ran out of candidates")
+    return np.hstack(cols)
+
+
+def svd_completed(S, tol=None):
+    """SVD of a square matrix with null singular vectors completed canonically."""
+    if tol is None:
+        tol = RANK_TOL
+    n = S.shape[0]
+    U, sv, Vt = np.linalg.svd(S)
+    p = int(np.count_nonzero(sv > tol * sv[0])) if sv[0] > 0 else 0
+    Uc = range_basis(U[:, :p], n, range(n), greedy=True)
+    Vc = range_basis(Vt[:p].T, n, range(n), greedy=True)
+    # keep the paired singular vectors exactly as the SVD returned them
+    Uc[:, :p] = U[:, :p]
+    Vc[:, :p] = Vt[:p].T
+    d = np.concatenate([sv[:p], np.zeros(n - p)])
+    return Uc, d, Vc
+
+
+def truncate(X, S, W, theta, theta_index, rank_cap, r_min=2):
+    """``truncate!`` (``SolverCSD.jl:3369-3399``).
+
+    tol = theta * ||sigma||_2^thetaIndex; keep the smallest j (1-based) whose
+    L1 tail sum(sigma[j:]) is below tol, else floor(size/2); clamp to
+    [r_min, rank_cap].
+    """
+    U, D, V = svd_completed(S)
+    n = len(D)
+    tol = theta * np.linalg.norm(D) ** theta_index
+    rmax = n // 2
+    for j in range(n):
+        if abs(D[j:].sum()) < tol:
+            rmax = j + 1
+            break
+    rmax = max(min(rmax, rank_cap), r_min)
+    return X @ U[:, :rmax], np.diag(D[:rmax]), W @ V[:, :rmax]
+
+
+# ---------------------------------------------------------------------------
+# energy loop
+# ---------------------------------------------------------------------------
+class _Ops:
+    def __init__(self, P):
+        self.P = P
+        self.nx, self.ny, self.dx, self.dy = P["nx"], P["ny"], P["dx"], P["dy"]
+        self.rho = np.asarray(P["density"], dtype=np.float64)
+        self.bcells = np.asarray(P["bcells"])
+        self.interior = np.asarray(P["interior"])
+        self.restore = np.asarray(P["restore_cells"])
+        self.Ax, self.Az = np.asarray(P["Ax"]), np.asarray(P["Az"])
+        self.aAx, self.aAz = np.asarray(P["absAx"]), np.asarray(P["absAz"])
+        self.M = np.asarray(P["M"])
+        self.deg = np.asarray(P["degree"])
+        self.Nm = P["Nm"]
+
+    def L(self, U):
+        return stencils(U, self.rho, self.nx, self.ny, self.dx, self.dy)
+
+    def galerkin_x(self, Xa, Xb):
+        """[Xa^T L1x Xb, Xa^T L1y Xb, Xa^T L2x Xb, Xa^T L2y Xb]."""
+        return [Xa.T @ LX for LX in self.L(Xb)]
+
+    def galerkin_w(self, Wa, Wb):
+        return [Wa.T @ self.aAx @ Wb, Wa.T @ self.aAz @ Wb,
+                Wa.T @ self.Ax @ Wb, Wa.T @ self.Az @ Wb]
+
+    def stream_K(self, K, WAW, dE):
+        """K - dE (L2x K WAxW + L2y K WAzW + L1x K W|Ax|W + L1y K W|Az|W)."""
+        L1x, L1y, L2x, L2y = self.L(K)
+        return K - dE * (L2x @ WAW[2] + L2y @ WAW[3] + L1x @ WAW[0] + L1y @ WAW[1])
+
+    def stream_L(self, L, XLX, dE):
+        """L - dE (Ax L XL2xX^T + Az L XL2yX^T + |Ax| L XL1xX^T + |Az| L XL1yX^T)."""
+        return L - dE * (self.Ax @ L @ XLX[2].T + self.Az @ L @ XLX[3].T
+                         + self.aAx @ L @ XLX[0].T + self.aAz @ L @ XLX[1].T)
+
+    def stream_S(self, S, XLX, WAW, dE):
+        return S - dE * (XLX[2] @ S @ WAW[2] + XLX[3] @ S @ WAW[3]
+                         + XLX[0] @ S @ WAW[0] + XLX[1] @ S @ WAW[1])
+
+
+def _sweep(ops, psi, sig0, dE, omega):
+    """Uncollided update with the reference's boundary restore (Appendix quirk Q2)."""
+    flux = upwind_flux(psi, ops.rho, omega, ops.nx, ops.ny, ops.dx, ops.dy)
+    keep = psi[ops.restore, 0].copy()
+    psi = (psi - dE * flux) / (1.0 + dE * sig0)
+    psi[ops.restore, 0] = keep
+    return psi
+
+
+def _begin_step(ops, P, psi, s):
+    """BC at E[s-1] (``SolverCSD.jl:977-988``)."""
+    psi[ops.bcells, :] = P["bc_base"] * P["bc_scale"][s - 1]
+    return psi
+
+
+def solve_fixed(P, steps=None):
+    """Fixed-rank first-collision-source DLRA (``SolverCSD.jl:891-1120``).
+
+    Returns dict(dose, X, S, W, psi, Xout=X U, sigma=Sigma/sqrt2, WV=W V).
+    """
+    ops = _Ops(P)
+    r = P["r"]
+    X = np.array(P["X0"], dtype=np.float64)
+    W = np.array(P["W0"], dtype=np.float64)
+    S = np.array(P["S0"], dtype=np.float64)
+    C, Q = P["C"], P["Q"]
+    psi = np.zeros((C, Q))
+    dose = np.zeros(C)
+    rho = ops.rho
+    dE = P["dE"]
+    Sst = P["Sst"]
+    omega = P["omega"]
+    M = ops.M
+    nE = P["nE"] if steps is None else min(P["nE"], steps + 1)
+    icand = ops.interior
+    mcand = range(ops.Nm)
+    X[ops.bcells, :] = 0.0
+    for s in range(1, nE):
+        sig = P["sigma"][s]
+        psi = _begin_step(ops, P, psi, s)
+        dose += 0.5 * dE * (X @ (S @ W[0, :]) + psi @ M[0, :]) * Sst[s - 1] / rho
+        psi = _sweep(ops, psi, sig[0], dE, omega)
+        Dvec = sig[ops.deg]
+        Dout = sig[0] - Dvec
+        if s > 1:
+            # K-step (SolverCSD.jl:1019-1031)
+            X[ops.bcells, :] = 0.0
+            K = X @ S
+            WAW = ops.galerkin_w(W, W)
+            K = ops.stream_K(K, WAW, dE)
+            XNew = range_basis(K, r, icand)
+            MUp = XNew.T @ X
+            # L-step (:1033-1046)
+            L = W @ S.T
+            XLX = ops.galerkin_x(X, X)
+            L = ops.stream_L(L, XLX, dE)
+            WNew = range_basis(L, r, mcand, greedy=True)
+            NUp = WNew.T @ W
+            W, X = WNew, XNew
+            # S-step (:1051-1063)
+            S = MUp @ S @ NUp.T
+            XLX = ops.galerkin_x(X, X)
+            WAW = ops.galerkin_w(W, W)
+            S = ops.stream_S(S, XLX, WAW, dE)
+        # out-scattering (:1067-1074): L = W S^T row-scaled, W,S from its SVD
+        L = (W @ S.T) / (1.0 + dE * Dout)[:, None]
+        W = range_basis(L, r, mcand, greedy=True)
+        S = L.T @ W
+        # in-scattering (:1079-1102)
+        X[ops.bcells, :] = 0.0
+        MtDW = M.T @ (Dvec[:, None] * W)
+        K = X @ S + dE * psi @ MtDW
+        K[ops.bcells, :] = 0.0
+        XNew = range_basis(K, r, icand)
+        MUp = XNew.T @ X
+        L = W @ S.T + dE * Dvec[:, None] * (M @ (psi.T @ X))
+        WNew = range_basis(L, r, mcand, greedy=True)
+        NUp = WNew.T @ W
+        W, X = WNew, XNew
+        S = MUp @ S @ NUp.T
+        S = S + dE * (X.T @ psi) @ (M.T @ (Dvec[:, None] * W))
+        dose += 0.5 * dE * (X @ (S @ W[0, :]) + psi @ M[0, :]) * Sst[s] / rho
+    U, sv, Vt = np.linalg.svd(S)
+    return dict(dose=dose, X=X, S=S, W=W, psi=psi, Xout=X @ U,
+                sigma=sv / np.sqrt(2.0), WV=W @ Vt.T, steps=nE - 1)
+
+
+def solve_bug(P, steps=None):
+    """Rank-adaptive augmented BUG (``SolverCSD.jl:2011-2374``), rejection off.
+
+    The reference's streaming-rejection branch re-runs a step after the dose
+    and sweep have been applied (double counting) and its scattering branch
+    reads an undefined ``rmax``; both are disabled here.  The eta indicator of
+    the streaming step is returned as a diagnostic.
+    """
+    ops = _Ops(P)
+    r = P["r"]
+    X = np.array(P["X0"], dtype=np.float64)
+    W = np.array(P["W0"], dtype=np.float64)
+    S = np.array(P["S0"], dtype=np.float64)
+    C, Q = P["C"], P["Q"]
+    psi = np.zeros((C, Q))
+    dose = np.zeros(C)
+    rho = ops.rho
+    dE = P["dE"]
+    Sst = P["Sst"]
+    omega = P["omega"]
+    M = ops.M
+    theta, thidx, cap = P["theta"], P["thetaIndex"], P["rank_cap"]
+    nE = P["nE"] if steps is None else min(P["nE"], steps + 1)
+    icand = ops.interior
+    mcand = range(ops.Nm)
+    ranks, etas = [], []
+    X[ops.bcells, :] = 0.0
+    for s in range(1, nE):
+        # the reference's while-loop breaks at n == nEnergies (:2104), so its
+        # last grid point is never stepped; steps=None follows that rule
+        if steps is None and s == P["nE"] - 1:
+            break
+        ranks.append(r)
+        sig = P["sigma"][s]
+        psi = _begin_step(ops, P, psi, s)
+        dose += 0.5 * dE * (X @ (S @ W[0, :]) + psi @ M[0, :]) * Sst[s - 1] / rho
+        psi = _sweep(ops, psi, sig[0], dE, omega)
+        Dvec = sig[ops.deg]
+        Dout = sig[0] - Dvec
+        if s > 1:
+            XLX = ops.galerkin_x(X, X)
+            WAW = ops.galerkin_w(W, W)
+            X[ops.bcells, :] = 0.0
+            K = ops.stream_K(X @ S, WAW, dE)
+            Kn = np.linalg.norm(K)
+            Xt = range_basis(K, r, icand, against=X, scale=Kn)
+            X1 = np.hstack([X, Xt])
+            Mu = X1.T @ X
+            L = ops.stream_L(W @ S.T, XLX, dE)
+            Ln = np.linalg.norm(L)
+            Wt = range_basis(L, r, mcand, against=W, scale=Ln, greedy=True)
+            W1 = np.hstack([W, Wt])
+            Nu = W1.T @ W
+            X, W = X1, W1
+            S = Mu @ S @ Nu.T
+            XLX = ops.galerkin_x(X, X)
+            WAW = ops.galerkin_w(W, W)
+            Sup = ops.stream_S(S, XLX, WAW, dE)
+            XUP, SUP, WUP = truncate(X, Sup, W, theta, thidx, cap)
+            # eta diagnostic (:2210-2220)
+            XLXt = ops.galerkin_x(Xt, X)
+            WAWt = ops.galerkin_w(W, Wt)
+            eta = np.linalg.norm(XLXt[2] @ S @ WAWt[2] + XLXt[3] @ S @ WAWt[3]
+                                 + XLXt[0] @ S @ WAWt[0] + XLXt[1] @ S @ WAWt[1])
+            etas.append(eta)
+            S, X, W = SUP, XUP, WUP
+            r = S.shape[0]
+        # out-scattering (:2242-2249)
+        L = (W @ S.T) / (1.0 + dE * Dout)[:, None]
+        W = range_basis(L, r, mcand, greedy=True)
+        S = L.T @ W
+        # in-scattering (:2252-2306)
+        MtDW = M.T @ (Dvec[:, None] * W)
+        Xtpsi = X.T @ psi
+        X[ops.bcells, :] = 0.0
+        K = X @ S + dE * psi @ MtDW
+        K[ops.bcells, :] = 0.0
+        Kn = np.linalg.norm(K)
+        Xt = range_basis(K, r, icand, against=X, scale=Kn)
+        X1 = np.hstack([X, Xt])
+        Mu = X1.T @ X
+        L = W @ S.T + dE * Dvec[:, None] * (M @ Xtpsi.T)
+        Ln = np.linalg.norm(L)
+        Wt = range_basis(L, r, mcand, against=W, scale=Ln, greedy=True)
+        W1 = np.hstack([W, Wt])
+        Nu = W1.T @ W
+        X, W = X1, W1
+        S = Mu @ S @ Nu.T
+        MtDW = M.T @ (Dvec[:, None] * W)
+        Sup = S + dE * (X.T @ psi) @ MtDW
+        X, S, W = truncate(X, Sup, W, theta, thidx, cap)
+        r = S.shape[0]
+        dose += 0.5 * dE * (X @ (S @ W[0, :]) + psi @ M[0, :]) * Sst[s] / rho
+    U, sv, Vt = np.linalg.svd(S)
+    return dict(dose=dose, X=X, S=S, W=W, psi=psi, Xout=X @ U, sigma=sv / np.sqrt(2.0),
+                WV=W @ Vt.T, ranks=np.array(ranks), etas=np.array(etas))
+
+
+def solve_dense(P, steps=None, stream=True):
+    """The DLR step's splitting applied to the full C x Nm moment matrix u."""
+    ops = _Ops(P)
+    C, Q, Nm = P["C"], P["Q"], P["Nm"]
+    u = np.zeros((C, Nm))
+    psi = np.zeros((C, Q))
+    dose = np.zeros(C)
+    rho, dE, Sst, omega, M = ops.rho, P["dE"], P["Sst"], P["omega"], ops.M
+    nE = P["nE"] if steps is None else min(P["nE"], steps + 1)
+    for s in range(1, nE):
+        sig = P["sigma"][s]
+        psi = _begin_step(ops, P, psi, s)
+        dose += 0.5 * dE * (u[:, 0] + psi @ M[0, :]) * Sst[s - 1] / rho
+        psi = _sweep(ops, psi, sig[0], dE, omega)
+        Dvec = sig[ops.deg]
+        if s > 1 and stream:
+            u[ops.bcells, :] = 0.0
+            L1x, L1y, L2x, L2y = ops.L(u)
+            u = u - dE * (L2x @ ops.Ax + L2y @ ops.Az + L1x @ ops.aAx + L1y @ ops.aAz)
+        u = u / (1.0 + dE * (sig[0] - Dvec))[None, :]
+        u[ops.bcells, :] = 0.0
+        u = u + dE * psi @ (M.T * Dvec[None, :])
+        u[ops.bcells, :] = 0.0
+        dose += 0.5 * dE * (u[:, 0] + psi @ M[0, :]) * Sst[s] / rho
+    return dict(dose=dose, u=u, psi=psi)

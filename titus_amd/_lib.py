@@ -1,0 +1,204 @@
+"""ctypes binding of ``libtitus_hip.so`` (the C-ABI of ``include/titus_hip.h``).
+
+The library is built in-tree (``python -m titus_amd.build``).  Loading fails
+loudly when it is missing: there is no CPU fallback for the energy step.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtitus_hip.so")
+
+ABI_VERSION = 1
+INTEGRATOR_FIXED, INTEGRATOR_BUG = 0, 1
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+
+_P = ctypes.POINTER(ctypes.c_double)
+_PI = ctypes.POINTER(ctypes.c_int32)
+
+ERRORS = {
+    -1: "TITUS_E_ARG", -2: "TITUS_E_HIP", -3: "TITUS_E_NONFINITE", -4: "TITUS_E_RANK",
+    -5: "TITUS_E_STATE", -6: "TITUS_E_NODEVICE",
+}
+
+# every symbol include/titus_hip.h declares
+EXPORTED = ("titus_create", "titus_step", "titus_solve", "titus_progress", "titus_get_state",
+            "titus_get_history", "titus_set_timing", "titus_kernel_time", "titus_destroy",
+            "titus_last_error", "titus_abi_version")
+
+
+class TitusError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class ProblemV1(ctypes.Structure):
+    """``titus_problem_v1``."""
+    _fields_ = [
+        ("version", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("density", _P),
+        ("nm", ctypes.c_int32), ("degree", _PI), ("Ax", _P), ("Az", _P), ("absAx", _P),
+        ("absAz", _P),
+        ("nq", ctypes.c_int32), ("omega", _P), ("M", _P),
+        ("ne", ctypes.c_int32), ("dE", ctypes.c_double), ("stopping", _P),
+        ("nsigma", ctypes.c_int32), ("sigma", _P),
+        ("nb", ctypes.c_int32), ("bcells", _PI), ("bc_base", _P), ("bc_scale", _P),
+        ("nrestore", ctypes.c_int32), ("restore_cells", _PI),
+        ("ncand", ctypes.c_int32), ("cand_cells", _PI),
+        ("integrator", ctypes.c_int32), ("precision", ctypes.c_int32), ("r", ctypes.c_int32),
+        ("rank_cap", ctypes.c_int32), ("theta", ctypes.c_double),
+        ("theta_index", ctypes.c_double), ("rank_tol", ctypes.c_double),
+        ("X0", _P), ("S0", _P), ("W0", _P),
+    ]
+
+
+_lib = None
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m titus_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    lib.titus_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(ProblemV1), ctypes.c_int32]
+    lib.titus_step.argtypes = [vp, ctypes.c_int32]
+    lib.titus_solve.argtypes = [vp, ctypes.c_int32]
+    lib.titus_progress.argtypes = [vp, _PI, _PI]
+    lib.titus_get_state.argtypes = [vp, _P, _P, _P, _P, _P]
+    lib.titus_get_history.argtypes = [vp, _PI, _P, ctypes.c_int32]
+    lib.titus_set_timing.argtypes = [vp, ctypes.c_int32]
+    lib.titus_kernel_time.argtypes = [vp, ctypes.c_char_p, _P, ctypes.POINTER(ctypes.c_int64)]
+    lib.titus_destroy.argtypes = [vp]
+    lib.titus_destroy.restype = None
+    lib.titus_last_error.restype = ctypes.c_char_p
+    lib.titus_abi_version.restype = ctypes.c_int32
+    for name in ("titus_create", "titus_step", "titus_solve", "titus_progress",
+                 "titus_get_state", "titus_get_history", "titus_set_timing",
+                 "titus_kernel_time"):
+        getattr(lib, name).restype = ctypes.c_int
+    if lib.titus_abi_version() != ABI_VERSION:
+        raise RuntimeError("libtitus_hip.so ABI version mismatch")
+    _lib = lib
+    return lib
+
+
+def check(code):
+    if code != 0:
+        msg = load().titus_last_error()
+        raise TitusError(code, msg.decode() if msg else "")
+
+
+def _dp(a):
+    return a.ctypes.data_as(_P)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_PI)
+
+
+def make_problem(P, integrator, precision=PRECISION_FP64, rank_tol=1e-7, initial=None):
+    """Fill a ``ProblemV1`` from ``titus_amd.problem.build_problem`` output.
+
+    Returns (struct, keepalive) -- the arrays must outlive ``titus_create``.
+    """
+    f64 = lambda a: np.asfortranarray(np.asarray(a, dtype=np.float64))
+    i32 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+    keep = {
+        "density": f64(P["density"]), "degree": i32(P["degree"]),
+        "Ax": f64(P["Ax"]), "Az": f64(P["Az"]), "absAx": f64(P["absAx"]), "absAz": f64(P["absAz"]),
+        "omega": f64(P["omega"]), "M": f64(P["M"]), "stopping": f64(P["Sst"]),
+        "sigma": np.ascontiguousarray(P["sigma"], dtype=np.float64),
+        "bcells": i32(P["bcells"]), "bc_base": f64(P["bc_base"]), "bc_scale": f64(P["bc_scale"]),
+        "restore": i32(P["restore_cells"]), "cand": i32(P["interior"]),
+    }
+    s = ProblemV1()
+    s.version = ABI_VERSION
+    s.nx, s.ny, s.dx, s.dy = P["nx"], P["ny"], P["dx"], P["dy"]
+    s.density = _dp(keep["density"])
+    s.nm, s.degree = P["Nm"], _ip(keep["degree"])
+    s.Ax, s.Az = _dp(keep["Ax"]), _dp(keep["Az"])
+    s.absAx, s.absAz = _dp(keep["absAx"]), _dp(keep["absAz"])
+    s.nq, s.omega, s.M = P["Q"], _dp(keep["omega"]), _dp(keep["M"])
+    s.ne, s.dE, s.stopping = P["nE"], P["dE"], _dp(keep["stopping"])
+    s.nsigma, s.sigma = keep["sigma"].shape[1], _dp(keep["sigma"])
+    s.nb, s.bcells = len(keep["bcells"]), _ip(keep["bcells"])
+    s.bc_base, s.bc_scale = _dp(keep["bc_base"]), _dp(keep["bc_scale"])
+    s.nrestore, s.restore_cells = len(keep["restore"]), _ip(keep["restore"])
+    s.ncand, s.cand_cells = len(keep["cand"]), _ip(keep["cand"])
+    s.integrator, s.precision = integrator, precision
+    s.r, s.rank_cap = P["r"], P["rank_cap"]
+    s.theta, s.theta_index, s.rank_tol = P["theta"], P["thetaIndex"], rank_tol
+    if initial is not None:
+        for name in ("X0", "S0", "W0"):
+            keep[name] = f64(initial[name])
+            setattr(s, name, _dp(keep[name]))
+    return s, keep
+
+
+class Context:
+    """Owner of one ``titus_ctx`` (a device-resident solver state)."""
+
+    def __init__(self, P, integrator, precision=PRECISION_FP64, device=0, rank_tol=1e-7,
+                 initial=None):
+        lib = load()
+        self._lib = lib
+        self.C, self.nm, self.nq = P["C"], P["Nm"], P["Q"]
+        self.ne = P["nE"]
+        st, keep = make_problem(P, integrator, precision, rank_tol, initial)
+        h = ctypes.c_void_p()
+        check(lib.titus_create(ctypes.byref(h), ctypes.byref(st), device))
+        del keep
+        self.h = h
+
+    def step(self, s):
+        check(self._lib.titus_step(self.h, s))
+
+    def solve(self, last_step=0):
+        check(self._lib.titus_solve(self.h, last_step))
+
+    def progress(self):
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        check(self._lib.titus_progress(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def state(self):
+        _, r = self.progress()
+        dose = np.zeros(self.C)
+        X = np.zeros((self.C, r), order="F")
+        S = np.zeros((r, r), order="F")
+        W = np.zeros((self.nm, r), order="F")
+        psi = np.zeros((self.C, self.nq), order="F")
+        check(self._lib.titus_get_state(self.h, _dp(dose), _dp(X), _dp(S), _dp(W), _dp(psi)))
+        return dict(dose=dose, X=X, S=S, W=W, psi=psi)
+
+    def history(self, n):
+        ranks = np.zeros(n, dtype=np.int32)
+        etas = np.zeros(n)
+        check(self._lib.titus_get_history(self.h, _ip(ranks), _dp(etas), n))
+        return ranks, etas
+
+    def set_timing(self, on):
+        check(self._lib.titus_set_timing(self.h, 1 if on else 0))
+
+    def kernel_time(self, name):
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        check(self._lib.titus_kernel_time(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.titus_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,59 @@
+"""Build the HIP library in-tree (gfx950) and the host-only small-LA test library.
+
+    python -m titus_amd.build          # or __graft_entry__.build()
+
+Outputs (git-ignored, shipped with the tree):
+    titus_amd/libtitus_hip.so   -- the C-ABI of include/titus_hip.h
+    titus_amd/_smallla_host.so  -- host dense LA with a C test interface
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libtitus_hip.so")
+SMALLLA = os.path.join(HERE, "_smallla_host.so")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    # packed FP32 instructions are kept out of every kernel
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def _hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd):
+    res = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("build step failed: " + " ".join(cmd[:3]) + " ...")
+    return res
+
+
+def build(verbose=False):
+    hipcc = _hipcc()
+    srcs = [os.path.join(CSRC, f) for f in ("kernels.hip", "solver.cpp", "smallla.cpp")]
+    tmp = LIB + ".tmp"
+    _run([hipcc, *HIP_FLAGS, "-shared", "-I", os.path.join(ROOT, "include"), *srcs, "-o", tmp])
+    os.replace(tmp, LIB)
+    tmp = SMALLLA + ".tmp"
+    _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared",
+          os.path.join(CSRC, "smallla.cpp"), os.path.join(CSRC, "smallla_capi.cpp"), "-o", tmp])
+    os.replace(tmp, SMALLLA)
+    if verbose:
+        print("built", LIB, "and", SMALLLA)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)

@@ -1,0 +1,70 @@
+// Device kernels of the DLRA energy step (gfx950) and their host launchers.
+//
+// Layout: every spatial block is column-major C x m with leading dimension
+// `ld` (>= C): column a of block A starts at A + a*ld, cell c = i*ny + j.
+// Small matrices (r x r, k x n) are column-major with their row count as
+// leading dimension.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+namespace titus {
+namespace dev {
+
+// Stencil operators of code/stencils.jl, in the order the Galerkin kernel and
+// the streaming kernel use them.
+enum StencilOp { OP_NONE = -1, OP_L1X = 0, OP_L1Y = 1, OP_L2X = 2, OP_L2Y = 3 };
+
+struct Grid {
+  int nx, ny, C;
+  double dx, dy;
+  const double* rho;            // C
+};
+
+// psi[bcells[b] + q*C] = bc_base[b + q*nb] * scale     (SolverCSD.jl:977-988)
+void launch_bc(hipStream_t st, double* psi, int C, int nq, const int* bcells, int nb,
+               const double* bc_base, double scale);
+
+// dose[c] += coef * (X[c,:] . v + psi[c,:] . m0) / rho[c]   (SolverCSD.jl:990-997)
+void launch_dose(hipStream_t st, const Grid& g, double* dose, const double* X, int ldx, int r,
+                 const double* v, const double* psi, int nq, const double* m0, double coef);
+
+// Uncollided upwind update with the boundary restore (SolverCSD.jl:388-463,
+// 1000-1005): pout = (pin - dE * flux(pin/rho)) / denom on interior cells,
+// pin / denom on ghost cells, pin itself for ordinate 0 of `restore` cells.
+void launch_sweep(hipStream_t st, const Grid& g, const double* pin, double* pout, int nq,
+                  const double* omega, double dE, double denom, const unsigned char* restore);
+
+// Streaming K-step (SolverCSD.jl:1020-1027) with the stencils applied matrix
+// free: K[c,:] = X[c,:] Z0 - dE * sum_{d in {c,e,w,n,s}} X[d,:]/rho[d] Zd on
+// interior cells and X[c,:] Z0 on ghost cells.  Z holds the six r x n blocks
+// [Z0 Zc Ze Zw Zn Zs] side by side (r x 6n, column-major).
+void launch_stream_k(hipStream_t st, const Grid& g, const double* X, int ldx, int r,
+                     const double* Z, int n, double dE, double* K, int ldk);
+
+// Tall-skinny A^T op(B) (m x n) with deterministic two-stage reduction.
+// op = OP_NONE: B itself; otherwise the stencil operator applied to B (the
+// Galerkin products X^T L X of SolverCSD.jl:1035-1038).
+// `partial` must hold tn_partial_size(C, m, n) doubles; result goes to `out`.
+size_t tn_partial_size(int C, int m, int n);
+void launch_tn(hipStream_t st, const Grid& g, const double* A, int lda, int m, const double* B,
+               int ldb, int n, int op, double* partial, double* out);
+
+// out[c, b] = beta * out[c, b] + alpha * sum_a A[c, a] T[a, b];  rows with
+// zero_mask[c] != 0 are set to 0 afterwards (when zero_mask is not null).
+void launch_gemm(hipStream_t st, int C, const double* A, int lda, int k, const double* T, int n,
+                 double alpha, double beta, double* out, int ldo, const unsigned char* zero_mask);
+
+// out[t + k*nrows] = A[rows[t] + k*lda], k < ncols.
+void launch_gather_rows(hipStream_t st, const double* A, int lda, int ncols, const int* rows,
+                        int nrows, double* out);
+
+// A[rows[t] + cols[t]*lda] += vals[t].
+void launch_scatter_add(hipStream_t st, double* A, int lda, const int* rows, const int* cols,
+                        const double* vals, int n);
+
+// Copy an m-row, n-column block (device to device, leading dimensions given).
+void copy_block(hipStream_t st, double* dst, int ldd, const double* src, int lds, int m, int n);
+
+}  // namespace dev
+}  // namespace titus

@@ -1,0 +1,385 @@
+// Small dense linear algebra on the host; see smallla.h.
+#include "smallla.h"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <stdexcept>
+
+namespace titus {
+
+HMat HMat::eye(int m, int n) {
+  HMat I(m, n);
+  for (int i = 0; i < std::min(m, n); ++i) I(i, i) = 1.0;
+  return I;
+}
+
+HMat matmul(const HMat& A, const HMat& B, bool ta, bool tb) {
+  const int m = ta ? A.n : A.m, k = ta ? A.m : A.n;
+  const int k2 = tb ? B.n : B.m, n = tb ? B.m : B.n;
+  if (k != k2) throw std::runtime_error("matmul: inner dimensions differ");
+  HMat C(m, n);
+  if (!ta && !tb) {
+    for (int j = 0; j < n; ++j) {
+      double* c = C.col(j);
+      for (int l = 0; l < k; ++l) {
+        const double b = B(l, j);
+        if (b == 0.0) continue;
+        const double* a = A.col(l);
+        for (int i = 0; i < m; ++i) c[i] += a[i] * b;
+      }
+    }
+  } else if (ta && !tb) {
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < m; ++i) {
+        const double* a = A.col(i);
+        const double* b = B.col(j);
+        double s = 0.0;
+        for (int l = 0; l < k; ++l) s += a[l] * b[l];
+        C(i, j) = s;
+      }
+  } else {
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < m; ++i) {
+        double s = 0.0;
+        for (int l = 0; l < k; ++l) s += (ta ? A(l, i) : A(i, l)) * (tb ? B(j, l) : B(l, j));
+        C(i, j) = s;
+      }
+  }
+  return C;
+}
+
+HMat transpose(const HMat& A) {
+  HMat T(A.n, A.m);
+  for (int j = 0; j < A.n; ++j)
+    for (int i = 0; i < A.m; ++i) T(j, i) = A(i, j);
+  return T;
+}
+
+HMat hcat(const HMat& A, const HMat& B) {
+  if (A.n == 0) return B;
+  if (B.n == 0) return A;
+  if (A.m != B.m) throw std::runtime_error("hcat: row counts differ");
+  HMat C(A.m, A.n + B.n);
+  std::copy(A.a.begin(), A.a.end(), C.a.begin());
+  std::copy(B.a.begin(), B.a.end(), C.a.begin() + A.a.size());
+  return C;
+}
+
+HMat cols(const HMat& A, int j0, int j1) {
+  HMat C(A.m, j1 - j0);
+  std::copy(A.a.begin() + (size_t)j0 * A.m, A.a.begin() + (size_t)j1 * A.m, C.a.begin());
+  return C;
+}
+
+double frob(const HMat& A) {
+  double s = 0.0;
+  for (double v : A.a) s += v * v;
+  return std::sqrt(s);
+}
+
+// ---------------------------------------------------------------------------
+// Householder QR of an m x n matrix, m >= n.  On return A holds R in its upper
+// triangle and the reflector vectors (unit leading entry implied) below.
+static void householder_qr(HMat& A, std::vector<double>& tau) {
+  const int m = A.m, n = A.n;
+  tau.assign(n, 0.0);
+  for (int j = 0; j < n; ++j) {
+    double* a = A.col(j);
+    double xnorm2 = 0.0;
+    for (int i = j + 1; i < m; ++i) xnorm2 += a[i] * a[i];
+    const double alpha = a[j];
+    if (xnorm2 == 0.0) { tau[j] = 0.0; continue; }
+    const double nrm = std::sqrt(alpha * alpha + xnorm2);
+    const double beta = alpha >= 0.0 ? -nrm : nrm;
+    tau[j] = (beta - alpha) / beta;
+    const double scal = 1.0 / (alpha - beta);
+    for (int i = j + 1; i < m; ++i) a[i] *= scal;
+    a[j] = beta;
+    // apply H = I - tau v v^T to the trailing columns
+    for (int k = j + 1; k < n; ++k) {
+      double* b = A.col(k);
+      double w = b[j];
+      for (int i = j + 1; i < m; ++i) w += a[i] * b[i];
+      w *= tau[j];
+      b[j] -= w;
+      for (int i = j + 1; i < m; ++i) b[i] -= w * a[i];
+    }
+  }
+}
+
+// Q * B for the reflectors stored in QR (m x n), B has m rows (in place).
+static void apply_q(const HMat& QR, const std::vector<double>& tau, HMat& B) {
+  const int m = QR.m, n = QR.n;
+  for (int j = n - 1; j >= 0; --j) {
+    if (tau[j] == 0.0) continue;
+    const double* v = QR.col(j);
+    for (int k = 0; k < B.n; ++k) {
+      double* b = B.col(k);
+      double w = b[j];
+      for (int i = j + 1; i < m; ++i) w += v[i] * b[i];
+      w *= tau[j];
+      b[j] -= w;
+      for (int i = j + 1; i < m; ++i) b[i] -= w * v[i];
+    }
+  }
+}
+
+// One-sided Jacobi on a square n x n matrix B (overwritten by U diag(s)).
+static void jacobi_square(HMat& B, HMat& V) {
+  const int n = B.n;
+  V = HMat::eye(n, n);
+  const double tol = 1e-15;
+  std::vector<double> nrm(n);
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    bool rotated = false;
+    for (int j = 0; j < n; ++j) {
+      const double* b = B.col(j);
+      double s = 0.0;
+      for (int i = 0; i < B.m; ++i) s += b[i] * b[i];
+      nrm[j] = s;
+    }
+    for (int p = 0; p < n - 1; ++p) {
+      for (int q = p + 1; q < n; ++q) {
+        double* bp = B.col(p);
+        double* bq = B.col(q);
+        const double alpha = nrm[p], beta = nrm[q];
+        if (alpha == 0.0 || beta == 0.0) continue;
+        double gamma = 0.0;
+        for (int i = 0; i < B.m; ++i) gamma += bp[i] * bq[i];
+        if (std::fabs(gamma) <= tol * std::sqrt(alpha * beta)) continue;
+        rotated = true;
+        const double zeta = (beta - alpha) / (2.0 * gamma);
+        const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+        const double c = 1.0 / std::sqrt(1.0 + t * t);
+        const double s = c * t;
+        for (int i = 0; i < B.m; ++i) {
+          const double x = bp[i], y = bq[i];
+          bp[i] = c * x - s * y;
+          bq[i] = s * x + c * y;
+        }
+        double* vp = V.col(p);
+        double* vq = V.col(q);
+        for (int i = 0; i < n; ++i) {
+          const double x = vp[i], y = vq[i];
+          vp[i] = c * x - s * y;
+          vq[i] = s * x + c * y;
+        }
+        double np = 0.0, nq = 0.0;
+        for (int i = 0; i < B.m; ++i) { np += bp[i] * bp[i]; nq += bq[i] * bq[i]; }
+        nrm[p] = np;
+        nrm[q] = nq;
+      }
+    }
+    if (!rotated) return;
+  }
+}
+
+void svd(const HMat& A0, HMat& U, std::vector<double>& s, HMat& V) {
+  if (A0.m < A0.n) {
+    HMat Ut, Vt;
+    svd(transpose(A0), Ut, s, Vt);
+    U = Vt;
+    V = Ut;
+    return;
+  }
+  const int m = A0.m, n = A0.n;
+  HMat QR = A0;
+  std::vector<double> tau;
+  householder_qr(QR, tau);
+  HMat R(n, n);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i <= j; ++i) R(i, j) = QR(i, j);
+  HMat Vr;
+  jacobi_square(R, Vr);
+  std::vector<double> sv(n);
+  for (int j = 0; j < n; ++j) {
+    const double* b = R.col(j);
+    double t = 0.0;
+    for (int i = 0; i < n; ++i) t += b[i] * b[i];
+    sv[j] = std::sqrt(t);
+  }
+  std::vector<int> ord(n);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return sv[a] > sv[b]; });
+  HMat Ur(m, n);   // padded to m rows for apply_q
+  V = HMat(n, n);
+  s.assign(n, 0.0);
+  for (int k = 0; k < n; ++k) {
+    const int j = ord[k];
+    s[k] = sv[j];
+    if (sv[j] > 0.0)
+      for (int i = 0; i < n; ++i) Ur(i, k) = R(i, j) / sv[j];
+    for (int i = 0; i < n; ++i) V(i, k) = Vr(i, j);
+  }
+  apply_q(QR, tau, Ur);
+  U = Ur;
+}
+
+bool cholesky_upper(const HMat& A, HMat& R) {
+  const int n = A.n;
+  R = HMat(n, n);
+  for (int j = 0; j < n; ++j) {
+    for (int i = 0; i <= j; ++i) {
+      double s = A(i, j);
+      for (int k = 0; k < i; ++k) s -= R(k, i) * R(k, j);
+      if (i == j) {
+        if (!(s > 0.0)) return false;
+        R(j, j) = std::sqrt(s);
+      } else {
+        R(i, j) = s / R(i, i);
+      }
+    }
+  }
+  return true;
+}
+
+HMat inv_upper(const HMat& R) {
+  const int n = R.n;
+  HMat X(n, n);
+  for (int j = 0; j < n; ++j) {
+    X(j, j) = 1.0 / R(j, j);
+    for (int i = j - 1; i >= 0; --i) {
+      double s = 0.0;
+      for (int k = i + 1; k <= j; ++k) s += R(i, k) * X(k, j);
+      X(i, j) = -s / R(i, i);
+    }
+  }
+  return X;
+}
+
+bool canonical_completion(const HMat& Bc, const std::vector<int>& cand_index, int need,
+                          bool greedy, std::vector<int>& chosen, HMat& Rinv) {
+  // Bc: ncand x q rows of the orthonormal block at the candidates.
+  // H(a, b) = [c_a == c_b] - Bc(a,:) . Bc(b,:).  Incremental Cholesky over the
+  // chosen candidates; L[s][t] = v_s[c_t].
+  const int nc = Bc.m, q = Bc.n;
+  chosen.clear();
+  std::vector<int> pos;            // positions (rows of Bc) of chosen candidates
+  std::vector<std::vector<double>> L;   // L[s][t] for all candidate rows t
+  std::vector<double> piv(nc);
+  auto H = [&](int a, int b) {
+    double s = (cand_index[a] == cand_index[b]) ? 1.0 : 0.0;
+    for (int k = 0; k < q; ++k) s -= Bc(a, k) * Bc(b, k);
+    return s;
+  };
+  auto pivot = [&](int t) {
+    double p = H(t, t);
+    for (size_t s = 0; s < L.size(); ++s) p -= L[s][t] * L[s][t];
+    return p;
+  };
+  auto accept = [&](int t, double p) {
+    const double nrm = std::sqrt(p);
+    std::vector<double> row(nc, 0.0);
+    for (int u = 0; u < nc; ++u) {
+      double h = H(t, u);
+      for (size_t s = 0; s < L.size(); ++s) h -= L[s][t] * L[s][u];
+      row[u] = h / nrm;
+    }
+    L.push_back(std::move(row));
+    pos.push_back(t);
+    chosen.push_back(cand_index[t]);
+  };
+  std::vector<char> used(nc, 0);
+  if (greedy) {
+    while ((int)chosen.size() < need) {
+      int best = -1;
+      double bp = 1e-8;
+      for (int t = 0; t < nc; ++t) {
+        if (used[t]) continue;
+        const double p = pivot(t);
+        if (p > bp) { bp = p; best = t; }
+      }
+      if (best < 0) return false;
+      used[best] = 1;
+      accept(best, bp);
+    }
+  } else {
+    for (int t = 0; t < nc && (int)chosen.size() < need; ++t) {
+      const double p = pivot(t);
+      if (p > 0.5) accept(t, p);
+    }
+    if ((int)chosen.size() < need) return false;
+  }
+  const int k = (int)chosen.size();
+  HMat R(k, k);
+  for (int s = 0; s < k; ++s)
+    for (int s2 = s; s2 < k; ++s2) R(s, s2) = L[s][pos[s2]];
+  Rinv = inv_upper(R);
+  return true;
+}
+
+HMat range_basis(const HMat& A, int ncols, const HMat* against, double scale, double tol,
+                 bool greedy, const std::vector<int>& candidates, int* rank_out) {
+  const int m = A.m;
+  HMat Ap = A;
+  if (against && against->n > 0) {
+    for (int pass = 0; pass < 2; ++pass) {
+      HMat P = matmul(*against, Ap, true, false);
+      HMat G = matmul(*against, P);
+      for (size_t i = 0; i < Ap.a.size(); ++i) Ap.a[i] -= G.a[i];
+    }
+  }
+  HMat U, V;
+  std::vector<double> sv;
+  int p = 0;
+  if (Ap.n > 0) {
+    svd(Ap, U, sv, V);
+    if (scale < 0.0) scale = sv.empty() ? 0.0 : sv[0];
+    if (scale > 0.0)
+      for (double x : sv) if (x > tol * scale) ++p;
+  }
+  p = std::min(p, ncols);
+  HMat B = p > 0 ? cols(U, 0, p) : HMat(m, 0);
+  if (rank_out) *rank_out = p;
+  const int need = ncols - p;
+  if (need == 0) return B;
+  HMat base = (against && against->n > 0) ? hcat(*against, B) : B;
+  std::vector<int> cand = candidates;
+  if (cand.empty()) {
+    cand.resize(m);
+    std::iota(cand.begin(), cand.end(), 0);
+  }
+  HMat Bc((int)cand.size(), base.n);
+  for (size_t t = 0; t < cand.size(); ++t)
+    for (int k = 0; k < base.n; ++k) Bc((int)t, k) = base(cand[t], k);
+  std::vector<int> chosen;
+  HMat Rinv;
+  if (!canonical_completion(Bc, cand, need, greedy, chosen, Rinv))
+    throw std::runtime_error("canonical completion ran out of candidates");
+  // Y = E_chosen - base * Bc_chosen^T, completion = Y * Rinv
+  HMat Y(m, need);
+  for (int s = 0; s < need; ++s) {
+    const int c = chosen[s];
+    for (int i = 0; i < m; ++i) {
+      double v = 0.0;
+      for (int k = 0; k < base.n; ++k) v += base(i, k) * base(c, k);
+      Y(i, s) = (i == c ? 1.0 : 0.0) - v;
+    }
+  }
+  return hcat(B, matmul(Y, Rinv));
+}
+
+void svd_completed(const HMat& S, double tol, HMat& U, std::vector<double>& d, HMat& V) {
+  const int n = S.n;
+  HMat Us, Vs;
+  std::vector<double> sv;
+  svd(S, Us, sv, Vs);
+  int p = 0;
+  if (!sv.empty() && sv[0] > 0.0)
+    for (double x : sv) if (x > tol * sv[0]) ++p;
+  HMat Up = cols(Us, 0, p), Vp = cols(Vs, 0, p);
+  std::vector<int> none;
+  HMat Ec(n, 0);
+  if (p < n) {
+    U = hcat(Up, range_basis(Ec, n - p, &Up, 1.0, tol, true, none));
+    V = hcat(Vp, range_basis(Ec, n - p, &Vp, 1.0, tol, true, none));
+  } else {
+    U = Up;
+    V = Vp;
+  }
+  d.assign(n, 0.0);
+  for (int i = 0; i < p; ++i) d[i] = sv[i];
+}
+
+}  // namespace titus

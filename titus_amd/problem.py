@@ -1,0 +1,90 @@
+"""Assemble the operands of the DLRA energy loop (the ``SolverCSD`` constructor).
+
+``build_problem(settings)`` restates ``SolverCSD(settings)`` (``code/SolverCSD.jl:63-146``)
+and the set-up part of the ``Solve*`` functions (ordinate reduction, initial
+factors, ``SolverCSD.jl:904-961``) with T = Float64.  The result is a plain
+dict of numpy arrays: exactly what crosses the C-ABI (``titus_problem_v1``).
+
+Layouts (all FP64, 0-based):
+
+* cells c = i*ny + j; ``density`` has length C = nx*ny;
+* matrices are column-major as in Julia: ``X`` is C x r with each basis vector
+  contiguous, ``W`` is Nm x r, ``S`` r x r; numpy arrays here are kept in
+  Fortran order so that their buffers are already in that layout;
+* the uncollided flux ``psi`` is stored C x Q (cell-major within each ordinate
+  column), i.e. ``Mat2Vec(psi)`` of ``utils.jl:67-78``.
+"""
+import numpy as np
+
+from . import physics as ph
+
+INTEGRATOR_FIXED = 0      # SolveFirstCollisionSourceDLR
+INTEGRATOR_BUG = 1        # SolveFirstCollisionSourceDLRBUGRejection (rejection off)
+
+
+def build_problem(s, moments="subset", integrator=INTEGRATOR_FIXED):
+    nx, ny = s.NCellsX, s.NCellsY
+    C = nx * ny
+    N = s.nPN
+    csd = ph.CSDGrid(s)
+    lk, full_idx, degree = ph.moment_set(N, moments)
+    Axf, _, Azf = ph.pn_matrices_full(N)
+    Ax = Axf[np.ix_(full_idx, full_idx)]
+    Az = Azf[np.ix_(full_idx, full_idx)]
+    absAx = ph.abs_matrix(Ax)
+    absAz = ph.abs_matrix(Az)
+    O, M, pts, w = ph.trafo_matrices(N, lk, cs_phase=s.csPhase)
+
+    # ordinate reduction: keep directions the beam reaches (SolverCSD.jl:913-925)
+    psi_beam = ph.beam_angular(s, pts)
+    idx = np.nonzero(psi_beam > s.floorPsi * psi_beam.max())[0]
+    omega = pts[idx]
+    Mred = M[:, idx]
+
+    # boundary source: separable into a per-(cell, ordinate) base and a per-step
+    # scale (SolverCSD.jl:977-988, PsiBeam :240-276)
+    bidx = ph.boundary_index(nx, ny)
+    bcells = np.unique(bidx)
+    bi, bj = bcells // ny, bcells % ny
+    spatial = ph.beam_spatial(s, s.xMid[bi], s.yMid[bj])
+    bc_base = np.asfortranarray(spatial[:, None] * ph.beam_angular(s, omega)[None, :])
+    bc_scale = np.array([ph.beam_energy_factor(s, csd.eGrid[n], csd.S[n])
+                         for n in range(csd.nEnergies)])
+
+    interior = np.ones(C, dtype=bool)
+    interior[bcells] = False
+
+    if integrator == INTEGRATOR_BUG:
+        r0 = s.r // 2                         # SolverCSD.jl:2013
+    else:
+        r0 = s.r
+    if r0 < 1:
+        raise ValueError("rank must be >= 1")
+    if r0 > C or r0 > len(lk):
+        raise ValueError("rank exceeds problem dimensions")
+
+    # svd!(zeros) -> leading identity columns; ghost rows of X zeroed (Q6)
+    X0 = np.zeros((C, r0), order="F")
+    X0[np.arange(r0), np.arange(r0)] = 1.0
+    X0[bcells, :] = 0.0
+    W0 = np.zeros((len(lk), r0), order="F")
+    W0[np.arange(r0), np.arange(r0)] = 1.0
+    S0 = np.zeros((r0, r0), order="F")
+
+    return dict(
+        nx=nx, ny=ny, C=C, dx=float(s.dx), dy=float(s.dy),
+        density=np.ascontiguousarray(s.density.reshape(C)),
+        N=N, Nm=len(lk), degree=degree.astype(np.int32), lk=lk,
+        Ax=np.asfortranarray(Ax), Az=np.asfortranarray(Az),
+        absAx=np.asfortranarray(absAx), absAz=np.asfortranarray(absAz),
+        Q=len(idx), Qfull=len(w), omega=np.asfortranarray(omega),
+        M=np.asfortranarray(Mred), O=np.asfortranarray(O), ordinate_index=idx,
+        nE=csd.nEnergies, dE=csd.dE, eGrid=csd.eGrid, Sst=csd.S,
+        sigma=np.ascontiguousarray(csd.sigma_table()),
+        bcells=bcells.astype(np.int32), bc_base=bc_base, bc_scale=bc_scale,
+        restore_cells=ph.boundary_restore_cells(nx, ny).astype(np.int32),
+        interior=np.nonzero(interior)[0].astype(np.int32),
+        integrator=integrator, r=r0, rank_cap=int(s.rank_cap),
+        theta=float(s.theta), thetaIndex=float(s.thetaIndex), cEta=float(s.cEta),
+        X0=X0, W0=W0, S0=S0,
+    )

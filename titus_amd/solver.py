@@ -1,0 +1,102 @@
+"""Host mirror of the reference's solver interface, backed by the HIP library.
+
+Reference interface (Julia, ``code/SolverCSD.jl``):
+
+* ``SolverCSD(settings)``                              -- ``:63-146``
+* ``SolveFirstCollisionSourceDLR(obj)``                 -- ``:891-1120``
+  returns ``(X*U, 0.5*sqrt(gamma[1])*Sigma, O*W*V, W*V, dose, psi)``
+* ``SolveFirstCollisionSourceDLRBUGRejection(obj, recordGif=false)`` -- ``:2011-2374``
+  returns the above plus ``[E; rank]``, ``[E eta]``, ``[E bound]`` before ``psi``
+
+Like the reference, ``Solve*`` mutates ``obj``: ``obj.dose`` accumulates and
+``obj.settings.dE`` is overwritten with the grid step.  The energy loop runs
+entirely in ``libtitus_hip.so`` on the GPU; this module only assembles the
+operands (the work the reference's constructor does) and unpacks the results.
+"""
+import math
+
+import numpy as np
+
+from . import _lib
+from .problem import INTEGRATOR_BUG, INTEGRATOR_FIXED, build_problem
+
+
+class SolverCSD:
+    """Operands of the CSD transport solve (``SolverCSD.jl:20-147``), FP64.
+
+    ``precision``: "fp64" (plain form).  ``moments``: "subset" (k >= 0 real
+    harmonics, the 2D-exact reduction) or "full" ((N+1)^2 as the reference).
+    """
+
+    def __init__(self, settings, device=0, precision="fp64", moments="subset", rank_tol=1e-7):
+        if precision != "fp64":
+            raise ValueError("only the FP64 form is available")
+        self.settings = settings
+        self.device = device
+        self.moments = moments
+        self.rank_tol = rank_tol
+        nx, ny = settings.NCellsX, settings.NCellsY
+        self.dose = np.zeros(nx * ny)
+        self.gamma = np.array([2.0 / (2 * n + 1) for n in range(settings.nPN + 1)])
+        self._problem = {}
+
+    def problem(self, integrator):
+        if integrator not in self._problem:
+            self._problem[integrator] = build_problem(self.settings, self.moments, integrator)
+        return self._problem[integrator]
+
+    @property
+    def qReduced(self):
+        return self.problem(INTEGRATOR_FIXED)["omega"]
+
+    @property
+    def MReduced(self):
+        return self.problem(INTEGRATOR_FIXED)["M"]
+
+    def context(self, integrator):
+        """A fresh device context for ``integrator`` (steps run through it)."""
+        return _lib.Context(self.problem(integrator), integrator, device=self.device,
+                            rank_tol=self.rank_tol)
+
+
+def _finish(obj, P, st):
+    S = st["S"]
+    U, sig, Vt = np.linalg.svd(S)
+    V = Vt.T
+    X, W = st["X"], st["W"]
+    obj.dose += st["dose"]
+    obj.settings.dE = P["dE"]
+    nx, ny = obj.settings.NCellsX, obj.settings.NCellsY
+    psi = st["psi"].reshape(nx, ny, -1)              # Vec2Mat per ordinate
+    return X @ U, 0.5 * math.sqrt(obj.gamma[0]) * sig, P["O"] @ (W @ V), W @ V, obj.dose, psi
+
+
+def SolveFirstCollisionSourceDLR(obj):
+    """Fixed-rank first-collision-source DLRA solve on the GPU."""
+    P = obj.problem(INTEGRATOR_FIXED)
+    ctx = obj.context(INTEGRATOR_FIXED)
+    try:
+        ctx.solve(0)
+        st = ctx.state()
+    finally:
+        ctx.close()
+    return _finish(obj, P, st)
+
+
+def SolveFirstCollisionSourceDLRBUGRejection(obj, recordGif=False):
+    """Rank-adaptive augmented-BUG solve on the GPU (rejection branches off)."""
+    P = obj.problem(INTEGRATOR_BUG)
+    ctx = obj.context(INTEGRATOR_BUG)
+    try:
+        ctx.solve(0)
+        st = ctx.state()
+        done, _ = ctx.progress()
+        ranks, etas = ctx.history(done)
+    finally:
+        ctx.close()
+    out = _finish(obj, P, st)
+    E = P["eGrid"][1:done + 1]
+    rank_hist = np.vstack([E, ranks])
+    eta_E = P["eGrid"][2:done + 1]                    # eta exists from step 2 on
+    eta_hist = np.column_stack([eta_E, etas[:len(eta_E)]]) if done > 1 else np.zeros((0, 2))
+    return (*out[:5], rank_hist, eta_hist, np.zeros((0, 2)), out[5])
