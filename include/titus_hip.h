@@ -117,12 +117,13 @@ int titus_get_state(struct titus_ctx* ctx, double* dose, double* X, double* S, d
  * streaming eta indicator (SolverCSD.jl:2220); n entries each at most. */
 int titus_get_history(const struct titus_ctx* ctx, int32_t* ranks, double* etas, int32_t n);
 
-/* Device time of the library's kernels since the last reset, measured with
- * HIP events on the context's stream: total milliseconds and launch count for
- * kernel `name` ("sweep", "stream_K", "galerkin", "gram", "gemm", ...). */
+/* Device time of the library's kernels since timing was (re)enabled, measured
+ * with HIP events on the context's stream: total milliseconds, launch count,
+ * and the algorithmic HBM bytes and flops of those launches, for kernel
+ * `name` ("bc", "dose", "sweep", "stream_K", "galerkin", "gram", "gemm"). */
 int titus_set_timing(struct titus_ctx* ctx, int32_t enabled);
-int titus_kernel_time(const struct titus_ctx* ctx, const char* name, double* total_ms,
-                      int64_t* launches);
+int titus_kernel_stats(const struct titus_ctx* ctx, const char* name, double* total_ms,
+                       int64_t* launches, double* bytes, double* flops);
 
 void titus_destroy(struct titus_ctx* ctx);
 const char* titus_last_error(void);

@@ -111,7 +111,23 @@ def range_basis(A, ncols, candidates, against=None, scale=None, tol=None, greedy
     cols = [B]
     need = ncols - p
     cand = list(candidates)
-    if greedy:
+    if not greedy:
+        for c in cand:
+            if need == 0:
+                break
+            v = np.zeros(m)
+            v[c] = 1.0
+            for _ in range(2):
+                v = v - base @ (base.T @ v)
+            nv = np.linalg.norm(v)
+            if nv * nv > 0.5:
+                v = (v / nv)[:, None]
+                cols.append(v)
+                base = np.hstack([base, v])
+                need -= 1
+    if need:
+        # greedy selection (and the fallback of the in-order rule when the
+        # candidates run out)
         while need:
             R = np.eye(m)[:, cand]
             for _ in range(2):
@@ -121,19 +137,6 @@ def range_basis(A, ncols, candidates, against=None, scale=None, tol=None, greedy
             if nr[t] <= 1e-8:
                 break
             v = (R[:, t] / np.sqrt(nr[t]))[:, None]
-            cols.append(v)
-            base = np.hstack([base, v])
-            need -= 1
-    for c in ([] if greedy else cand):
-        if need == 0:
-            break
-        v = np.zeros(m)
-        v[c] = 1.0
-        for _ in range(2):
-            v = v - base @ (base.T @ v)
-        nv = np.linalg.norm(v)
-        if nv * nv > 0.5:
-            v = (v / nv)[:, None]
             cols.append(v)
             base = np.hstack([base, v])
             need -= 1

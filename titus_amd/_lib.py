@@ -25,7 +25,7 @@ ERRORS = {
 
 # every symbol include/titus_hip.h declares
 EXPORTED = ("titus_create", "titus_step", "titus_solve", "titus_progress", "titus_get_state",
-            "titus_get_history", "titus_set_timing", "titus_kernel_time", "titus_destroy",
+            "titus_get_history", "titus_set_timing", "titus_kernel_stats", "titus_destroy",
             "titus_last_error", "titus_abi_version")
 
 
@@ -74,14 +74,15 @@ def load():
     lib.titus_get_state.argtypes = [vp, _P, _P, _P, _P, _P]
     lib.titus_get_history.argtypes = [vp, _PI, _P, ctypes.c_int32]
     lib.titus_set_timing.argtypes = [vp, ctypes.c_int32]
-    lib.titus_kernel_time.argtypes = [vp, ctypes.c_char_p, _P, ctypes.POINTER(ctypes.c_int64)]
+    lib.titus_kernel_stats.argtypes = [vp, ctypes.c_char_p, _P, ctypes.POINTER(ctypes.c_int64),
+                                       _P, _P]
     lib.titus_destroy.argtypes = [vp]
     lib.titus_destroy.restype = None
     lib.titus_last_error.restype = ctypes.c_char_p
     lib.titus_abi_version.restype = ctypes.c_int32
     for name in ("titus_create", "titus_step", "titus_solve", "titus_progress",
                  "titus_get_state", "titus_get_history", "titus_set_timing",
-                 "titus_kernel_time"):
+                 "titus_kernel_stats"):
         getattr(lib, name).restype = ctypes.c_int
     if lib.titus_abi_version() != ABI_VERSION:
         raise RuntimeError("libtitus_hip.so ABI version mismatch")
@@ -187,10 +188,15 @@ class Context:
     def set_timing(self, on):
         check(self._lib.titus_set_timing(self.h, 1 if on else 0))
 
-    def kernel_time(self, name):
+    KERNELS = ("bc", "dose", "sweep", "stream_K", "galerkin", "gram", "gemm")
+
+    def kernel_stats(self, name):
+        """(total ms, launches, algorithmic bytes, flops) since timing was enabled."""
         ms, n = ctypes.c_double(), ctypes.c_int64()
-        check(self._lib.titus_kernel_time(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
-        return ms.value, n.value
+        by, fl = ctypes.c_double(), ctypes.c_double()
+        check(self._lib.titus_kernel_stats(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n),
+                                           ctypes.byref(by), ctypes.byref(fl)))
+        return ms.value, n.value, by.value, fl.value
 
     def close(self):
         if getattr(self, "h", None):

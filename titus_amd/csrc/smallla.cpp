@@ -281,7 +281,15 @@ bool canonical_completion(const HMat& Bc, const std::vector<int>& cand_index, in
     chosen.push_back(cand_index[t]);
   };
   std::vector<char> used(nc, 0);
-  if (greedy) {
+  if (!greedy) {
+    for (int t = 0; t < nc && (int)chosen.size() < need; ++t) {
+      const double p = pivot(t);
+      if (p > 0.5) { used[t] = 1; accept(t, p); }
+    }
+  }
+  // greedy selection; also the fallback of the in-order rule when the
+  // candidates run out
+  {
     while ((int)chosen.size() < need) {
       int best = -1;
       double bp = 1e-8;
@@ -294,12 +302,6 @@ bool canonical_completion(const HMat& Bc, const std::vector<int>& cand_index, in
       used[best] = 1;
       accept(best, bp);
     }
-  } else {
-    for (int t = 0; t < nc && (int)chosen.size() < need; ++t) {
-      const double p = pivot(t);
-      if (p > 0.5) accept(t, p);
-    }
-    if ((int)chosen.size() < need) return false;
   }
   const int k = (int)chosen.size();
   HMat R(k, k);

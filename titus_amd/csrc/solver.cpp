@@ -61,12 +61,12 @@ T* dalloc(size_t n) {
 // Kernel timing with HIP events on the context's stream.
 struct Timing {
   bool enabled = false;
-  struct Rec { int id; hipEvent_t a, b; };
+  struct Rec { int id; hipEvent_t a, b; double bytes, flops; };
   std::vector<Rec> pending;
   std::vector<hipEvent_t> pool;
   std::map<std::string, int> ids;
   std::vector<std::string> names;
-  std::vector<double> total_ms;
+  std::vector<double> total_ms, total_bytes, total_flops;
   std::vector<int64_t> launches;
 
   hipEvent_t ev() {
@@ -80,6 +80,7 @@ struct Timing {
     if (it != ids.end()) return it->second;
     int k = (int)names.size();
     ids[n] = k; names.push_back(n); total_ms.push_back(0.0); launches.push_back(0);
+    total_bytes.push_back(0.0); total_flops.push_back(0.0);
     return k;
   }
   void resolve() {
@@ -89,6 +90,8 @@ struct Timing {
       HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
       total_ms[r.id] += ms;
       launches[r.id] += 1;
+      total_bytes[r.id] += r.bytes;
+      total_flops[r.id] += r.flops;
       pool.push_back(r.a); pool.push_back(r.b);
     }
     pending.clear();
@@ -136,15 +139,17 @@ struct titus_ctx {
   }
 
   // ---- timing -------------------------------------------------------------
+  // `bytes` / `flops`: algorithmic HBM traffic and arithmetic of the launch
+  // (compulsory operand reads and result writes; reduction partials excluded)
   template <class F>
-  void timed(const char* name, F&& f) {
+  void timed(const char* name, double bytes, double flops, F&& f) {
     if (!timing.enabled) { f(); return; }
     const int id = timing.id(name);
     hipEvent_t a = timing.ev(), b = timing.ev();
     HIPCHK(hipEventRecord(a, st));
     f();
     HIPCHK(hipEventRecord(b, st));
-    timing.pending.push_back({id, a, b});
+    timing.pending.push_back({id, a, b, bytes, flops});
   }
 
   // ---- transfers ------------------------------------------------------------
@@ -179,7 +184,8 @@ struct titus_ctx {
   // arena[off : off + m*n] = A^T op(B)
   void tn(const double* A, int m, const double* B, int n, int op, size_t off) {
     if (off + (size_t)m * n > arena_n) throw TitusError(TITUS_E_ARG, "arena too small");
-    timed(op == dev::OP_NONE ? "gram" : "galerkin", [&] {
+    const double by = 8.0 * C * (A == B ? m : m + n) + (op == dev::OP_NONE ? 0.0 : 8.0 * C);
+    timed(op == dev::OP_NONE ? "gram" : "galerkin", by, 2.0 * C * m * n, [&] {
       dev::launch_tn(st, g, A, C, m, B, C, n, op, partial, arena + off);
     });
   }
@@ -199,7 +205,7 @@ struct titus_ctx {
   void gemm(const double* A, int k, const HMat& T, double alpha, double beta, double* out,
             const unsigned char* mask, size_t upl_off = 0) {
     const double* Td = upload(T, upl_off);
-    timed("gemm", [&] {
+    timed("gemm", 8.0 * C * (k + T.n * (beta != 0.0 ? 2 : 1)), 2.0 * C * k * T.n, [&] {
       dev::launch_gemm(st, C, A, C, k, Td, T.n, alpha, beta, out, C, mask);
     });
   }
@@ -267,8 +273,10 @@ struct titus_ctx {
       scale = std::sqrt(tr);                              // |K|_F
       for (int pass = 0; pass < 2; ++pass) {
         if ((size_t)q * k > arena_n) throw TitusError(TITUS_E_ARG, "arena too small");
-        timed("gram", [&] { dev::launch_tn(st, g, Xa, C, q, Kw, C, k, dev::OP_NONE, partial, arena); });
-        timed("gemm", [&] { dev::launch_gemm(st, C, Xa, C, q, arena, k, -1.0, 1.0, Kw, C, nullptr); });
+        timed("gram", 8.0 * C * (q + k), 2.0 * C * q * k,
+              [&] { dev::launch_tn(st, g, Xa, C, q, Kw, C, k, dev::OP_NONE, partial, arena); });
+        timed("gemm", 8.0 * C * (q + 2 * k), 2.0 * C * q * k,
+              [&] { dev::launch_gemm(st, C, Xa, C, q, arena, k, -1.0, 1.0, Kw, C, nullptr); });
       }
     }
     // shifted CholeskyQR: Q1 = K R1^-1, Q2 = Q1 R2^-1, R3 = chol(Q2^T Q2)
@@ -388,16 +396,16 @@ struct titus_ctx {
       for (int b = 0; b < rank; ++b) v[a] += S(a, b) * W(0, b);
     const double* vd = upload_vec(v, 0);
     const double coef = 0.5 * dE * Sn;
-    timed("dose", [&] {
+    timed("dose", 8.0 * C * (rank + nq + 3), 2.0 * C * (rank + nq), [&] {
       dev::launch_dose(st, g, dose, X, C, rank, vd, psi, nq, m0, coef);
     });
   }
   void begin_step(int s, std::vector<double>& Dvec, std::vector<double>& Dout) {
     const std::vector<double> sg = sig(s);
-    timed("bc", [&] { dev::launch_bc(st, psi, C, nq, bcells, nb, bc_base, bc_scale[s - 1]); });
+    timed("bc", 16.0 * nb * nq + 4.0 * nb, 1.0 * nb * nq, [&] { dev::launch_bc(st, psi, C, nq, bcells, nb, bc_base, bc_scale[s - 1]); });
     dose_half(stopping[s - 1]);
     const double denom = 1.0 + dE * sg[0];
-    timed("sweep", [&] {
+    timed("sweep", 16.0 * C * nq + 9.0 * C, 12.0 * C * nq, [&] {
       dev::launch_sweep(st, g, psi, psi2, nq, omega, dE, denom, restore);
     });
     std::swap(psi, psi2);
@@ -438,7 +446,8 @@ struct titus_ctx {
       // K-step (SolverCSD.jl:1019-1031)
       std::vector<HMat> WAW = galerkin_w(W, W);
       const double* Zd = upload(stream_Z(S, WAW));
-      timed("stream_K", [&] { dev::launch_stream_k(st, g, X, C, r, Zd, r, dE, K, C); });
+      timed("stream_K", 8.0 * C * (2 * r + 1), 12.0 * C * r * r,
+            [&] { dev::launch_stream_k(st, g, X, C, r, Zd, r, dE, K, C); });
       range_basis_dev(K, r, nullptr, 0, r, Xn);
       HMat MUp = tn_fetch(Xn, r, X, r);
       // L-step (:1033-1046)
@@ -520,7 +529,8 @@ struct titus_ctx {
       std::vector<HMat> XLX = galerkin_x(X, r, X, r);
       std::vector<HMat> WAW = galerkin_w(W, W);
       const double* Zd = upload(stream_Z(S, WAW));
-      timed("stream_K", [&] { dev::launch_stream_k(st, g, X, C, r, Zd, r, dE, K, C); });
+      timed("stream_K", 8.0 * C * (2 * r + 1), 12.0 * C * r * r,
+            [&] { dev::launch_stream_k(st, g, X, C, r, Zd, r, dE, K, C); });
       augment_x();                                            // Xn = X1 (C x 2r)
       HMat Mu = tn_fetch(Xn, 2 * r, X, r);
       HMat W1 = augment_w(stream_L(titus::matmul(W, S, false, true), XLX));
@@ -805,11 +815,14 @@ int titus_set_timing(titus_ctx* ctx, int32_t enabled) {
     ctx->timing.resolve();
     for (auto& t : ctx->timing.total_ms) t = 0.0;
     for (auto& l : ctx->timing.launches) l = 0;
+    for (auto& b : ctx->timing.total_bytes) b = 0.0;
+    for (auto& f : ctx->timing.total_flops) f = 0.0;
     ctx->timing.enabled = enabled != 0;
   });
 }
 
-int titus_kernel_time(const titus_ctx* cctx, const char* name, double* total_ms, int64_t* launches) {
+int titus_kernel_stats(const titus_ctx* cctx, const char* name, double* total_ms,
+                       int64_t* launches, double* bytes, double* flops) {
   return guarded([&] {
     require(cctx != nullptr && name != nullptr, "bad argument");
     titus_ctx* ctx = const_cast<titus_ctx*>(cctx);
@@ -817,7 +830,10 @@ int titus_kernel_time(const titus_ctx* cctx, const char* name, double* total_ms,
     ctx->timing.resolve();
     auto it = ctx->timing.ids.find(name);
     if (total_ms) *total_ms = it == ctx->timing.ids.end() ? 0.0 : ctx->timing.total_ms[it->second];
-    if (launches) *launches = it == ctx->timing.ids.end() ? 0 : ctx->timing.launches[it->second];
+    const bool found = it != ctx->timing.ids.end();
+    if (launches) *launches = found ? ctx->timing.launches[it->second] : 0;
+    if (bytes) *bytes = found ? ctx->timing.total_bytes[it->second] : 0.0;
+    if (flops) *flops = found ? ctx->timing.total_flops[it->second] : 0.0;
   });
 }
 
