@@ -1,0 +1,72 @@
+"""The C-ABI library loads without a GPU and exports what include/titus_hip.h
+declares; the ctypes mirror of titus_problem_v1 matches the header."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from titus_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "titus_hip.h")
+
+
+def header_functions():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(titus_[a-z_0-9]+)\s*\(", txt)))
+
+
+def header_struct_fields():
+    txt = open(HEADER).read()
+    body = re.search(r"typedef struct titus_problem_v1 \{(.*?)\} titus_problem_v1;", txt, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        m = re.match(r"(const\s+)?(int32_t|double)\s*(\*?)\s*(.*)", decl)
+        assert m, decl
+        typ = m.group(2) + ("*" if m.group(3) else "")
+        for name in m.group(4).split(","):
+            name = name.strip()
+            if name.startswith("*"):
+                typ, name = m.group(2) + "*", name[1:].strip()
+            fields.append((name, typ))
+    return fields
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    declared = header_functions()
+    assert set(declared) == set(_lib.EXPORTED)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert lib.titus_abi_version() == _lib.ABI_VERSION
+
+
+def test_problem_struct_mirror_matches_header():
+    ct = {"int32_t": ctypes.c_int32, "double": ctypes.c_double,
+          "int32_t*": ctypes.POINTER(ctypes.c_int32), "double*": ctypes.POINTER(ctypes.c_double)}
+    fields = header_struct_fields()
+    assert [n for n, _ in fields] == [n for n, _ in _lib.ProblemV1._fields_]
+    for (n, t), (n2, t2) in zip(fields, _lib.ProblemV1._fields_):
+        assert ct[t] == t2, (n, t, t2)
+
+
+def test_create_reports_missing_device_or_bad_args_without_compute():
+    import torch
+    import cases
+    P = cases.small_fixed()
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present: covered by the gpu tests")
+    with pytest.raises(_lib.TitusError) as e:
+        _lib.Context(P, _lib.INTEGRATOR_FIXED)
+    assert e.value.code in (-6, -2)
+    # argument validation runs before any device call
+    st, keep = _lib.make_problem(P, _lib.INTEGRATOR_FIXED)
+    st.version = 99
+    h = ctypes.c_void_p()
+    assert _lib.load().titus_create(ctypes.byref(h), ctypes.byref(st), 0) == -1
+    assert b"version" in _lib.load().titus_last_error()

@@ -1,0 +1,117 @@
+"""Host dense LA of the library (csrc/smallla.cpp) against numpy and the oracle's
+basis rules (host build, no GPU)."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from oracle import dlra
+
+LIB = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "titus_amd",
+                   "_smallla_host.so")
+P = ctypes.POINTER(ctypes.c_double)
+PI = ctypes.POINTER(ctypes.c_int)
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        pytest.skip("host LA library not built")
+    L = ctypes.CDLL(LIB)
+    L.smallla_svd.argtypes = [ctypes.c_int, ctypes.c_int, P, P, P, P]
+    L.smallla_range_basis.argtypes = [ctypes.c_int, ctypes.c_int, P, ctypes.c_int, ctypes.c_int, P,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                      PI, P, PI]
+    L.smallla_svd_completed.argtypes = [ctypes.c_int, P, ctypes.c_double, P, P, P]
+    return L
+
+
+def f(a):
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+def dp(a):
+    return a.ctypes.data_as(P)
+
+
+def csvd(lib, A):
+    m, n = A.shape
+    k = min(m, n)
+    A = f(A)
+    U, s, V = f(np.zeros((m, k))), np.zeros(k), f(np.zeros((n, k)))
+    assert lib.smallla_svd(m, n, dp(A), dp(U), dp(s), dp(V)) == 0
+    return U, s, V
+
+
+@pytest.mark.parametrize("shape", [(6, 6), (40, 7), (253, 18), (5, 9)])
+def test_svd_matches_numpy(lib, shape):
+    rng = np.random.default_rng(sum(shape))
+    A = rng.standard_normal(shape) * np.logspace(0, -8, shape[1])[None, :]
+    U, s, V = csvd(lib, A)
+    s_ref = np.linalg.svd(A, compute_uv=False)
+    np.testing.assert_allclose(s, s_ref, rtol=1e-12, atol=1e-15 * s_ref[0])
+    np.testing.assert_allclose(U @ np.diag(s) @ V.T, A, atol=1e-13 * s_ref[0])
+    np.testing.assert_allclose(V.T @ V, np.eye(V.shape[1]), atol=1e-13)
+
+
+def crange(lib, A, ncols, against=None, scale=-1.0, greedy=False, cand=None, tol=1e-7):
+    m, k = A.shape
+    q = 0 if against is None else against.shape[1]
+    out = f(np.zeros((m, ncols)))
+    rank = ctypes.c_int()
+    Ag = f(against) if q else f(np.zeros((m, 1)))
+    c = np.ascontiguousarray(cand if cand is not None else [], dtype=np.int32)
+    assert lib.smallla_range_basis(m, k, dp(f(A)), ncols, q, dp(Ag), scale, tol, int(greedy),
+                                   len(c), c.ctypes.data_as(PI) if cand is not None else None,
+                                   dp(out), ctypes.byref(rank)) == 0
+    return out, rank.value
+
+
+@pytest.mark.parametrize("greedy", [False, True])
+def test_range_basis_same_rule_as_oracle(lib, greedy):
+    rng = np.random.default_rng(11)
+    m, k, ncols = 60, 6, 6
+    A = rng.standard_normal((m, 3)) @ rng.standard_normal((3, k))      # rank 3
+    A[:5] = 0.0
+    cand = list(range(5, m))
+    B, p = crange(lib, A, ncols, greedy=greedy, cand=cand)
+    Bo = dlra.range_basis(A, ncols, cand, greedy=greedy)
+    assert p == 3
+    np.testing.assert_allclose(B.T @ B, np.eye(ncols), atol=1e-13)
+    # same range part (up to rotation) and the same completion columns
+    np.testing.assert_allclose(B[:, :3] @ B[:, :3].T, Bo[:, :3] @ Bo[:, :3].T, atol=1e-12)
+    np.testing.assert_allclose(B[:, 3:], Bo[:, 3:], atol=1e-12)
+
+
+def test_range_basis_against_block(lib):
+    rng = np.random.default_rng(12)
+    m, r = 40, 4
+    X, _ = np.linalg.qr(rng.standard_normal((m, r)))
+    K = X @ rng.standard_normal((r, r)) + rng.standard_normal((m, 2)) @ rng.standard_normal((2, r))
+    scale = np.linalg.norm(K)
+    B, p = crange(lib, K, r, against=X, scale=scale, greedy=True)
+    Bo = dlra.range_basis(K, r, range(m), against=X, scale=scale, greedy=True)
+    assert p == 2
+    np.testing.assert_allclose(X.T @ B, 0, atol=1e-13)
+    np.testing.assert_allclose(B @ B.T, Bo @ Bo.T, atol=1e-12)
+    np.testing.assert_allclose(B[:, 2:], Bo[:, 2:], atol=1e-12)
+
+
+def test_zero_block_gives_identity_columns(lib):
+    B, p = crange(lib, np.zeros((10, 4)), 4, greedy=True)
+    assert p == 0
+    np.testing.assert_allclose(B, np.eye(10)[:, :4])
+
+
+def test_svd_completed_matches_oracle(lib):
+    rng = np.random.default_rng(13)
+    n = 8
+    S = rng.standard_normal((n, 3)) @ rng.standard_normal((3, n))
+    U, d, V = f(np.zeros((n, n))), np.zeros(n), f(np.zeros((n, n)))
+    assert lib.smallla_svd_completed(n, dp(f(S)), 1e-7, dp(U), dp(d), dp(V)) == 0
+    Uo, do, Vo = dlra.svd_completed(S)
+    np.testing.assert_allclose(d, do, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(U @ np.diag(d) @ V.T, S, atol=1e-12)
+    np.testing.assert_allclose(U[:, 3:] @ U[:, 3:].T, Uo[:, 3:] @ Uo[:, 3:].T, atol=1e-12)
+    np.testing.assert_allclose(U.T @ U, np.eye(n), atol=1e-13)

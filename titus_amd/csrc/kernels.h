@@ -50,6 +50,12 @@ size_t tn_partial_size(int C, int m, int n);
 void launch_tn(hipStream_t st, const Grid& g, const double* A, int lda, int m, const double* B,
                int ldb, int n, int op, double* partial, double* out);
 
+// Cholesky of a k x k SPD block on the device (k <= 96): G + s I = R^T R with
+// the shift schedule of the host code (s = 0 first when allow_plain, then
+// 1e-14 tr(G) x 10^i); writes R, R^-1 and status[0] (attempts, or -1).
+void launch_chol_inv(hipStream_t st, const double* G, int k, bool allow_plain, double* R,
+                     double* Rinv, double* status);
+
 // out[c, b] = beta * out[c, b] + alpha * sum_a A[c, a] T[a, b];  rows with
 // zero_mask[c] != 0 are set to 0 afterwards (when zero_mask is not null).
 void launch_gemm(hipStream_t st, int C, const double* A, int lda, int k, const double* T, int n,

@@ -11,7 +11,7 @@ namespace dev {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kRowsPerChunk = 128;   // rows per block of the TN reductions
+constexpr int kMaxChunks = 256;      // blocks (row chunks) of the TN reductions
 constexpr int kSub = 32;             // rows staged in LDS at a time
 constexpr int kPairsPerThread = 8;   // outputs accumulated per thread
 constexpr int kPairsPerGroup = kBlock * kPairsPerThread;
@@ -133,13 +133,13 @@ __device__ inline double stencil_value(const double* __restrict__ B, int ldb, in
 __global__ void k_tn_partial(const double* __restrict__ A, int lda, int m,
                              const double* __restrict__ B, int ldb, int n, int C, int op, int nx,
                              int ny, double dx, double dy, const double* __restrict__ rho,
-                             double* __restrict__ partial) {
+                             int rows_per_chunk, double* __restrict__ partial) {
   extern __shared__ double lds[];
   double* As = lds;                          // m x (kSub+1)
   double* Bs = lds + (size_t)m * (kSub + 1); // n x (kSub+1)
   const int chunk = blockIdx.x;
-  const int c0 = chunk * kRowsPerChunk;
-  const int c1 = min(C, c0 + kRowsPerChunk);
+  const int c0 = chunk * rows_per_chunk;
+  const int c1 = min(C, c0 + rows_per_chunk);
   const int mn = m * n;
   const int tid = threadIdx.x;
   int pa[kPairsPerThread], pb[kPairsPerThread];
@@ -181,13 +181,84 @@ __global__ void k_tn_partial(const double* __restrict__ A, int lda, int m,
     if (pa[k] >= 0) partial[(size_t)chunk * mn + pa[k] + (size_t)pb[k] * m] = acc[k];
 }
 
+// One wavefront per output: lane l sums chunks l, l+64, ...; then a fixed
+// shuffle tree (deterministic order).
 __global__ void k_tn_reduce(const double* __restrict__ partial, int nchunks, int mn,
                             double* __restrict__ out) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (p >= mn) return;
   double s = 0.0;
-  for (int k = 0; k < nchunks; ++k) s += partial[(size_t)k * mn + p];
-  out[p] = s;
+  for (int k = lane; k < nchunks; k += 64) s += partial[(size_t)k * mn + p];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) out[p] = s;
+}
+
+// Cholesky G + s I = R^T R of a k x k SPD matrix (one workgroup, LDS) and the
+// inverse of R.  Shift schedule: (s = 0 first when allow_plain), then
+// s = 1e-14 tr(G) growing tenfold per failed attempt, at most 12 attempts.
+// status[0] = number of attempts used, or -1 when all failed.
+__global__ void k_chol_inv(const double* __restrict__ G, int k, int allow_plain,
+                           double* __restrict__ R, double* __restrict__ Rinv,
+                           double* __restrict__ status) {
+  extern __shared__ double A[];              // k x k, column-major
+  __shared__ int ok;
+  __shared__ double tr;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  if (tid == 0) {
+    double t = 0.0;
+    for (int i = 0; i < k; ++i) t += G[(size_t)i * k + i];
+    tr = t;
+  }
+  __syncthreads();
+  double s = allow_plain ? 0.0 : (tr > 0.0 ? 1e-14 * tr : 1e-300);
+  int attempt = 0;
+  for (; attempt < 13; ++attempt) {
+    for (int e = tid; e < k * k; e += nt) {
+      const int i = e % k, j = e / k;
+      A[e] = G[e] + (i == j ? s : 0.0);
+    }
+    if (tid == 0) ok = 1;
+    __syncthreads();
+    for (int j = 0; j < k; ++j) {
+      if (tid == 0) {
+        const double d = A[(size_t)j * k + j];
+        if (!(d > 0.0)) ok = 0;
+        else A[(size_t)j * k + j] = sqrt(d);
+      }
+      __syncthreads();
+      if (!ok) break;
+      const double piv = A[(size_t)j * k + j];
+      for (int c = j + 1 + tid; c < k; c += nt) A[(size_t)c * k + j] /= piv;
+      __syncthreads();
+      const int w = k - j - 1;                // trailing block (j, k) x (j, k)
+      for (int e = tid; e < w * w; e += nt) {
+        const int i = j + 1 + e % w, c = j + 1 + e / w;
+        if (i <= c) A[(size_t)c * k + i] -= A[(size_t)i * k + j] * A[(size_t)c * k + j];
+      }
+      __syncthreads();
+    }
+    if (ok) break;
+    __syncthreads();
+    s = (s == 0.0) ? (tr > 0.0 ? 1e-14 * tr : 1e-300) : 10.0 * s;
+  }
+  const int good = ok;
+  __syncthreads();
+  for (int e = tid; e < k * k; e += nt) {
+    const int i = e % k, j = e / k;
+    R[e] = (i <= j) ? A[e] : 0.0;
+  }
+  // column c of R^-1 by back substitution (one thread per column)
+  for (int c = tid; c < k; c += nt) {
+    for (int i = 0; i < k; ++i) Rinv[(size_t)c * k + i] = 0.0;
+    Rinv[(size_t)c * k + c] = 1.0 / A[(size_t)c * k + c];
+    for (int i = c - 1; i >= 0; --i) {
+      double acc = 0.0;
+      for (int m = i + 1; m <= c; ++m) acc += A[(size_t)m * k + i] * Rinv[(size_t)c * k + m];
+      Rinv[(size_t)c * k + i] = -acc / A[(size_t)i * k + i];
+    }
+  }
+  if (tid == 0) status[0] = good ? (double)(attempt + 1) : -1.0;
 }
 
 __global__ void k_gemm(int C, const double* __restrict__ A, int lda, int k,
@@ -257,25 +328,45 @@ void launch_stream_k(hipStream_t st, const Grid& g, const double* X, int ldx, in
   check_launch("stream_K");
 }
 
+static void tn_chunks(int C, int* nchunks, int* rpc) {
+  int nc = std::min(kMaxChunks, (C + kSub - 1) / kSub);
+  int r = (C + nc - 1) / nc;
+  r = ((r + kSub - 1) / kSub) * kSub;          // whole LDS sub-chunks
+  *rpc = r;
+  *nchunks = (C + r - 1) / r;
+}
+
 size_t tn_partial_size(int C, int m, int n) {
-  const size_t nchunks = (C + kRowsPerChunk - 1) / kRowsPerChunk;
-  return nchunks * (size_t)m * n;
+  int nc, rpc;
+  tn_chunks(C, &nc, &rpc);
+  return (size_t)nc * m * n;
 }
 
 void launch_tn(hipStream_t st, const Grid& g, const double* A, int lda, int m, const double* B,
                int ldb, int n, int op, double* partial, double* out) {
   if (m == 0 || n == 0) return;
-  const int nchunks = (g.C + kRowsPerChunk - 1) / kRowsPerChunk;
+  int nchunks, rpc;
+  tn_chunks(g.C, &nchunks, &rpc);
   const int mn = m * n;
   const int groups = (mn + kPairsPerGroup - 1) / kPairsPerGroup;
   const size_t lds = (size_t)(m + n) * (kSub + 1) * sizeof(double);
   if (lds > 160 * 1024) throw std::runtime_error("tn: block too wide for LDS");
   hipLaunchKernelGGL(k_tn_partial, dim3(nchunks, groups), dim3(kBlock), lds, st, A, lda, m, B,
-                     ldb, n, g.C, op, g.nx, g.ny, g.dx, g.dy, g.rho, partial);
+                     ldb, n, g.C, op, g.nx, g.ny, g.dx, g.dy, g.rho, rpc, partial);
   check_launch("tn_partial");
-  hipLaunchKernelGGL(k_tn_reduce, dim3(blocks_for(mn)), dim3(kBlock), 0, st, partial, nchunks,
-                     mn, out);
+  const int per_block = kBlock / 64;
+  hipLaunchKernelGGL(k_tn_reduce, dim3((mn + per_block - 1) / per_block), dim3(kBlock), 0, st,
+                     partial, nchunks, mn, out);
   check_launch("tn_reduce");
+}
+
+void launch_chol_inv(hipStream_t st, const double* G, int k, bool allow_plain, double* R,
+                     double* Rinv, double* status) {
+  if (k <= 0) return;
+  if (k > 96) throw std::runtime_error("chol_inv: block wider than 96 columns");
+  hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(kBlock), (size_t)k * k * sizeof(double), st, G, k,
+                     allow_plain ? 1 : 0, R, Rinv, status);
+  check_launch("chol_inv");
 }
 
 void launch_gemm(hipStream_t st, int C, const double* A, int lda, int k, const double* T, int n,

@@ -2,9 +2,11 @@
 //
 // Division of work: every operation whose size grows with the cell count C
 // (boundary source, dose, uncollided sweep, streaming stencils, Galerkin
-// products, tall-skinny Gram and projection products, basis updates) runs on
-// the device; the r x r, 2r x 2r and Nm x r algebra (angular factor, S
-// updates, SVDs, Cholesky factors, completion) runs on the host in FP64.
+// products, tall-skinny Gram and projection products, Cholesky passes, basis
+// updates) runs on the device; the r x r, 2r x 2r and Nm x r algebra (angular
+// factor, S updates, SVDs, completion) runs on the host in FP64.  Device
+// results that the host needs are collected in one arena and fetched with a
+// single transfer per phase: four host round trips per energy step.
 //
 // Spatial bases.  svd!(K).U (code/SolverCSD.jl:1029,1085) and
 // qr([X K]).Q (code/SolverCSD.jl:2173,2262) only matter through the subspace
@@ -58,6 +60,14 @@ T* dalloc(size_t n) {
   return static_cast<T*>(p);
 }
 
+template <class T>
+T* halloc(size_t n) {
+  void* p = nullptr;
+  if (n == 0) n = 1;
+  HIPCHK(hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault));
+  return static_cast<T*>(p);
+}
+
 // Kernel timing with HIP events on the context's stream.
 struct Timing {
   bool enabled = false;
@@ -102,6 +112,14 @@ struct Timing {
   }
 };
 
+// Device launches of one range basis, waiting for the host to finish it.
+struct RangeJob {
+  double* Kw = nullptr;
+  int k = 0, q = 0, nc = 0;
+  const double* Xa = nullptr;
+  size_t g0 = 0, g1 = 0, r1 = 0, r2 = 0, r3 = 0, st = 0, rows_q = 0, rows_x = 0;
+};
+
 }  // namespace
 
 struct titus_ctx {
@@ -110,7 +128,7 @@ struct titus_ctx {
   // problem (host copy)
   int nx = 0, ny = 0, C = 0, nm = 0, nq = 0, ne = 0, nsig = 0, integrator = 0, precision = 0;
   double dx = 0, dy = 0, dE = 0, theta = 0, theta_index = 0, rank_tol = 1e-7;
-  int rank_cap = 0, capx = 0, capw = 0;
+  int rank_cap = 0, capx = 0;
   std::vector<double> stopping, sigma, bc_scale;
   std::vector<int> degree, cand;
   HMat Ax, Az, aAx, aAz, M;
@@ -121,7 +139,10 @@ struct titus_ctx {
   double *partial = nullptr, *arena = nullptr, *upl = nullptr, *svals = nullptr;
   int *bcells = nullptr, *cand_d = nullptr, *srows = nullptr, *scols = nullptr;
   unsigned char *bmask = nullptr, *restore = nullptr;
-  size_t arena_n = 0, upl_n = 0, scat_n = 0;
+  // pinned host mirrors
+  double *h_arena = nullptr, *h_upl = nullptr, *h_svals = nullptr;
+  int *h_srows = nullptr, *h_scols = nullptr;
+  size_t arena_n = 0, arena_top = 0, upl_n = 0, upl_top = 0, scat_n = 0;
   dev::Grid g{};
   int nb = 0;
   // state
@@ -135,10 +156,12 @@ struct titus_ctx {
     void* ptrs[] = {rho, omega, bc_base, m0, psi, psi2, dose, X, Xn, K, Q1, Q2, partial, arena,
                     upl, svals, bcells, cand_d, srows, scols, bmask, restore};
     for (void* p : ptrs) if (p) hipFree(p);
+    void* hptrs[] = {h_arena, h_upl, h_svals, h_srows, h_scols};
+    for (void* p : hptrs) if (p) hipHostFree(p);
     if (st) hipStreamDestroy(st);
   }
 
-  // ---- timing -------------------------------------------------------------
+  // ---- timing ---------------------------------------------------------------
   // `bytes` / `flops`: algorithmic HBM traffic and arithmetic of the launch
   // (compulsory operand reads and result writes; reduction partials excluded)
   template <class F>
@@ -152,65 +175,74 @@ struct titus_ctx {
     timing.pending.push_back({id, a, b, bytes, flops});
   }
 
-  // ---- transfers ------------------------------------------------------------
-  double* upload(const HMat& h, size_t offset = 0) {
-    if (offset + h.a.size() > upl_n) throw TitusError(TITUS_E_ARG, "upload area too small");
-    double* d = upl + offset;
-    if (!h.a.empty())
-      HIPCHK(hipMemcpyAsync(d, h.a.data(), h.a.size() * sizeof(double), hipMemcpyHostToDevice, st));
-    return d;
+  // ---- arena (device results for the host) and upload ring -------------------
+  size_t aalloc(size_t n) {
+    const size_t off = arena_top;
+    arena_top += n;
+    if (arena_top > arena_n) throw TitusError(TITUS_E_ARG, "result arena too small");
+    return off;
   }
-  double* upload_vec(const std::vector<double>& v, size_t offset) {
-    if (offset + v.size() > upl_n) throw TitusError(TITUS_E_ARG, "upload area too small");
-    double* d = upl + offset;
-    if (!v.empty())
-      HIPCHK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, st));
-    return d;
-  }
-  // copy `n` doubles of the arena (from `off`) to the host and wait
-  std::vector<double> fetch(size_t off, size_t n) {
-    std::vector<double> h(n);
-    if (n) HIPCHK(hipMemcpyAsync(h.data(), arena + off, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  void areset() { arena_top = 0; }
+  // one transfer of everything the arena holds, then wait
+  void fetch() {
+    if (arena_top)
+      HIPCHK(hipMemcpyAsync(h_arena, arena, arena_top * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    return h;
+    upl_top = 0;          // every queued upload has been consumed
   }
-  static HMat as_mat(const std::vector<double>& v, size_t off, int m, int n) {
+  HMat view(size_t off, int m, int n) const {
     HMat A(m, n);
-    std::copy(v.begin() + off, v.begin() + off + (size_t)m * n, A.a.begin());
+    std::memcpy(A.a.data(), h_arena + off, sizeof(double) * (size_t)m * n);
     return A;
   }
+  double* upload_raw(const double* src, size_t n) {
+    if (upl_top + n > upl_n) {
+      HIPCHK(hipStreamSynchronize(st));
+      upl_top = 0;
+      if (n > upl_n) throw TitusError(TITUS_E_ARG, "upload larger than the staging ring");
+    }
+    double* h = h_upl + upl_top;
+    double* d = upl + upl_top;
+    std::memcpy(h, src, n * sizeof(double));
+    if (n) HIPCHK(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, st));
+    upl_top += (n + 31) & ~size_t(31);
+    return d;
+  }
+  double* upload(const HMat& h) { return upload_raw(h.a.data(), h.a.size()); }
 
-  // ---- device building blocks --------------------------------------------
+  // ---- device building blocks --------------------------------------------------
   // arena[off : off + m*n] = A^T op(B)
-  void tn(const double* A, int m, const double* B, int n, int op, size_t off) {
-    if (off + (size_t)m * n > arena_n) throw TitusError(TITUS_E_ARG, "arena too small");
+  size_t tn(const double* A, int m, const double* B, int n, int op) {
+    const size_t off = aalloc((size_t)m * n);
     const double by = 8.0 * C * (A == B ? m : m + n) + (op == dev::OP_NONE ? 0.0 : 8.0 * C);
     timed(op == dev::OP_NONE ? "gram" : "galerkin", by, 2.0 * C * m * n, [&] {
       dev::launch_tn(st, g, A, C, m, B, C, n, op, partial, arena + off);
     });
+    return off;
   }
-  // [Xa^T L1x Xb, Xa^T L1y Xb, Xa^T L2x Xb, Xa^T L2y Xb]
-  std::vector<HMat> galerkin_x(const double* Xa, int ma, const double* Xb, int mb) {
-    const size_t blk = (size_t)ma * mb;
-    for (int op = 0; op < 4; ++op) tn(Xa, ma, Xb, mb, op, op * blk);
-    std::vector<double> h = fetch(0, 4 * blk);
+  // [Xa^T L1x Xb, Xa^T L1y Xb, Xa^T L2x Xb, Xa^T L2y Xb] at the returned offset
+  size_t galerkin_x(const double* Xa, int ma, const double* Xb, int mb) {
+    const size_t off = tn(Xa, ma, Xb, mb, dev::OP_L1X);
+    for (int op = 1; op < 4; ++op) tn(Xa, ma, Xb, mb, op);
+    return off;
+  }
+  std::vector<HMat> galerkin_view(size_t off, int ma, int mb) const {
     std::vector<HMat> out;
-    for (int op = 0; op < 4; ++op) out.push_back(as_mat(h, op * blk, ma, mb));
+    for (int op = 0; op < 4; ++op) out.push_back(view(off + (size_t)op * ma * mb, ma, mb));
     return out;
   }
-  HMat tn_fetch(const double* A, int m, const double* B, int n) {
-    tn(A, m, B, n, dev::OP_NONE, 0);
-    return as_mat(fetch(0, (size_t)m * n), 0, m, n);
-  }
-  void gemm(const double* A, int k, const HMat& T, double alpha, double beta, double* out,
-            const unsigned char* mask, size_t upl_off = 0) {
-    const double* Td = upload(T, upl_off);
-    timed("gemm", 8.0 * C * (k + T.n * (beta != 0.0 ? 2 : 1)), 2.0 * C * k * T.n, [&] {
-      dev::launch_gemm(st, C, A, C, k, Td, T.n, alpha, beta, out, C, mask);
+  void gemm_dev(const double* A, int k, const double* Td, int n, double alpha, double beta,
+                double* out, const unsigned char* mask) {
+    timed("gemm", 8.0 * C * (k + n * (beta != 0.0 ? 2 : 1)), 2.0 * C * k * n, [&] {
+      dev::launch_gemm(st, C, A, C, k, Td, n, alpha, beta, out, C, mask);
     });
   }
+  void gemm(const double* A, int k, const HMat& T, double alpha, double beta, double* out,
+            const unsigned char* mask) {
+    gemm_dev(A, k, upload(T), T.n, alpha, beta, out, mask);
+  }
 
-  // ---- host building blocks (angular side) -----------------------------------
+  // ---- host building blocks (angular side) -------------------------------------
   std::vector<HMat> galerkin_w(const HMat& Wa, const HMat& Wb) const {
     return {titus::matmul(Wa, titus::matmul(aAx, Wb), true), titus::matmul(Wa, titus::matmul(aAz, Wb), true),
             titus::matmul(Wa, titus::matmul(Ax, Wb), true), titus::matmul(Wa, titus::matmul(Az, Wb), true)};
@@ -258,63 +290,78 @@ struct titus_ctx {
     static const std::vector<int> all;
     return titus::range_basis(L, ncols, against, scale, rank_tol, true, all);
   }
+  void stream_k(int r) {
+    std::vector<HMat> WAW = galerkin_w(W, W);
+    const double* Zd = upload(stream_Z(S, WAW));
+    timed("stream_K", 8.0 * C * (2 * r + 1), 12.0 * C * r * r,
+          [&] { dev::launch_stream_k(st, g, X, C, r, Zd, r, dE, K, C); });
+  }
 
-  // ---- device range basis -------------------------------------------------
-  // out (C x ncols) = orthonormal basis of range(Kw) minus span(Xa) (C x q),
-  // completed from the interior candidates.  Kw (C x k) is overwritten.
-  // scale < 0: relative to sigma_1 of the projected block.
-  int range_basis_dev(double* Kw, int k, const double* Xa, int q, int ncols, double* out) {
-    double scale = -1.0;
-    if (Xa && q > 0) {
-      tn(Kw, k, Kw, k, dev::OP_NONE, 0);
-      std::vector<double> g0 = fetch(0, (size_t)k * k);
-      double tr = 0.0;
-      for (int i = 0; i < k; ++i) tr += g0[(size_t)i * k + i];
-      scale = std::sqrt(tr);                              // |K|_F
+  // ---- device range basis --------------------------------------------------------
+  // Launch the device part of the basis of range(Kw) minus span(Xa): the
+  // projection (twice), three Gram + shifted-Cholesky passes and the candidate
+  // rows; everything the host needs lands in the arena.  Kw is overwritten.
+  RangeJob range_launch(double* Kw, int k, const double* Xa, int q) {
+    RangeJob j;
+    j.Kw = Kw; j.k = k; j.Xa = (Xa && q > 0) ? Xa : nullptr; j.q = j.Xa ? q : 0;
+    if (j.Xa) {
+      j.g0 = tn(Kw, k, Kw, k, dev::OP_NONE);              // |K|_F for the rank scale
       for (int pass = 0; pass < 2; ++pass) {
-        if ((size_t)q * k > arena_n) throw TitusError(TITUS_E_ARG, "arena too small");
-        timed("gram", 8.0 * C * (q + k), 2.0 * C * q * k,
-              [&] { dev::launch_tn(st, g, Xa, C, q, Kw, C, k, dev::OP_NONE, partial, arena); });
-        timed("gemm", 8.0 * C * (q + 2 * k), 2.0 * C * q * k,
-              [&] { dev::launch_gemm(st, C, Xa, C, q, arena, k, -1.0, 1.0, Kw, C, nullptr); });
+        const size_t p = tn(Xa, q, Kw, k, dev::OP_NONE);
+        gemm_dev(Xa, q, arena + p, k, -1.0, 1.0, Kw, nullptr);
       }
     }
-    // shifted CholeskyQR: Q1 = K R1^-1, Q2 = Q1 R2^-1, R3 = chol(Q2^T Q2)
-    auto chol_shift = [&](const HMat& G, HMat& R) {
-      double tr = 0.0;
-      for (int i = 0; i < G.n; ++i) tr += G(i, i);
-      double s = tr > 0.0 ? 1e-14 * tr : 1e-300;
-      for (int attempt = 0; attempt < 12; ++attempt, s *= 10.0) {
-        HMat Gs = G;
-        for (int i = 0; i < G.n; ++i) Gs(i, i) += s;
-        if (titus::cholesky_upper(Gs, R)) return;
-      }
-      throw TitusError(TITUS_E_RANK, "shifted Cholesky failed");
-    };
-    HMat G1 = tn_fetch(Kw, k, Kw, k);
+    const size_t sz = (size_t)k * k;
+    j.g1 = tn(Kw, k, Kw, k, dev::OP_NONE);
+    j.r1 = aalloc(sz); const size_t i1 = aalloc(sz);
+    j.r2 = aalloc(sz); const size_t i2 = aalloc(sz);
+    j.r3 = aalloc(sz); const size_t i3 = aalloc(sz);
+    j.st = aalloc(4);
+    timed("chol", 8.0 * 3 * sz, sz * k / 3.0, [&] {
+      dev::launch_chol_inv(st, arena + j.g1, k, false, arena + j.r1, arena + i1, arena + j.st);
+    });
+    gemm_dev(Kw, k, arena + i1, k, 1.0, 0.0, Q1, nullptr);
+    const size_t g2 = tn(Q1, k, Q1, k, dev::OP_NONE);
+    timed("chol", 8.0 * 3 * sz, sz * k / 3.0, [&] {
+      dev::launch_chol_inv(st, arena + g2, k, false, arena + j.r2, arena + i2, arena + j.st + 1);
+    });
+    gemm_dev(Q1, k, arena + i2, k, 1.0, 0.0, Q2, nullptr);
+    const size_t g3 = tn(Q2, k, Q2, k, dev::OP_NONE);
+    timed("chol", 8.0 * 3 * sz, sz * k / 3.0, [&] {
+      dev::launch_chol_inv(st, arena + g3, k, true, arena + j.r3, arena + i3, arena + j.st + 2);
+    });
+    j.nc = std::min<int>((int)cand.size(), capx + 64 + j.q);
+    j.rows_q = aalloc((size_t)j.nc * k);
+    dev::launch_gather_rows(st, Q2, C, k, cand_d, j.nc, arena + j.rows_q);
+    if (j.Xa) {
+      j.rows_x = aalloc((size_t)j.nc * j.q);
+      dev::launch_gather_rows(st, j.Xa, C, j.q, cand_d, j.nc, arena + j.rows_x);
+    }
+    return j;
+  }
+
+  // Host part after fetch(): SVD of R = R3 R2 R1, rank, canonical completion,
+  // and the device launches that write the basis (C x ncols) to `out`.
+  int range_finish(const RangeJob& j, int ncols, double* out) {
+    const int k = j.k, q = j.q;
+    HMat G1 = view(j.g1, k, k);
     double tr1 = 0.0;
     for (int i = 0; i < k; ++i) tr1 += G1(i, i);
-    if (!(std::isfinite(tr1))) throw TitusError(TITUS_E_NONFINITE, "non-finite spatial block");
-    HMat R1, R2, R3;
-    const int nc = std::min<int>((int)cand.size(), ncols + 64 + (Xa ? q : 0));
-    std::vector<int> cidx(cand.begin(), cand.begin() + nc);
-    HMat T, Bc;
+    if (!std::isfinite(tr1)) throw TitusError(TITUS_E_NONFINITE, "non-finite spatial block");
+    double scale = -1.0;
+    if (j.Xa) {
+      HMat G0 = view(j.g0, k, k);
+      double t = 0.0;
+      for (int i = 0; i < k; ++i) t += G0(i, i);
+      scale = std::sqrt(t);
+    }
+    std::vector<int> cidx(cand.begin(), cand.begin() + j.nc);
+    HMat T(k, 0), Bc;
     int p = 0;
     if (tr1 > 0.0) {
-      chol_shift(G1, R1);
-      gemm(Kw, k, titus::inv_upper(R1), 1.0, 0.0, Q1, nullptr);
-      HMat G2 = tn_fetch(Q1, k, Q1, k);
-      chol_shift(G2, R2);
-      gemm(Q1, k, titus::inv_upper(R2), 1.0, 0.0, Q2, nullptr);
-      // G3 and the candidate rows of Q2 (and of Xa) in one round trip
-      tn(Q2, k, Q2, k, dev::OP_NONE, 0);
-      const size_t off_q = (size_t)k * k, off_x = off_q + (size_t)nc * k;
-      if (off_x + (size_t)nc * q > arena_n) throw TitusError(TITUS_E_ARG, "arena too small");
-      dev::launch_gather_rows(st, Q2, C, k, cand_d, nc, arena + off_q);
-      if (Xa && q > 0) dev::launch_gather_rows(st, Xa, C, q, cand_d, nc, arena + off_x);
-      std::vector<double> h = fetch(0, off_x + (size_t)nc * (Xa ? q : 0));
-      HMat G3 = as_mat(h, 0, k, k);
-      if (!titus::cholesky_upper(G3, R3)) chol_shift(G3, R3);
+      for (int t = 0; t < 3; ++t)
+        if (h_arena[j.st + t] < 0.0) throw TitusError(TITUS_E_RANK, "shifted Cholesky failed");
+      HMat R1 = view(j.r1, k, k), R2 = view(j.r2, k, k), R3 = view(j.r3, k, k);
       HMat R = titus::matmul(R3, titus::matmul(R2, R1));
       HMat P, Zr;
       std::vector<double> sv;
@@ -324,69 +371,57 @@ struct titus_ctx {
         for (double x : sv) if (x > rank_tol * sc) ++p;
       p = std::min(p, ncols);
       T = titus::matmul(titus::inv_upper(R3), titus::cols(P, 0, p));     // k x p
-      HMat Qc = as_mat(h, off_q, nc, k);
-      HMat Uc = titus::matmul(Qc, T);                                      // nc x p
-      Bc = (Xa && q > 0) ? titus::hcat(as_mat(h, off_x, nc, q), Uc) : Uc;
+      HMat Uc = titus::matmul(view(j.rows_q, j.nc, k), T);               // nc x p
+      Bc = j.Xa ? titus::hcat(view(j.rows_x, j.nc, q), Uc) : Uc;
     } else {
-      T = HMat(k, 0);
-      if (Xa && q > 0) {
-        dev::launch_gather_rows(st, Xa, C, q, cand_d, nc, arena);
-        Bc = as_mat(fetch(0, (size_t)nc * q), 0, nc, q);
-      } else {
-        Bc = HMat(nc, 0);
-      }
+      Bc = j.Xa ? view(j.rows_x, j.nc, q) : HMat(j.nc, 0);
     }
     const int need = ncols - p;
     std::vector<int> chosen;
     HMat Rinv;
     if (need > 0 && !titus::canonical_completion(Bc, cidx, need, false, chosen, Rinv))
       throw TitusError(TITUS_E_RANK, "canonical completion ran out of candidates");
-    // completion columns: E_ch Rinv - Xa (Xa[ch]^T Rinv) - U (U[ch]^T Rinv)
     std::vector<int> chpos;
-    for (int c : chosen) chpos.push_back((int)(std::find(cidx.begin(), cidx.end(), c) - cidx.begin()));
-    if (p > 0) {
-      HMat Tfull(k, ncols);
-      std::copy(T.a.begin(), T.a.end(), Tfull.a.begin());
-      if (need > 0) {
-        HMat Uch(need, p);
-        const int off = (Xa && q > 0) ? q : 0;
-        for (int s = 0; s < need; ++s)
-          for (int j = 0; j < p; ++j) Uch(s, j) = Bc(chpos[s], off + j);
-        HMat Tc = titus::matmul(T, titus::matmul(Uch, Rinv, true, false));   // k x need
-        for (size_t i = 0; i < Tc.a.size(); ++i) Tfull.a[(size_t)p * k + i] = -Tc.a[i];
-      }
-      gemm(Q2, k, Tfull, 1.0, 0.0, out, nullptr);
-    } else {
-      // no range: start from zero columns (a zero-coefficient product keeps
-      // the path free of memset nodes)
-      gemm(Kw, k, HMat(k, ncols), 1.0, 0.0, out, nullptr);
+    for (int c : chosen)
+      chpos.push_back((int)(std::find(cidx.begin(), cidx.end(), c) - cidx.begin()));
+    // range columns Q2 T; completion columns E_ch Rinv - Xa (Xa[ch]^T Rinv) - U (U[ch]^T Rinv)
+    HMat Tfull(k, ncols);
+    std::copy(T.a.begin(), T.a.end(), Tfull.a.begin());
+    if (p > 0 && need > 0) {
+      HMat Uch(need, p);
+      for (int s = 0; s < need; ++s)
+        for (int c = 0; c < p; ++c) Uch(s, c) = Bc(chpos[s], q + c);
+      HMat Tc = titus::matmul(T, titus::matmul(Uch, Rinv, true, false));   // k x need
+      for (size_t i = 0; i < Tc.a.size(); ++i) Tfull.a[(size_t)p * k + i] = -Tc.a[i];
     }
+    // a zero block is written by a zero-coefficient product (no memset nodes)
+    gemm((p > 0) ? Q2 : j.Kw, k, Tfull, 1.0, 0.0, out, nullptr);
     if (need > 0) {
-      if (Xa && q > 0) {
+      if (j.Xa) {
         HMat Xch(need, q);
         for (int s = 0; s < need; ++s)
-          for (int j = 0; j < q; ++j) Xch(s, j) = Bc(chpos[s], j);
+          for (int c = 0; c < q; ++c) Xch(s, c) = Bc(chpos[s], c);
         HMat Tx = titus::matmul(Xch, Rinv, true, false);                     // q x need
         for (double& v : Tx.a) v = -v;
-        gemm(Xa, q, Tx, 1.0, 1.0, out + (size_t)p * C, nullptr);
+        gemm(j.Xa, q, Tx, 1.0, 1.0, out + (size_t)p * C, nullptr);
       }
-      std::vector<int> rows, colsv;
-      std::vector<double> vals;
+      int ns = 0;
       for (int s = 0; s < need; ++s)
         for (int t = s; t < need; ++t) {
-          rows.push_back(chosen[s]); colsv.push_back(p + t); vals.push_back(Rinv(s, t));
+          if ((size_t)ns >= scat_n) throw TitusError(TITUS_E_ARG, "scatter area too small");
+          h_srows[ns] = chosen[s]; h_scols[ns] = p + t; h_svals[ns] = Rinv(s, t); ++ns;
         }
-      const int ns = (int)rows.size();
-      if ((size_t)ns > scat_n) throw TitusError(TITUS_E_ARG, "scatter area too small");
-      HIPCHK(hipMemcpyAsync(srows, rows.data(), ns * sizeof(int), hipMemcpyHostToDevice, st));
-      HIPCHK(hipMemcpyAsync(scols, colsv.data(), ns * sizeof(int), hipMemcpyHostToDevice, st));
-      HIPCHK(hipMemcpyAsync(svals, vals.data(), ns * sizeof(double), hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(srows, h_srows, ns * sizeof(int), hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(scols, h_scols, ns * sizeof(int), hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(svals, h_svals, ns * sizeof(double), hipMemcpyHostToDevice, st));
       dev::launch_scatter_add(st, out, C, srows, scols, svals, ns);
+      // the pinned scatter lists are reused next time: wait for these copies
+      HIPCHK(hipStreamSynchronize(st));
     }
     return p;
   }
 
-  // ---- one energy step ------------------------------------------------------
+  // ---- pieces of one energy step ---------------------------------------------
   std::vector<double> sig(int s) const {
     return std::vector<double>(sigma.begin() + (size_t)s * nsig, sigma.begin() + (size_t)(s + 1) * nsig);
   }
@@ -394,7 +429,7 @@ struct titus_ctx {
     std::vector<double> v(rank, 0.0);
     for (int a = 0; a < rank; ++a)
       for (int b = 0; b < rank; ++b) v[a] += S(a, b) * W(0, b);
-    const double* vd = upload_vec(v, 0);
+    const double* vd = upload_raw(v.data(), v.size());
     const double coef = 0.5 * dE * Sn;
     timed("dose", 8.0 * C * (rank + nq + 3), 2.0 * C * (rank + nq), [&] {
       dev::launch_dose(st, g, dose, X, C, rank, vd, psi, nq, m0, coef);
@@ -402,7 +437,8 @@ struct titus_ctx {
   }
   void begin_step(int s, std::vector<double>& Dvec, std::vector<double>& Dout) {
     const std::vector<double> sg = sig(s);
-    timed("bc", 16.0 * nb * nq + 4.0 * nb, 1.0 * nb * nq, [&] { dev::launch_bc(st, psi, C, nq, bcells, nb, bc_base, bc_scale[s - 1]); });
+    timed("bc", 16.0 * nb * nq + 4.0 * nb, 1.0 * nb * nq,
+          [&] { dev::launch_bc(st, psi, C, nq, bcells, nb, bc_base, bc_scale[s - 1]); });
     dose_half(stopping[s - 1]);
     const double denom = 1.0 + dE * sg[0];
     timed("sweep", 16.0 * C * nq + 9.0 * C, 12.0 * C * nq, [&] {
@@ -435,7 +471,7 @@ struct titus_ctx {
     gemm(X, rank, S, 1.0, 0.0, K, nullptr);
     HMat T = MtDW;
     for (double& v : T.a) v *= dE;
-    gemm(psi, nq, T, 1.0, 1.0, K, bmask, (size_t)rank * rank);
+    gemm(psi, nq, T, 1.0, 1.0, K, bmask);
   }
 
   void step_fixed(int s) {
@@ -443,43 +479,54 @@ struct titus_ctx {
     begin_step(s, Dvec, Dout);
     const int r = rank;
     if (s > 1) {
-      // K-step (SolverCSD.jl:1019-1031)
-      std::vector<HMat> WAW = galerkin_w(W, W);
-      const double* Zd = upload(stream_Z(S, WAW));
-      timed("stream_K", 8.0 * C * (2 * r + 1), 12.0 * C * r * r,
-            [&] { dev::launch_stream_k(st, g, X, C, r, Zd, r, dE, K, C); });
-      range_basis_dev(K, r, nullptr, 0, r, Xn);
-      HMat MUp = tn_fetch(Xn, r, X, r);
-      // L-step (:1033-1046)
-      std::vector<HMat> XLX = galerkin_x(X, r, X, r);
+      // K-step and old Galerkin products (SolverCSD.jl:1019-1038)
+      areset();
+      const size_t gx = galerkin_x(X, r, X, r);
+      stream_k(r);
+      RangeJob j = range_launch(K, r, nullptr, 0);
+      fetch();
+      std::vector<HMat> XLX = galerkin_view(gx, r, r);
+      range_finish(j, r, Xn);
+      // L-step on the host while the device writes XNew (:1033-1046)
       HMat L = stream_L(titus::matmul(W, S, false, true), XLX);
       HMat WNew = range_w(L, r, nullptr, -1.0);
       HMat NUp = titus::matmul(WNew, W, true, false);
+      areset();
+      const size_t mu = tn(Xn, r, X, r, dev::OP_NONE);
+      const size_t gn = galerkin_x(Xn, r, Xn, r);
+      fetch();
+      HMat MUp = view(mu, r, r);
+      XLX = galerkin_view(gn, r, r);
       W = WNew;
       std::swap(X, Xn);
       // S-step (:1051-1063)
       S = titus::matmul(titus::matmul(MUp, S), NUp, false, true);
-      XLX = galerkin_x(X, r, X, r);
-      WAW = galerkin_w(W, W);
-      S = stream_S(S, XLX, WAW);
+      S = stream_S(S, XLX, galerkin_w(W, W));
     }
     out_scatter(Dout);
     // in-scattering (:1079-1102)
-    HMat MtDW = mtdw(Dvec, W);
-    scatter_K(MtDW);
-    range_basis_dev(K, r, nullptr, 0, r, Xn);
-    HMat MUp = tn_fetch(Xn, r, X, r);
-    HMat PtX = tn_fetch(psi, nq, X, r);                       // psi^T X, old X
+    areset();
+    const size_t ptx = tn(psi, nq, X, r, dev::OP_NONE);     // psi^T X, old X
+    scatter_K(mtdw(Dvec, W));
+    RangeJob j = range_launch(K, r, nullptr, 0);
+    fetch();
+    HMat PtX = view(ptx, nq, r);
+    range_finish(j, r, Xn);
     HMat L = titus::matmul(W, S, false, true);
     HMat MP = titus::matmul(M, PtX);
-    for (int j = 0; j < r; ++j)
-      for (int i = 0; i < nm; ++i) L(i, j) += dE * Dvec[i] * MP(i, j);
+    for (int c = 0; c < r; ++c)
+      for (int i = 0; i < nm; ++i) L(i, c) += dE * Dvec[i] * MP(i, c);
     HMat WNew = range_w(L, r, nullptr, -1.0);
     HMat NUp = titus::matmul(WNew, W, true, false);
+    areset();
+    const size_t mu = tn(Xn, r, X, r, dev::OP_NONE);
+    const size_t xtp = tn(Xn, r, psi, nq, dev::OP_NONE);
+    fetch();
+    HMat MUp = view(mu, r, r);
+    HMat XtP = view(xtp, r, nq);
     W = WNew;
     std::swap(X, Xn);
     S = titus::matmul(titus::matmul(MUp, S), NUp, false, true);
-    HMat XtP = tn_fetch(X, r, psi, nq);
     HMat add = titus::matmul(XtP, mtdw(Dvec, W));
     for (size_t i = 0; i < S.a.size(); ++i) S.a[i] += dE * add.a[i];
     dose_half(stopping[s]);
@@ -497,27 +544,23 @@ struct titus_ctx {
     nrm = std::sqrt(nrm);
     const double tol = theta * std::pow(nrm, theta_index);
     int rmax = n / 2;
-    for (int j = 0; j < n; ++j) {
+    for (int jj = 0; jj < n; ++jj) {
       double tail = 0.0;
-      for (int i = j; i < n; ++i) tail += D[i];
-      if (std::fabs(tail) < tol) { rmax = j + 1; break; }
+      for (int i = jj; i < n; ++i) tail += D[i];
+      if (std::fabs(tail) < tol) { rmax = jj + 1; break; }
     }
     rmax = std::max(std::min(rmax, rank_cap), 2);
-    HMat Uk = titus::cols(U, 0, rmax);
-    gemm(X1, n, Uk, 1.0, 0.0, X, nullptr);
+    gemm(X1, n, titus::cols(U, 0, rmax), 1.0, 0.0, X, nullptr);
     W = titus::matmul(W1, titus::cols(V, 0, rmax));
     S = HMat(rmax, rmax);
     for (int i = 0; i < rmax; ++i) S(i, i) = D[i];
     rank = rmax;
   }
-  // X1 = [X | basis of range(K) minus span(X)] into Xn; returns nothing
-  void augment_x() {
-    HIPCHK(hipMemcpyAsync(Xn, X, (size_t)C * rank * sizeof(double), hipMemcpyDeviceToDevice, st));
-    range_basis_dev(K, rank, X, rank, rank, Xn + (size_t)rank * C);
-  }
   HMat augment_w(const HMat& L) const {
-    const double Ln = titus::frob(L);
-    return titus::hcat(W, range_w(L, rank, &W, Ln));
+    return titus::hcat(W, range_w(L, rank, &W, titus::frob(L)));
+  }
+  void copy_x_to_aug(int r) {
+    HIPCHK(hipMemcpyAsync(Xn, X, (size_t)C * r * sizeof(double), hipMemcpyDeviceToDevice, st));
   }
 
   void step_bug(int s) {
@@ -526,58 +569,65 @@ struct titus_ctx {
     begin_step(s, Dvec, Dout);
     if (s > 1) {
       const int r = rank;
-      std::vector<HMat> XLX = galerkin_x(X, r, X, r);
-      std::vector<HMat> WAW = galerkin_w(W, W);
-      const double* Zd = upload(stream_Z(S, WAW));
-      timed("stream_K", 8.0 * C * (2 * r + 1), 12.0 * C * r * r,
-            [&] { dev::launch_stream_k(st, g, X, C, r, Zd, r, dE, K, C); });
-      augment_x();                                            // Xn = X1 (C x 2r)
-      HMat Mu = tn_fetch(Xn, 2 * r, X, r);
+      areset();
+      const size_t gx = galerkin_x(X, r, X, r);
+      stream_k(r);
+      copy_x_to_aug(r);
+      RangeJob j = range_launch(K, r, X, r);
+      fetch();
+      std::vector<HMat> XLX = galerkin_view(gx, r, r);
+      range_finish(j, r, Xn + (size_t)r * C);               // Xn = X1 = [X | Xt]
       HMat W1 = augment_w(stream_L(titus::matmul(W, S, false, true), XLX));
       HMat Nu = titus::matmul(W1, W, true, false);
-      std::swap(X, Xn);                                       // X = X1
-      W = W1;
-      S = titus::matmul(titus::matmul(Mu, S), Nu, false, true);   // 2r x 2r
-      XLX = galerkin_x(X, 2 * r, X, 2 * r);
-      WAW = galerkin_w(W, W);
-      HMat Sup = stream_S(S, XLX, WAW);
+      areset();
+      const size_t mu = tn(Xn, 2 * r, X, r, dev::OP_NONE);
+      const size_t g1 = galerkin_x(Xn, 2 * r, Xn, 2 * r);
+      fetch();
+      HMat Mu = view(mu, 2 * r, r);
+      XLX = galerkin_view(g1, 2 * r, 2 * r);
+      HMat S1 = titus::matmul(titus::matmul(Mu, S), Nu, false, true);   // 2r x 2r
+      std::vector<HMat> WAW = galerkin_w(W1, W1);
+      HMat Sup = stream_S(S1, XLX, WAW);
       // eta diagnostic (:2210-2220): rows r.. of X1^T L X1, columns r.. of W1^T A W1
       HMat acc(r, r);
       for (int k : {2, 3, 0, 1}) {
         HMat xl(r, 2 * r), wa(2 * r, r);
-        for (int j = 0; j < 2 * r; ++j)
-          for (int i = 0; i < r; ++i) xl(i, j) = XLX[k](r + i, j);
-        for (int j = 0; j < r; ++j)
-          for (int i = 0; i < 2 * r; ++i) wa(i, j) = WAW[k](i, r + j);
-        HMat t = titus::matmul(titus::matmul(xl, S), wa);
+        for (int c = 0; c < 2 * r; ++c)
+          for (int i = 0; i < r; ++i) xl(i, c) = XLX[k](r + i, c);
+        for (int c = 0; c < r; ++c)
+          for (int i = 0; i < 2 * r; ++i) wa(i, c) = WAW[k](i, r + c);
+        HMat t = titus::matmul(titus::matmul(xl, S1), wa);
         for (size_t i = 0; i < acc.a.size(); ++i) acc.a[i] += t.a[i];
       }
       etas.push_back(titus::frob(acc));
-      // X currently holds X1; truncate writes the new basis into Xn, then swap
-      std::swap(X, Xn);                                       // Xn = X1
-      HMat W1c = W;
-      truncate_to(Sup, Xn, W1c);
+      truncate_to(Sup, Xn, W1);
     }
     out_scatter(Dout);
     // in-scattering (:2252-2306)
     const int r = rank;
-    HMat MtDW = mtdw(Dvec, W);
-    HMat Xtpsi = tn_fetch(X, r, psi, nq);                    // old X
-    scatter_K(MtDW);
-    augment_x();
-    HMat Mu = tn_fetch(Xn, 2 * r, X, r);
+    areset();
+    const size_t xtp = tn(X, r, psi, nq, dev::OP_NONE);       // old X
+    scatter_K(mtdw(Dvec, W));
+    copy_x_to_aug(r);
+    RangeJob j = range_launch(K, r, X, r);
+    fetch();
+    HMat Xtpsi = view(xtp, r, nq);
+    range_finish(j, r, Xn + (size_t)r * C);
     HMat L = titus::matmul(W, S, false, true);
     HMat MX = titus::matmul(M, Xtpsi, false, true);
-    for (int j = 0; j < r; ++j)
-      for (int i = 0; i < nm; ++i) L(i, j) += dE * Dvec[i] * MX(i, j);
+    for (int c = 0; c < r; ++c)
+      for (int i = 0; i < nm; ++i) L(i, c) += dE * Dvec[i] * MX(i, c);
     HMat W1 = augment_w(L);
     HMat Nu = titus::matmul(W1, W, true, false);
-    S = titus::matmul(titus::matmul(Mu, S), Nu, false, true);
-    HMat X1tpsi = tn_fetch(Xn, 2 * r, psi, nq);
-    HMat add = titus::matmul(X1tpsi, mtdw(Dvec, W1));
-    HMat Sup = S;
-    for (size_t i = 0; i < Sup.a.size(); ++i) Sup.a[i] += dE * add.a[i];
-    truncate_to(Sup, Xn, W1);
+    areset();
+    const size_t mu = tn(Xn, 2 * r, X, r, dev::OP_NONE);
+    const size_t x1p = tn(Xn, 2 * r, psi, nq, dev::OP_NONE);
+    fetch();
+    HMat Mu = view(mu, 2 * r, r);
+    HMat S1 = titus::matmul(titus::matmul(Mu, S), Nu, false, true);
+    HMat add = titus::matmul(view(x1p, 2 * r, nq), mtdw(Dvec, W1));
+    for (size_t i = 0; i < S1.a.size(); ++i) S1.a[i] += dE * add.a[i];
+    truncate_to(S1, Xn, W1);
     dose_half(stopping[s]);
   }
 
@@ -671,6 +721,7 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     x.capx = p->integrator == TITUS_INTEGRATOR_BUG ? 2 * x.rank_cap : x.rank;
     require(x.capx <= p->nm || p->integrator == TITUS_INTEGRATOR_FIXED, "2 * rank cap exceeds the moment count");
     require(x.rank <= p->nm, "rank exceeds the moment count");
+    require(x.capx <= 96, "rank cap too large for the device Cholesky (2 r <= 96)");
     require(x.capx + 8 <= p->ncand, "too few completion candidates for the rank cap");
     x.stopping.assign(p->stopping, p->stopping + p->ne);
     x.sigma.assign(p->sigma, p->sigma + (size_t)p->ne * p->nsigma);
@@ -699,15 +750,22 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     x.Q2 = dalloc<double>(cx);
     const int wmax = std::max(x.capx, p->nq);
     x.partial = dalloc<double>(dev::tn_partial_size(C, wmax, wmax));
-    const int ncand_used = std::min<int>(p->ncand, 2 * x.capx + 64);
-    x.arena_n = std::max<size_t>(4 * (size_t)wmax * wmax, (size_t)x.capx * x.capx + (size_t)ncand_used * 2 * x.capx);
+    const size_t w2 = (size_t)wmax * wmax;
+    const size_t ncand_used = std::min<size_t>(p->ncand, (size_t)2 * x.capx + 64);
+    // largest phase: Galerkin of the augmented basis plus one range job
+    x.arena_n = 8 * w2 + 10 * (size_t)x.capx * x.capx + ncand_used * 2 * x.capx + 256;
     x.arena = dalloc<double>(x.arena_n);
-    x.upl_n = std::max<size_t>(6 * (size_t)x.capx * x.capx + 64, (size_t)x.capx * (x.capx + p->nq) + 2 * (size_t)wmax * wmax);
+    x.h_arena = halloc<double>(x.arena_n);
+    x.upl_n = 64 * w2 + 8192;
     x.upl = dalloc<double>(x.upl_n);
+    x.h_upl = halloc<double>(x.upl_n);
     x.scat_n = (size_t)x.capx * x.capx;
     x.srows = dalloc<int>(x.scat_n);
     x.scols = dalloc<int>(x.scat_n);
     x.svals = dalloc<double>(x.scat_n);
+    x.h_srows = halloc<int>(x.scat_n);
+    x.h_scols = halloc<int>(x.scat_n);
+    x.h_svals = halloc<double>(x.scat_n);
     x.bcells = dalloc<int>(p->nb);
     x.cand_d = dalloc<int>(p->ncand);
     x.bmask = dalloc<unsigned char>(C);
@@ -829,8 +887,8 @@ int titus_kernel_stats(const titus_ctx* cctx, const char* name, double* total_ms
     HIPCHK(hipStreamSynchronize(ctx->st));
     ctx->timing.resolve();
     auto it = ctx->timing.ids.find(name);
-    if (total_ms) *total_ms = it == ctx->timing.ids.end() ? 0.0 : ctx->timing.total_ms[it->second];
     const bool found = it != ctx->timing.ids.end();
+    if (total_ms) *total_ms = found ? ctx->timing.total_ms[it->second] : 0.0;
     if (launches) *launches = found ? ctx->timing.launches[it->second] : 0;
     if (bytes) *bytes = found ? ctx->timing.total_bytes[it->second] : 0.0;
     if (flops) *flops = found ? ctx->timing.total_flops[it->second] : 0.0;
