@@ -5,17 +5,22 @@ heterogeneous ellipse phantom over 10x10 cm (seed 0: 1.0 g/cc background,
 0.25 g/cc lung and 1.85 g/cc bone ellipses), 15 MeV electron pencil beam,
 P_21 (253 moments), rank-adaptive augmented BUG with truncation tolerance 1e-2
 (relative) and rank cap 40, energy grid by the reference's rule (823 points).
-The plain FP64 form runs here.
+
+Both forms run on the same workload: the plain FP64 form and the mixed form
+(FP32 uncollided sweep, streaming stencil, Galerkin and psi products; FP64
+Cholesky-QR, truncation SVD and dose sum).  `value` is the mixed form's rate;
+the plain form's rate, their ratio and the dose difference after the timed
+steps are reported beside it.
 
 A step is one iteration of the energy loop (boundary source, dose, uncollided
-sweep, streaming K/L/S, truncation, out- and in-scattering, dose).  The
-context (all operands in HBM) is created before the timed region; W warmup
+sweep, streaming K/L/S, truncation, out- and in-scattering, dose).  Each
+context (all operands in HBM) is created before its timed region; W warmup
 steps run first, then K steps are timed between stream synchronisations.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds T]
 
 Prints one JSON line.  `roofline` is computed from the library's own HIP-event
-timing of its dominant kernel during the timed region and the algorithmic
+timing of the dominant kernel during the timed region and the algorithmic
 bytes/flops of those launches; `cpu_baseline` times the FP64 oracle (numpy,
 all BLAS threads) on a bounded prefix of the same solve.
 """
@@ -25,13 +30,14 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "energy steps/s & beamlets/s, mixed vs FP64 GPU at dose rel-L2 error ≤1e-4"
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
-RIDGE = FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 
 # HBM bytes per launch of the dominant kernel measured with rocprofv3 --pmc
 # (FETCH_SIZE + WRITE_SIZE, scaled by a calibration copy); None until measured.
@@ -68,6 +74,46 @@ def cpu_baseline(P, seconds):
             "sample": f"FP64 numpy oracle, steps 1..{steps} of the same solve, {elapsed:.1f} s"}
 
 
+def run_form(P, precision, W, K, events, sync):
+    from titus_amd import _lib
+    ctx = _lib.Context(P, _lib.INTEGRATOR_BUG, precision=precision)
+    try:
+        ctx.solve(W)
+        sync()
+        ctx.set_timing(events)
+        t0 = time.perf_counter()
+        ctx.solve(W + K)
+        sync()
+        elapsed = time.perf_counter() - t0
+        stats = {name: ctx.kernel_stats(name) for name in ctx.KERNELS}
+        done, _ = ctx.progress()
+        ranks, _ = ctx.history(done)
+        dose = ctx.state()["dose"]
+    finally:
+        ctx.close()
+    return dict(elapsed=elapsed, rate=K / elapsed, stats=stats, ranks=ranks, dose=dose)
+
+
+def roofline(stats, fp32):
+    dom = max(stats, key=lambda k: stats[k][0])
+    ms, launches, by, fl = stats[dom]
+    t_avg = ms / max(launches, 1) * 1e-3
+    peak_f = PEAK_TFLOPS["f32" if fp32 and dom in ("sweep", "stream_K", "galerkin") else "f64"]
+    ridge = peak_f * 1e12 / (HBM_PEAK_GBS * 1e9)
+    if by > 0 and fl / by > ridge:
+        roof = {"bound": "mfma", "achieved": fl / max(launches, 1) / t_avg / 1e12,
+                "peak": peak_f, "unit": "TFLOP/s"}
+    else:
+        roof = {"bound": "hbm", "achieved": by / max(launches, 1) / t_avg / 1e9,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = MEASURED_TRAFFIC.get(dom)
+    roof["kernel"] = dom
+    roof["avg_launch_us"] = t_avg * 1e6
+    roof["launches"] = launches
+    return roof
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +121,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-events", action="store_true",
+                    help="time without per-kernel HIP events (no roofline)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -86,68 +134,50 @@ def main():
     last = P["nE"] - 2
     W = min(args.warmup, last - 1)
     K = min(args.steps, last - W)
-    ctx = _lib.Context(P, _lib.INTEGRATOR_BUG)
     try:
         import torch
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
     except Exception:
         sync = lambda: None
-    ctx.solve(W)
-    sync()
-    ctx.set_timing(True)
-    t0 = time.perf_counter()
-    ctx.solve(W + K)
-    sync()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    stats = {name: ctx.kernel_stats(name) for name in ctx.KERNELS}
-    done, rank = ctx.progress()
-    ranks, _ = ctx.history(done)
-    ctx.close()
-
-    dev_ms = sum(v[0] for v in stats.values())
-    dom = max(stats, key=lambda k: stats[k][0])
-    ms, launches, by, fl = stats[dom]
-    t_avg = ms / max(launches, 1) * 1e-3
-    intensity = fl / by if by > 0 else 0.0
-    if intensity > RIDGE:
-        roof = {"bound": "mfma", "achieved": fl / max(launches, 1) / t_avg / 1e12,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s"}
-    else:
-        roof = {"bound": "hbm", "achieved": by / max(launches, 1) / t_avg / 1e9,
-                "peak": HBM_PEAK_GBS, "unit": "GB/s"}
-    roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = MEASURED_TRAFFIC.get(dom)
-    roof["kernel"] = dom
-    roof["avg_launch_us"] = t_avg * 1e6
-    roof["launches"] = launches
+    events = not args.no_events
+    f64 = run_form(P, _lib.PRECISION_FP64, W, K, events, sync)
+    mix = run_form(P, _lib.PRECISION_MIXED, W, K, events, sync)
+    derr = float(np.linalg.norm(mix["dose"] - f64["dose"]) / np.linalg.norm(f64["dose"]))
 
     out = {
         "metric": METRIC,
-        "value": K / elapsed,
+        "value": mix["rate"],
         "unit": "energy steps/s",
         "n_gpus": 1,
         "steps": K,
         "warmup": W,
-        "ms_per_step": elapsed / K * 1e3,
+        "ms_per_step": mix["elapsed"] / K * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "mixed (f32 sweep/stencil/Galerkin/psi products, f64 QR/SVD/dose)",
         "data": "synthetic (seeded ellipse phantom, analytic beam source)",
         "config": {
             "workload": "2D heterogeneous phantom 200x200 over 10x10 cm, 15 MeV e-, P_21 "
                         "(253 moments), rank-adaptive BUG theta=1e-2, rank cap 40, "
-                        f"{P['nE']}-point energy grid, plain FP64",
+                        f"{P['nE']}-point energy grid, single beamlet",
             "cells": P["C"], "moments": P["Nm"], "ordinates": P["Q"],
-            "steps_timed": f"{W + 1}..{W + K}", "rank_range": [int(ranks[W:].min()), int(ranks[W:].max())],
+            "steps_timed": f"{W + 1}..{W + K}",
+            "rank_range": [int(mix["ranks"][W:].min()), int(mix["ranks"][W:].max())],
             "parallelism": "single GPU",
         },
-        "device_ms_per_step": dev_ms / K,
-        "kernel_ms": {k: round(v[0], 3) for k, v in stats.items()},
-        "roofline": roof,
-        "cpu_baseline": None if args.no_cpu else cpu_baseline(P, args.cpu_seconds),
+        "fp64": {"value": f64["rate"], "ms_per_step": f64["elapsed"] / K * 1e3,
+                 "device_ms_per_step": sum(v[0] for v in f64["stats"].values()) / K},
+        "mixed_over_fp64": mix["rate"] / f64["rate"],
+        "dose_rel_l2_mixed_vs_fp64": derr,
+        "device_ms_per_step": sum(v[0] for v in mix["stats"].values()) / K,
+        "kernel_ms": {k: round(v[0], 3) for k, v in mix["stats"].items()},
+        "events_in_timed_region": events,
     }
+    if events:
+        out["roofline"] = roofline(mix["stats"], fp32=True)
+        out["roofline_fp64"] = roofline(f64["stats"], fp32=False)
+    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(P, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

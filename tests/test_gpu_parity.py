@@ -22,8 +22,8 @@ def rel(a, b):
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
 
 
-def run_gpu(P, integrator, last_step=0):
-    ctx = _lib.Context(P, integrator)
+def run_gpu(P, integrator, last_step=0, precision=_lib.PRECISION_FP64):
+    ctx = _lib.Context(P, integrator, precision=precision)
     try:
         ctx.solve(last_step)
         st = ctx.state()
@@ -127,6 +127,25 @@ def test_bug_narrow_beam_phantom():
     o = dlra.solve_bug(P)
     assert list(g["ranks"]) == list(o["ranks"])
     check_close(g, o, "bug narrow")
+
+
+MIXED_TOL = 1e-4   # mixed form: FP32 sweeps/stencils/Galerkin, FP64 QR/SVD/dose
+
+
+@pytest.mark.parametrize("name", ["slab_beam", "small_bug", "bug_narrow"])
+def test_mixed_form_dose_error(name):
+    P = {"slab_beam": cases.slab_beam,
+         "small_bug": cases.small_bug,
+         "bug_narrow": lambda: cases.small_bug(n=48, N=9, cap=16, seed=1, angular=1e4)}[name]()
+    announce(f"mixed {name}", P)
+    integ = _lib.INTEGRATOR_FIXED if name == "slab_beam" else _lib.INTEGRATOR_BUG
+    g = run_gpu(P, integ, precision=_lib.PRECISION_MIXED)
+    o = dlra.solve_fixed(P) if integ == _lib.INTEGRATOR_FIXED else dlra.solve_bug(P)
+    ed = rel(g["dose"], o["dose"])
+    print(f"[mixed {name}] dose rel vs FP64 oracle {ed:.3e}", flush=True)
+    assert np.all(np.isfinite(g["dose"]))
+    assert ed <= MIXED_TOL
+    assert ed > 0.0          # FP32 arithmetic is really in use
 
 
 def test_step_order_errors():

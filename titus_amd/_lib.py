@@ -188,7 +188,7 @@ class Context:
     def set_timing(self, on):
         check(self._lib.titus_set_timing(self.h, 1 if on else 0))
 
-    KERNELS = ("bc", "dose", "sweep", "stream_K", "galerkin", "gram", "gemm")
+    KERNELS = ("bc", "dose", "sweep", "stream_K", "galerkin", "gram", "gemm", "chol")
 
     def kernel_stats(self, name):
         """(total ms, launches, algorithmic bytes, flops) since timing was enabled."""

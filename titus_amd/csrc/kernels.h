@@ -4,6 +4,11 @@
 // `ld` (>= C): column a of block A starts at A + a*ld, cell c = i*ny + j.
 // Small matrices (r x r, k x n) are column-major with their row count as
 // leading dimension.
+//
+// Precision: spatial bases are stored in FP64.  The uncollided flux psi is
+// stored in FP64 (plain form) or FP32 (mixed form); the streaming stencil,
+// the Galerkin products and the psi contractions can compute in FP32
+// (`fp32`), the Cholesky-QR passes and the dose accumulation always in FP64.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstddef>
@@ -11,9 +16,12 @@
 namespace titus {
 namespace dev {
 
-// Stencil operators of code/stencils.jl, in the order the Galerkin kernel and
-// the streaming kernel use them.
-enum StencilOp { OP_NONE = -1, OP_L1X = 0, OP_L1Y = 1, OP_L2X = 2, OP_L2Y = 3 };
+// OP_ALL4 computes the four stencil products at once (output: 4 consecutive
+// m x n blocks in the order L1x, L1y, L2x, L2y).
+enum StencilOp { OP_NONE = -1, OP_L1X = 0, OP_L1Y = 1, OP_L2X = 2, OP_L2Y = 3, OP_ALL4 = 4 };
+
+// element type of a device block
+enum class DT { F64, F32 };
 
 struct Grid {
   int nx, ny, C;
@@ -22,17 +30,18 @@ struct Grid {
 };
 
 // psi[bcells[b] + q*C] = bc_base[b + q*nb] * scale     (SolverCSD.jl:977-988)
-void launch_bc(hipStream_t st, double* psi, int C, int nq, const int* bcells, int nb,
+void launch_bc(hipStream_t st, void* psi, DT tpsi, int C, int nq, const int* bcells, int nb,
                const double* bc_base, double scale);
 
-// dose[c] += coef * (X[c,:] . v + psi[c,:] . m0) / rho[c]   (SolverCSD.jl:990-997)
+// dose[c] += coef * (X[c,:] . v + psi[c,:] . m0) / rho[c]   (SolverCSD.jl:990-997), FP64
 void launch_dose(hipStream_t st, const Grid& g, double* dose, const double* X, int ldx, int r,
-                 const double* v, const double* psi, int nq, const double* m0, double coef);
+                 const double* v, const void* psi, DT tpsi, int nq, const double* m0, double coef);
 
 // Uncollided upwind update with the boundary restore (SolverCSD.jl:388-463,
 // 1000-1005): pout = (pin - dE * flux(pin/rho)) / denom on interior cells,
 // pin / denom on ghost cells, pin itself for ordinate 0 of `restore` cells.
-void launch_sweep(hipStream_t st, const Grid& g, const double* pin, double* pout, int nq,
+// Computes in the storage type of psi.
+void launch_sweep(hipStream_t st, const Grid& g, const void* pin, void* pout, DT tpsi, int nq,
                   const double* omega, double dE, double denom, const unsigned char* restore);
 
 // Streaming K-step (SolverCSD.jl:1020-1027) with the stencils applied matrix
@@ -40,15 +49,20 @@ void launch_sweep(hipStream_t st, const Grid& g, const double* pin, double* pout
 // interior cells and X[c,:] Z0 on ghost cells.  Z holds the six r x n blocks
 // [Z0 Zc Ze Zw Zn Zs] side by side (r x 6n, column-major).
 void launch_stream_k(hipStream_t st, const Grid& g, const double* X, int ldx, int r,
-                     const double* Z, int n, double dE, double* K, int ldk);
+                     const double* Z, int n, double dE, double* K, int ldk, bool fp32);
 
 // Tall-skinny A^T op(B) (m x n) with deterministic two-stage reduction.
-// op = OP_NONE: B itself; otherwise the stencil operator applied to B (the
-// Galerkin products X^T L X of SolverCSD.jl:1035-1038).
-// `partial` must hold tn_partial_size(C, m, n) doubles; result goes to `out`.
+// op = OP_NONE: B itself; otherwise the stencil operator(s) applied to B (the
+// Galerkin products X^T L X of SolverCSD.jl:1035-1038).  `fp32`: stage and
+// accumulate in FP32.  `partial` must hold tn_partial_size(C, m, n) doubles.
 size_t tn_partial_size(int C, int m, int n);
-void launch_tn(hipStream_t st, const Grid& g, const double* A, int lda, int m, const double* B,
-               int ldb, int n, int op, double* partial, double* out);
+void launch_tn(hipStream_t st, const Grid& g, const void* A, DT ta, int lda, int m, const void* B,
+               DT tb, int ldb, int n, int op, bool fp32, double* partial, double* out);
+// FP64 convenience form
+inline void launch_tn(hipStream_t st, const Grid& g, const double* A, int lda, int m,
+                      const double* B, int ldb, int n, int op, double* partial, double* out) {
+  launch_tn(st, g, A, DT::F64, lda, m, B, DT::F64, ldb, n, op, false, partial, out);
+}
 
 // Cholesky of a k x k SPD block on the device (k <= 96): G + s I = R^T R with
 // the shift schedule of the host code (s = 0 first when allow_plain, then
@@ -56,10 +70,17 @@ void launch_tn(hipStream_t st, const Grid& g, const double* A, int lda, int m, c
 void launch_chol_inv(hipStream_t st, const double* G, int k, bool allow_plain, double* R,
                      double* Rinv, double* status);
 
-// out[c, b] = beta * out[c, b] + alpha * sum_a A[c, a] T[a, b];  rows with
-// zero_mask[c] != 0 are set to 0 afterwards (when zero_mask is not null).
-void launch_gemm(hipStream_t st, int C, const double* A, int lda, int k, const double* T, int n,
-                 double alpha, double beta, double* out, int ldo, const unsigned char* zero_mask);
+// out[c, b] = beta * out[c, b] + alpha * sum_a A[c, a] T[a, b] (out FP64);
+// rows with zero_mask[c] != 0 are set to 0 afterwards (when zero_mask is not
+// null).  `fp32`: products and sums in FP32.
+void launch_gemm(hipStream_t st, int C, const void* A, DT ta, int lda, int k, const double* T,
+                 int n, double alpha, double beta, double* out, int ldo,
+                 const unsigned char* zero_mask, bool fp32);
+inline void launch_gemm(hipStream_t st, int C, const double* A, int lda, int k, const double* T,
+                        int n, double alpha, double beta, double* out, int ldo,
+                        const unsigned char* zero_mask) {
+  launch_gemm(st, C, A, DT::F64, lda, k, T, n, alpha, beta, out, ldo, zero_mask, false);
+}
 
 // out[t + k*nrows] = A[rows[t] + k*lda], k < ncols.
 void launch_gather_rows(hipStream_t st, const double* A, int lda, int ncols, const int* rows,
@@ -68,9 +89,6 @@ void launch_gather_rows(hipStream_t st, const double* A, int lda, int ncols, con
 // A[rows[t] + cols[t]*lda] += vals[t].
 void launch_scatter_add(hipStream_t st, double* A, int lda, const int* rows, const int* cols,
                         const double* vals, int n);
-
-// Copy an m-row, n-column block (device to device, leading dimensions given).
-void copy_block(hipStream_t st, double* dst, int ldd, const double* src, int lds, int m, int n);
 
 }  // namespace dev
 }  // namespace titus

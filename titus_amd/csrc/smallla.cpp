@@ -14,6 +14,37 @@ HMat HMat::eye(int m, int n) {
   return I;
 }
 
+SpMat SpMat::from_dense(const HMat& A) {
+  SpMat S;
+  S.m = A.m;
+  S.n = A.n;
+  S.rowptr.assign(A.m + 1, 0);
+  for (int i = 0; i < A.m; ++i) {
+    for (int j = 0; j < A.n; ++j)
+      if (A(i, j) != 0.0) {
+        S.col.push_back(j);
+        S.val.push_back(A(i, j));
+      }
+    S.rowptr[i + 1] = (int)S.col.size();
+  }
+  return S;
+}
+
+HMat spmm(const SpMat& A, const HMat& B) {
+  if (A.n != B.m) throw std::runtime_error("spmm: inner dimensions differ");
+  HMat C(A.m, B.n);
+  for (int j = 0; j < B.n; ++j) {
+    const double* b = B.col(j);
+    double* c = C.col(j);
+    for (int i = 0; i < A.m; ++i) {
+      double s = 0.0;
+      for (int t = A.rowptr[i]; t < A.rowptr[i + 1]; ++t) s += A.val[t] * b[A.col[t]];
+      c[i] = s;
+    }
+  }
+  return C;
+}
+
 HMat matmul(const HMat& A, const HMat& B, bool ta, bool tb) {
   const int m = ta ? A.n : A.m, k = ta ? A.m : A.n;
   const int k2 = tb ? B.n : B.m, n = tb ? B.m : B.n;
@@ -165,10 +196,10 @@ static void jacobi_square(HMat& B, HMat& V) {
           vp[i] = c * x - s * y;
           vq[i] = s * x + c * y;
         }
-        double np = 0.0, nq = 0.0;
-        for (int i = 0; i < B.m; ++i) { np += bp[i] * bp[i]; nq += bq[i] * bq[i]; }
-        nrm[p] = np;
-        nrm[q] = nq;
+        // squared norms after the rotation (exact in exact arithmetic);
+        // recomputed from the columns at the start of every sweep
+        nrm[p] = alpha - t * gamma;
+        nrm[q] = beta + t * gamma;
       }
     }
     if (!rotated) return;

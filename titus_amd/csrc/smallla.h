@@ -26,6 +26,17 @@ struct HMat {
   static HMat eye(int m, int n);
 };
 
+// Compressed-row sparse matrix (the P_N streaming matrices have 2..64
+// nonzeros per row out of Nm).
+struct SpMat {
+  int m = 0, n = 0;
+  std::vector<int> rowptr, col;
+  std::vector<double> val;
+  static SpMat from_dense(const HMat& A);
+};
+// A * B for sparse A and dense B.
+HMat spmm(const SpMat& A, const HMat& B);
+
 // C = op(A) * op(B) with op = transpose when the flag is set.
 HMat matmul(const HMat& A, const HMat& B, bool ta = false, bool tb = false);
 HMat transpose(const HMat& A);

@@ -132,9 +132,13 @@ struct titus_ctx {
   std::vector<double> stopping, sigma, bc_scale;
   std::vector<int> degree, cand;
   HMat Ax, Az, aAx, aAz, M;
+  titus::SpMat sAx[4];      // |Ax|, |Az|, Ax, Az in compressed rows
   // device buffers
   double *rho = nullptr, *omega = nullptr, *bc_base = nullptr, *m0 = nullptr;
-  double *psi = nullptr, *psi2 = nullptr, *dose = nullptr;
+  void *psi = nullptr, *psi2 = nullptr;    // C x nq, FP64 (plain) or FP32 (mixed)
+  dev::DT tpsi = dev::DT::F64;
+  bool mixed = false;                       // FP32 sweeps, stencils, Galerkin, psi products
+  double* dose = nullptr;
   double *X = nullptr, *Xn = nullptr, *K = nullptr, *Q1 = nullptr, *Q2 = nullptr;
   double *partial = nullptr, *arena = nullptr, *upl = nullptr, *svals = nullptr;
   int *bcells = nullptr, *cand_d = nullptr, *srows = nullptr, *scols = nullptr;
@@ -213,18 +217,21 @@ struct titus_ctx {
   // ---- device building blocks --------------------------------------------------
   // arena[off : off + m*n] = A^T op(B)
   size_t tn(const double* A, int m, const double* B, int n, int op) {
-    const size_t off = aalloc((size_t)m * n);
+    const int nops = op == dev::OP_ALL4 ? 4 : 1;
+    const size_t off = aalloc((size_t)nops * m * n);
     const double by = 8.0 * C * (A == B ? m : m + n) + (op == dev::OP_NONE ? 0.0 : 8.0 * C);
-    timed(op == dev::OP_NONE ? "gram" : "galerkin", by, 2.0 * C * m * n, [&] {
-      dev::launch_tn(st, g, A, C, m, B, C, n, op, partial, arena + off);
+    // Galerkin products run in FP32 in the mixed form; Gram and projection
+    // products of the QR passes stay FP64
+    const bool fp32 = mixed && op != dev::OP_NONE;
+    timed(op == dev::OP_NONE ? "gram" : "galerkin", by, 2.0 * nops * C * m * n, [&] {
+      dev::launch_tn(st, g, A, dev::DT::F64, C, m, B, dev::DT::F64, C, n, op, fp32, partial,
+                     arena + off);
     });
     return off;
   }
   // [Xa^T L1x Xb, Xa^T L1y Xb, Xa^T L2x Xb, Xa^T L2y Xb] at the returned offset
   size_t galerkin_x(const double* Xa, int ma, const double* Xb, int mb) {
-    const size_t off = tn(Xa, ma, Xb, mb, dev::OP_L1X);
-    for (int op = 1; op < 4; ++op) tn(Xa, ma, Xb, mb, op);
-    return off;
+    return tn(Xa, ma, Xb, mb, dev::OP_ALL4);
   }
   std::vector<HMat> galerkin_view(size_t off, int ma, int mb) const {
     std::vector<HMat> out;
@@ -244,15 +251,14 @@ struct titus_ctx {
 
   // ---- host building blocks (angular side) -------------------------------------
   std::vector<HMat> galerkin_w(const HMat& Wa, const HMat& Wb) const {
-    return {titus::matmul(Wa, titus::matmul(aAx, Wb), true), titus::matmul(Wa, titus::matmul(aAz, Wb), true),
-            titus::matmul(Wa, titus::matmul(Ax, Wb), true), titus::matmul(Wa, titus::matmul(Az, Wb), true)};
+    return {titus::matmul(Wa, titus::spmm(sAx[0], Wb), true), titus::matmul(Wa, titus::spmm(sAx[1], Wb), true),
+            titus::matmul(Wa, titus::spmm(sAx[2], Wb), true), titus::matmul(Wa, titus::spmm(sAx[3], Wb), true)};
   }
   // L - dE (Ax L XL2xX^T + Az L XL2yX^T + |Ax| L XL1xX^T + |Az| L XL1yX^T)
   HMat stream_L(const HMat& L, const std::vector<HMat>& XLX) const {
     HMat out = L;
-    const HMat* A[4] = {&aAx, &aAz, &Ax, &Az};
     for (int k : {2, 3, 0, 1}) {
-      HMat t = titus::matmul(titus::matmul(*A[k], L), XLX[k], false, true);
+      HMat t = titus::matmul(titus::spmm(sAx[k], L), XLX[k], false, true);
       for (size_t i = 0; i < out.a.size(); ++i) out.a[i] -= dE * t.a[i];
     }
     return out;
@@ -294,7 +300,7 @@ struct titus_ctx {
     std::vector<HMat> WAW = galerkin_w(W, W);
     const double* Zd = upload(stream_Z(S, WAW));
     timed("stream_K", 8.0 * C * (2 * r + 1), 12.0 * C * r * r,
-          [&] { dev::launch_stream_k(st, g, X, C, r, Zd, r, dE, K, C); });
+          [&] { dev::launch_stream_k(st, g, X, C, r, Zd, r, dE, K, C, mixed); });
   }
 
   // ---- device range basis --------------------------------------------------------
@@ -431,18 +437,20 @@ struct titus_ctx {
       for (int b = 0; b < rank; ++b) v[a] += S(a, b) * W(0, b);
     const double* vd = upload_raw(v.data(), v.size());
     const double coef = 0.5 * dE * Sn;
-    timed("dose", 8.0 * C * (rank + nq + 3), 2.0 * C * (rank + nq), [&] {
-      dev::launch_dose(st, g, dose, X, C, rank, vd, psi, nq, m0, coef);
+    const double ps = mixed ? 4.0 : 8.0;     // bytes per psi entry
+    timed("dose", 8.0 * C * (rank + 3) + ps * C * nq, 2.0 * C * (rank + nq), [&] {
+      dev::launch_dose(st, g, dose, X, C, rank, vd, psi, tpsi, nq, m0, coef);
     });
   }
   void begin_step(int s, std::vector<double>& Dvec, std::vector<double>& Dout) {
     const std::vector<double> sg = sig(s);
-    timed("bc", 16.0 * nb * nq + 4.0 * nb, 1.0 * nb * nq,
-          [&] { dev::launch_bc(st, psi, C, nq, bcells, nb, bc_base, bc_scale[s - 1]); });
+    const double ps = mixed ? 4.0 : 8.0;
+    timed("bc", (8.0 + ps) * nb * nq + 4.0 * nb, 1.0 * nb * nq,
+          [&] { dev::launch_bc(st, psi, tpsi, C, nq, bcells, nb, bc_base, bc_scale[s - 1]); });
     dose_half(stopping[s - 1]);
     const double denom = 1.0 + dE * sg[0];
-    timed("sweep", 16.0 * C * nq + 9.0 * C, 12.0 * C * nq, [&] {
-      dev::launch_sweep(st, g, psi, psi2, nq, omega, dE, denom, restore);
+    timed("sweep", 2.0 * ps * C * nq + 9.0 * C, 12.0 * C * nq, [&] {
+      dev::launch_sweep(st, g, psi, psi2, tpsi, nq, omega, dE, denom, restore);
     });
     std::swap(psi, psi2);
     Dvec.resize(nm);
@@ -471,7 +479,21 @@ struct titus_ctx {
     gemm(X, rank, S, 1.0, 0.0, K, nullptr);
     HMat T = MtDW;
     for (double& v : T.a) v *= dE;
-    gemm(psi, nq, T, 1.0, 1.0, K, bmask);
+    const double* Td = upload(T);
+    const double ps = mixed ? 4.0 : 8.0;
+    timed("gemm", ps * C * nq + 16.0 * C * T.n, 2.0 * C * nq * T.n, [&] {
+      dev::launch_gemm(st, C, psi, tpsi, C, nq, Td, T.n, 1.0, 1.0, K, C, bmask, mixed);
+    });
+  }
+  // arena[off : off + m*nq] = Xa^T psi (the psi contractions; FP32 in the mixed form)
+  size_t tn_psi(const double* Xa, int m) {
+    const size_t off = aalloc((size_t)m * nq);
+    const double ps = mixed ? 4.0 : 8.0;
+    timed("gram", 8.0 * C * m + ps * C * nq, 2.0 * C * m * nq, [&] {
+      dev::launch_tn(st, g, Xa, dev::DT::F64, C, m, psi, tpsi, C, nq, dev::OP_NONE, mixed, partial,
+                     arena + off);
+    });
+    return off;
   }
 
   void step_fixed(int s) {
@@ -506,11 +528,11 @@ struct titus_ctx {
     out_scatter(Dout);
     // in-scattering (:1079-1102)
     areset();
-    const size_t ptx = tn(psi, nq, X, r, dev::OP_NONE);     // psi^T X, old X
+    const size_t xtp0 = tn_psi(X, r);                       // X^T psi, old X
     scatter_K(mtdw(Dvec, W));
     RangeJob j = range_launch(K, r, nullptr, 0);
     fetch();
-    HMat PtX = view(ptx, nq, r);
+    HMat PtX = titus::transpose(view(xtp0, r, nq));
     range_finish(j, r, Xn);
     HMat L = titus::matmul(W, S, false, true);
     HMat MP = titus::matmul(M, PtX);
@@ -520,7 +542,7 @@ struct titus_ctx {
     HMat NUp = titus::matmul(WNew, W, true, false);
     areset();
     const size_t mu = tn(Xn, r, X, r, dev::OP_NONE);
-    const size_t xtp = tn(Xn, r, psi, nq, dev::OP_NONE);
+    const size_t xtp = tn_psi(Xn, r);
     fetch();
     HMat MUp = view(mu, r, r);
     HMat XtP = view(xtp, r, nq);
@@ -606,7 +628,7 @@ struct titus_ctx {
     // in-scattering (:2252-2306)
     const int r = rank;
     areset();
-    const size_t xtp = tn(X, r, psi, nq, dev::OP_NONE);       // old X
+    const size_t xtp = tn_psi(X, r);                          // old X
     scatter_K(mtdw(Dvec, W));
     copy_x_to_aug(r);
     RangeJob j = range_launch(K, r, X, r);
@@ -621,7 +643,7 @@ struct titus_ctx {
     HMat Nu = titus::matmul(W1, W, true, false);
     areset();
     const size_t mu = tn(Xn, 2 * r, X, r, dev::OP_NONE);
-    const size_t x1p = tn(Xn, 2 * r, psi, nq, dev::OP_NONE);
+    const size_t x1p = tn_psi(Xn, 2 * r);
     fetch();
     HMat Mu = view(mu, 2 * r, r);
     HMat S1 = titus::matmul(titus::matmul(Mu, S), Nu, false, true);
@@ -690,7 +712,8 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
                 p->cand_cells, "missing operand");
     require(p->integrator == TITUS_INTEGRATOR_FIXED || p->integrator == TITUS_INTEGRATOR_BUG,
             "unknown integrator");
-    require(p->precision == TITUS_PRECISION_FP64, "only the FP64 form is available in this build");
+    require(p->precision == TITUS_PRECISION_FP64 || p->precision == TITUS_PRECISION_MIXED,
+            "unknown precision form");
     require(p->r >= 1, "rank must be >= 1");
     const int C = p->nx * p->ny;
     for (int c = 0; c < C; ++c)
@@ -713,6 +736,8 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     HIPCHK(hipStreamCreateWithFlags(&x.st, hipStreamNonBlocking));
     x.nx = p->nx; x.ny = p->ny; x.C = C; x.nm = p->nm; x.nq = p->nq; x.ne = p->ne;
     x.nsig = p->nsigma; x.integrator = p->integrator; x.precision = p->precision;
+    x.mixed = p->precision == TITUS_PRECISION_MIXED;
+    x.tpsi = x.mixed ? dev::DT::F32 : dev::DT::F64;
     x.dx = p->dx; x.dy = p->dy; x.dE = p->dE; x.theta = p->theta; x.theta_index = p->theta_index;
     x.rank_tol = p->rank_tol > 0.0 ? p->rank_tol : 1e-7;
     x.rank = p->r;
@@ -733,6 +758,10 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     x.aAx = copy_mat(p->absAx, p->nm, p->nm);
     x.aAz = copy_mat(p->absAz, p->nm, p->nm);
     x.M = copy_mat(p->M, p->nm, p->nq);
+    x.sAx[0] = titus::SpMat::from_dense(x.aAx);
+    x.sAx[1] = titus::SpMat::from_dense(x.aAz);
+    x.sAx[2] = titus::SpMat::from_dense(x.Ax);
+    x.sAx[3] = titus::SpMat::from_dense(x.Az);
     x.nb = p->nb;
 
     const size_t cx = (size_t)C * x.capx;
@@ -740,8 +769,9 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     x.omega = dalloc<double>((size_t)p->nq * 3);
     x.bc_base = dalloc<double>((size_t)p->nb * p->nq);
     x.m0 = dalloc<double>(p->nq);
-    x.psi = dalloc<double>((size_t)C * p->nq);
-    x.psi2 = dalloc<double>((size_t)C * p->nq);
+    const size_t psz = x.mixed ? sizeof(float) : sizeof(double);
+    x.psi = dalloc<unsigned char>((size_t)C * p->nq * psz);
+    x.psi2 = dalloc<unsigned char>((size_t)C * p->nq * psz);
     x.dose = dalloc<double>(C);
     x.X = dalloc<double>(cx);
     x.Xn = dalloc<double>(cx);
@@ -785,8 +815,8 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     for (int b = 0; b < p->nrestore; ++b) rest[p->restore_cells[b]] = 1;
     HIPCHK(hipMemcpy(x.bmask, mask.data(), C, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(x.restore, rest.data(), C, hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(x.psi, 0, sizeof(double) * (size_t)C * p->nq));
-    HIPCHK(hipMemset(x.psi2, 0, sizeof(double) * (size_t)C * p->nq));
+    HIPCHK(hipMemset(x.psi, 0, psz * (size_t)C * p->nq));
+    HIPCHK(hipMemset(x.psi2, 0, psz * (size_t)C * p->nq));
     HIPCHK(hipMemset(x.dose, 0, sizeof(double) * C));
     HIPCHK(hipMemset(x.X, 0, sizeof(double) * cx));
 
@@ -850,7 +880,15 @@ int titus_get_state(titus_ctx* ctx, double* dose, double* X, double* S, double* 
     const size_t C = ctx->C;
     if (dose) HIPCHK(hipMemcpy(dose, ctx->dose, sizeof(double) * C, hipMemcpyDeviceToHost));
     if (X) HIPCHK(hipMemcpy(X, ctx->X, sizeof(double) * C * ctx->rank, hipMemcpyDeviceToHost));
-    if (psi) HIPCHK(hipMemcpy(psi, ctx->psi, sizeof(double) * C * ctx->nq, hipMemcpyDeviceToHost));
+    if (psi) {
+      if (ctx->mixed) {
+        std::vector<float> f(C * ctx->nq);
+        HIPCHK(hipMemcpy(f.data(), ctx->psi, sizeof(float) * f.size(), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < f.size(); ++i) psi[i] = f[i];
+      } else {
+        HIPCHK(hipMemcpy(psi, ctx->psi, sizeof(double) * C * ctx->nq, hipMemcpyDeviceToHost));
+      }
+    }
     if (S) std::memcpy(S, ctx->S.a.data(), sizeof(double) * ctx->S.a.size());
     if (W) std::memcpy(W, ctx->W.a.data(), sizeof(double) * ctx->W.a.size());
   });
