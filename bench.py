@@ -94,6 +94,51 @@ def run_form(P, precision, W, K, events, sync):
     return dict(elapsed=elapsed, rate=K / elapsed, stats=stats, ranks=ranks, dose=dose)
 
 
+def beamlet_batch(n_beamlets, nthreads, sync):
+    """Config-4 plan: FP64 solves one after another vs the mixed batch.
+
+    Both sets of contexts are created (operands in HBM) before their timed
+    region.  Returns throughput of both, their ratio and per-beamlet dose
+    differences (mixed vs FP64) of the same plan.
+    """
+    from titus_amd import _lib
+    from titus_amd.beamlets import beamlet_plan
+    plan = beamlet_plan(n_beamlets=n_beamlets)
+    total_steps = sum(P["nE"] - 2 for _, P in plan)
+
+    def run(precision, threads):
+        ctxs = [_lib.Context(P, _lib.INTEGRATOR_BUG, precision=precision) for _, P in plan]
+        try:
+            sync()
+            t0 = time.perf_counter()
+            _lib.solve_batch(ctxs, threads)
+            sync()
+            dt = time.perf_counter() - t0
+            doses = np.stack([c.state()["dose"] for c in ctxs], axis=1)
+        finally:
+            for c in ctxs:
+                c.close()
+        return dt, doses
+
+    t64, d64 = run(_lib.PRECISION_FP64, 1)
+    tmx, dmx = run(_lib.PRECISION_MIXED, nthreads)
+    errs = np.linalg.norm(dmx - d64, axis=0) / np.linalg.norm(d64, axis=0)
+    w = np.ones(len(plan))
+    plan_err = float(np.linalg.norm((dmx - d64) @ w) / np.linalg.norm(d64 @ w))
+    return {
+        "workload": f"{n_beamlets} pencil beamlets on a 128x128 ellipse phantom (10x10 cm), "
+                    "P_15 (136 moments), 6-20 MeV, +-15 deg, rank caps 5-40, seed 0",
+        "beamlet_steps": int(total_steps),
+        "mixed_batched": {"beamlets_per_s": n_beamlets / tmx, "energy_steps_per_s": total_steps / tmx,
+                          "seconds": tmx, "host_threads": nthreads},
+        "fp64_one_at_a_time": {"beamlets_per_s": n_beamlets / t64,
+                               "energy_steps_per_s": total_steps / t64, "seconds": t64},
+        "speedup": t64 / tmx,
+        "dose_rel_l2_per_beamlet": {"max": float(errs.max()), "median": float(np.median(errs))},
+        "plan_dose_rel_l2": plan_err,
+    }
+
+
 def roofline(stats, fp32):
     dom = max(stats, key=lambda k: stats[k][0])
     ms, launches, by, fl = stats[dom]
@@ -123,6 +168,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-events", action="store_true",
                     help="time without per-kernel HIP events (no roofline)")
+    ap.add_argument("--beamlets", type=int, default=64,
+                    help="beamlets of the batched plan (0: skip)")
+    ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1),
+                    help="host workers of the beamlet batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -177,6 +226,8 @@ def main():
     if events:
         out["roofline"] = roofline(mix["stats"], fp32=True)
         out["roofline_fp64"] = roofline(f64["stats"], fp32=False)
+    if args.beamlets > 0:
+        out["beamlets"] = beamlet_batch(args.beamlets, args.threads, sync)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(P, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 

@@ -104,6 +104,15 @@ int titus_step(struct titus_ctx* ctx, int32_t step);
  * of the grid as the reference's loop does). */
 int titus_solve(struct titus_ctx* ctx, int32_t last_step);
 
+/* Run `n` independent solves (e.g. pencil beamlets of a treatment plan) to
+ * the end of their energy grids, `nthreads` host workers at a time (<= 0:
+ * one per hardware thread, at most n).  Each context keeps its own stream, so
+ * the beamlets' kernels and host algebra overlap.  status[i] (optional)
+ * receives each solve's return code; the first failure is returned.
+ * No reference counterpart: the reference solves one beam per call
+ * (code/SolverCSD.jl:891, 2011). */
+int titus_solve_batch(struct titus_ctx** ctxs, int32_t n, int32_t nthreads, int32_t* status);
+
 /* Number of steps taken so far and the current rank. */
 int titus_progress(const struct titus_ctx* ctx, int32_t* steps_done, int32_t* rank);
 

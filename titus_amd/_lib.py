@@ -24,7 +24,8 @@ ERRORS = {
 }
 
 # every symbol include/titus_hip.h declares
-EXPORTED = ("titus_create", "titus_step", "titus_solve", "titus_progress", "titus_get_state",
+EXPORTED = ("titus_create", "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
+            "titus_get_state",
             "titus_get_history", "titus_set_timing", "titus_kernel_stats", "titus_destroy",
             "titus_last_error", "titus_abi_version")
 
@@ -70,6 +71,7 @@ def load():
     lib.titus_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(ProblemV1), ctypes.c_int32]
     lib.titus_step.argtypes = [vp, ctypes.c_int32]
     lib.titus_solve.argtypes = [vp, ctypes.c_int32]
+    lib.titus_solve_batch.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32, _PI]
     lib.titus_progress.argtypes = [vp, _PI, _PI]
     lib.titus_get_state.argtypes = [vp, _P, _P, _P, _P, _P]
     lib.titus_get_history.argtypes = [vp, _PI, _P, ctypes.c_int32]
@@ -80,7 +82,7 @@ def load():
     lib.titus_destroy.restype = None
     lib.titus_last_error.restype = ctypes.c_char_p
     lib.titus_abi_version.restype = ctypes.c_int32
-    for name in ("titus_create", "titus_step", "titus_solve", "titus_progress",
+    for name in ("titus_create", "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
                  "titus_get_state", "titus_get_history", "titus_set_timing",
                  "titus_kernel_stats"):
         getattr(lib, name).restype = ctypes.c_int
@@ -141,6 +143,20 @@ def make_problem(P, integrator, precision=PRECISION_FP64, rank_tol=1e-7, initial
             keep[name] = f64(initial[name])
             setattr(s, name, _dp(keep[name]))
     return s, keep
+
+
+def solve_batch(contexts, nthreads=0):
+    """Run every context to the end of its grid concurrently (titus_solve_batch).
+
+    Returns the per-context status codes; raises on the first failure.
+    """
+    lib = load()
+    n = len(contexts)
+    arr = (ctypes.c_void_p * n)(*[c.h for c in contexts])
+    status = np.zeros(n, dtype=np.int32)
+    code = lib.titus_solve_batch(arr, n, nthreads, _ip(status))
+    check(code)
+    return status
 
 
 class Context:

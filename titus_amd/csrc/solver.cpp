@@ -27,6 +27,8 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <atomic>
+#include <thread>
 #include <vector>
 
 #include "../../include/titus_hip.h"
@@ -862,6 +864,39 @@ int titus_solve(titus_ctx* ctx, int32_t last_step) {
     for (int s = ctx->steps_done + 1; s <= last; ++s) ctx->step(s);
     HIPCHK(hipStreamSynchronize(ctx->st));
   });
+}
+
+int titus_solve_batch(titus_ctx** ctxs, int32_t n, int32_t nthreads, int32_t* status) {
+  if (!ctxs || n < 0) {
+    g_last_error = "bad argument";
+    return TITUS_E_ARG;
+  }
+  int nt = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
+  nt = std::max(1, std::min<int>(nt, n));
+  std::vector<int> codes(n, TITUS_OK);
+  std::vector<std::string> msgs(n);
+  std::atomic<int> next{0};
+  auto worker = [&] {
+    for (;;) {
+      const int i = next.fetch_add(1);
+      if (i >= n) return;
+      codes[i] = titus_solve(ctxs[i], 0);
+      if (codes[i] != TITUS_OK) msgs[i] = g_last_error;   // thread-local message
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto& th : pool) th.join();
+  int first = TITUS_OK;
+  for (int i = 0; i < n; ++i) {
+    if (status) status[i] = codes[i];
+    if (first == TITUS_OK && codes[i] != TITUS_OK) {
+      first = codes[i];
+      g_last_error = "solve " + std::to_string(i) + ": " + msgs[i];
+    }
+  }
+  return first;
 }
 
 int titus_progress(const titus_ctx* ctx, int32_t* steps_done, int32_t* rank) {

@@ -14,6 +14,8 @@ Layouts (all FP64, 0-based):
 * the uncollided flux ``psi`` is stored C x Q (cell-major within each ordinate
   column), i.e. ``Mat2Vec(psi)`` of ``utils.jl:67-78``.
 """
+import functools
+
 import numpy as np
 
 from . import physics as ph
@@ -22,18 +24,27 @@ INTEGRATOR_FIXED = 0      # SolveFirstCollisionSourceDLR
 INTEGRATOR_BUG = 1        # SolveFirstCollisionSourceDLRBUGRejection (rejection off)
 
 
-def build_problem(s, moments="subset", integrator=INTEGRATOR_FIXED):
-    nx, ny = s.NCellsX, s.NCellsY
-    C = nx * ny
-    N = s.nPN
-    csd = ph.CSDGrid(s)
+@functools.lru_cache(maxsize=8)
+def angular_operands(N, moments, cs_phase):
+    """P_N, Roe and transform matrices of degree N (shared by every beam)."""
     lk, full_idx, degree = ph.moment_set(N, moments)
     Axf, _, Azf = ph.pn_matrices_full(N)
     Ax = Axf[np.ix_(full_idx, full_idx)]
     Az = Azf[np.ix_(full_idx, full_idx)]
     absAx = ph.abs_matrix(Ax)
     absAz = ph.abs_matrix(Az)
-    O, M, pts, w = ph.trafo_matrices(N, lk, cs_phase=s.csPhase)
+    O, M, pts, w = ph.trafo_matrices(N, lk, cs_phase=cs_phase)
+    for a in (Ax, Az, absAx, absAz, O, M, pts, w):
+        a.setflags(write=False)
+    return lk, degree, Ax, Az, absAx, absAz, O, M, pts, w
+
+
+def build_problem(s, moments="subset", integrator=INTEGRATOR_FIXED):
+    nx, ny = s.NCellsX, s.NCellsY
+    C = nx * ny
+    N = s.nPN
+    csd = ph.CSDGrid(s)
+    lk, degree, Ax, Az, absAx, absAz, O, M, pts, w = angular_operands(N, moments, bool(s.csPhase))
 
     # ordinate reduction: keep directions the beam reaches (SolverCSD.jl:913-925)
     psi_beam = ph.beam_angular(s, pts)
