@@ -139,6 +139,51 @@ def beamlet_batch(n_beamlets, nthreads, sync):
     }
 
 
+def stencil_scale(sync, steps=4, n=1024, calibrate=False):
+    """HBM efficiency of the sweep and streaming-stencil kernels at scale.
+
+    A 1024x1024 homogeneous slab, P_5 with a broad beam (72 ordinates), fixed
+    rank 16: the sweep moves 2 x 72 x 8 MB per launch (FP64), large enough to
+    fill the chip.  Per kernel: achieved GB/s = algorithmic bytes / HIP-event
+    time, and the fraction of the 8 TB/s peak.  `calibrate` also runs a plain
+    256 MiB device copy (the reference kernel for the PMC traffic scaling).
+    """
+    from titus_amd import _lib
+    from titus_amd.problem import INTEGRATOR_FIXED, build_problem
+    from titus_amd.settings import Settings
+    s = Settings(n + 1, n + 1, 16, "waterBeamElectrons", "Electrons", a=0.0, b=20.0, c=0.0,
+                 d=20.0, x0=10.0, y0=0.0, nPN=5, eMin=1.0, eMax=10.5, sigmaX=2.0, sigmaY=2.0,
+                 sigmaO1Inv=2.0, sigmaO3Inv=2.0)
+    P = build_problem(s, "subset", INTEGRATOR_FIXED)
+    out = {"workload": f"{n}x{n} slab, P_5, {P['Q']} ordinates, rank 16, {steps} steps"}
+    for name, prec in (("fp64", _lib.PRECISION_FP64), ("mixed", _lib.PRECISION_MIXED)):
+        ctx = _lib.Context(P, INTEGRATOR_FIXED, precision=prec)
+        try:
+            ctx.solve(2)
+            sync()
+            ctx.set_timing(True)
+            ctx.solve(2 + steps)
+            res = {}
+            for k in ("sweep", "stream_K", "galerkin", "gemm"):
+                ms, launches, by, fl = ctx.kernel_stats(k)
+                if launches:
+                    gbs = by / (ms * 1e-3) / 1e9
+                    res[k] = {"GB/s": round(gbs, 1), "frac_of_8TB/s": round(gbs / HBM_PEAK_GBS, 3),
+                              "avg_launch_us": round(ms / launches * 1e3, 2),
+                              "bytes_per_launch": by / launches}
+            out[name] = res
+        finally:
+            ctx.close()
+    if calibrate:
+        import torch
+        a = torch.empty(256 * 2 ** 20 // 8, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+    return out
+
+
 def roofline(stats, fp32):
     dom = max(stats, key=lambda k: stats[k][0])
     ms, launches, by, fl = stats[dom]
@@ -172,6 +217,8 @@ def main():
                     help="beamlets of the batched plan (0: skip)")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="host workers of the beamlet batch")
+    ap.add_argument("--stencil-only", action="store_true",
+                    help="only the large-grid stencil measurement (+ calibration copy), for PMC runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -179,15 +226,18 @@ def main():
         sys.exit(2)
 
     from titus_amd import _lib
-    P = build_workload()
-    last = P["nE"] - 2
-    W = min(args.warmup, last - 1)
-    K = min(args.steps, last - W)
     try:
         import torch
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
     except Exception:
         sync = lambda: None
+    if args.stencil_only:
+        print(json.dumps({"stencil_scale": stencil_scale(sync, calibrate=True)}), flush=True)
+        return
+    P = build_workload()
+    last = P["nE"] - 2
+    W = min(args.warmup, last - 1)
+    K = min(args.steps, last - W)
     events = not args.no_events
     f64 = run_form(P, _lib.PRECISION_FP64, W, K, events, sync)
     mix = run_form(P, _lib.PRECISION_MIXED, W, K, events, sync)
@@ -228,6 +278,7 @@ def main():
         out["roofline_fp64"] = roofline(f64["stats"], fp32=False)
     if args.beamlets > 0:
         out["beamlets"] = beamlet_batch(args.beamlets, args.threads, sync)
+    out["stencil_scale"] = stencil_scale(sync)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(P, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
