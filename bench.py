@@ -7,8 +7,10 @@ P_21 (253 moments), rank-adaptive augmented BUG with truncation tolerance 1e-2
 (relative) and rank cap 40, energy grid by the reference's rule (823 points).
 
 Both forms run on the same workload: the plain FP64 form and the mixed form
-(FP32 uncollided sweep, streaming stencil, Galerkin and psi products; FP64
-Cholesky-QR, truncation SVD and dose sum).  `value` is the mixed form's rate;
+(FP32 psi storage and uncollided sweep; FP64 streaming stencil, Galerkin and
+psi products, Cholesky-QR, truncation SVD and dose sum).  The form with the
+streaming stencil, psi and Galerkin products in FP32 as well is run untimed
+beside them for its error.  `value` is the mixed form's rate;
 the plain form's rate, their ratio and the dose difference after the timed
 steps are reported beside it.
 
@@ -74,9 +76,9 @@ def cpu_baseline(P, seconds):
             "sample": f"FP64 numpy oracle, steps 1..{steps} of the same solve, {elapsed:.1f} s"}
 
 
-def run_form(P, precision, W, K, events, sync):
+def run_form(P, precision, W, K, events, sync, rank_tol):
     from titus_amd import _lib
-    ctx = _lib.Context(P, _lib.INTEGRATOR_BUG, precision=precision)
+    ctx = _lib.Context(P, _lib.INTEGRATOR_BUG, precision=precision, rank_tol=rank_tol)
     try:
         ctx.solve(W)
         sync()
@@ -103,11 +105,13 @@ def beamlet_batch(n_beamlets, nthreads, sync):
     """
     from titus_amd import _lib
     from titus_amd.beamlets import beamlet_plan
+    from titus_amd.solver import MIXED_RANK_TOL
     plan = beamlet_plan(n_beamlets=n_beamlets)
     total_steps = sum(P["nE"] - 2 for _, P in plan)
 
     def run(precision, threads):
-        ctxs = [_lib.Context(P, _lib.INTEGRATOR_BUG, precision=precision) for _, P in plan]
+        ctxs = [_lib.Context(P, _lib.INTEGRATOR_BUG, precision=precision, rank_tol=MIXED_RANK_TOL)
+                for _, P in plan]
         try:
             sync()
             t0 = time.perf_counter()
@@ -226,6 +230,7 @@ def main():
         sys.exit(2)
 
     from titus_amd import _lib
+    from titus_amd.solver import MIXED_RANK_TOL, RANK_TOL
     try:
         import torch
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
@@ -239,9 +244,18 @@ def main():
     W = min(args.warmup, last - 1)
     K = min(args.steps, last - W)
     events = not args.no_events
-    f64 = run_form(P, _lib.PRECISION_FP64, W, K, events, sync)
-    mix = run_form(P, _lib.PRECISION_MIXED, W, K, events, sync)
+    # both forms at the mixed form's rank cut (FP32 noise must not decide which
+    # near-null directions enter the basis); the FP64 form at the FP64 cut is
+    # run as well, untimed, to report what the cut itself changes
+    f64 = run_form(P, _lib.PRECISION_FP64, W, K, events, sync, MIXED_RANK_TOL)
+    mix = run_form(P, _lib.PRECISION_MIXED, W, K, events, sync, MIXED_RANK_TOL)
+    f64_cut = run_form(P, _lib.PRECISION_FP64, W, K, False, sync, RANK_TOL)
+    cut_err = float(np.linalg.norm(f64["dose"] - f64_cut["dose"]) / np.linalg.norm(f64_cut["dose"]))
+    g32 = run_form(P, _lib.PRECISION_MIXED_FULL_FP32, W, K, False, sync, MIXED_RANK_TOL)
+    g32_err = float(np.linalg.norm(g32["dose"] - f64["dose"]) / np.linalg.norm(f64["dose"]))
     derr = float(np.linalg.norm(mix["dose"] - f64["dose"]) / np.linalg.norm(f64["dose"]))
+    n_r = min(len(mix["ranks"]), len(f64["ranks"]))
+    diff = np.nonzero(np.asarray(mix["ranks"][:n_r]) != np.asarray(f64["ranks"][:n_r]))[0]
 
     out = {
         "metric": METRIC,
@@ -254,7 +268,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "mixed (f32 sweep/stencil/Galerkin/psi products, f64 QR/SVD/dose)",
+        "dtype": "mixed (f32 psi storage and uncollided sweep; f64 stencils, Galerkin, QR/SVD, dose)",
         "data": "synthetic (seeded ellipse phantom, analytic beam source)",
         "config": {
             "workload": "2D heterogeneous phantom 200x200 over 10x10 cm, 15 MeV e-, P_21 "
@@ -269,6 +283,11 @@ def main():
                  "device_ms_per_step": sum(v[0] for v in f64["stats"].values()) / K},
         "mixed_over_fp64": mix["rate"] / f64["rate"],
         "dose_rel_l2_mixed_vs_fp64": derr,
+        "rank_history_mixed_vs_fp64": {"equal": bool(len(diff) == 0),
+                                       "first_difference_step": int(diff[0]) + 1 if len(diff) else None},
+        "rank_tol": MIXED_RANK_TOL,
+        "mixed_full_fp32": {"value": g32["rate"], "dose_rel_l2_vs_fp64": g32_err},
+        "fp64_dose_rel_l2_rank_tol_1e-4_vs_1e-7": cut_err,
         "device_ms_per_step": sum(v[0] for v in mix["stats"].values()) / K,
         "kernel_ms": {k: round(v[0], 3) for k, v in mix["stats"].items()},
         "events_in_timed_region": events,

@@ -45,7 +45,14 @@ enum {
 };
 
 enum { TITUS_INTEGRATOR_FIXED = 0, TITUS_INTEGRATOR_BUG = 1 };
-enum { TITUS_PRECISION_FP64 = 0, TITUS_PRECISION_MIXED = 1 };
+/* FP64: every operation in FP64.
+ * MIXED: the uncollided flux psi stored and swept in FP32 (half the HBM
+ *   traffic of the dominant kernel); everything else in FP64.
+ * MIXED_FULL_FP32: MIXED plus the streaming stencil, the psi contractions and
+ *   the Galerkin products X^T L X in FP32 with FP32 accumulation.  The
+ *   rank-adaptive solve amplifies their rounding on heterogeneous phantoms
+ *   (dose ~1e-3 from FP64), so it is not the default mixed form. */
+enum { TITUS_PRECISION_FP64 = 0, TITUS_PRECISION_MIXED = 1, TITUS_PRECISION_MIXED_FULL_FP32 = 2 };
 
 /* Problem description: the operands SolverCSD(settings) builds plus the
  * ordinate reduction and initial factors of the Solve* functions

@@ -27,6 +27,11 @@ Deliberate, documented choices where the reference is not deterministic:
   it -- which is what LAPACK returns for an exactly zero matrix.
 * ``truncate!`` keeps singular values below ``RANK_TOL * sigma_1`` as exact
   zeros and completes their singular vectors the same way.
+* The greedy completion (angular bases, truncation SVD) takes the candidate of
+  largest residual; residuals within ``COMPLETION_TIE`` (relative) of the
+  largest are ties, won by the lowest index.  Exactly tied residuals are
+  common (canonical vectors orthogonal to everything so far), and a strict
+  argmax would let rounding-level perturbations pick among them.
 
 Parity status of each function is listed in the module docstrings of
 ``tests/test_oracle_*.py``.
@@ -34,6 +39,7 @@ Parity status of each function is listed in the module docstrings of
 import numpy as np
 
 RANK_TOL = 1e-7
+COMPLETION_TIE = 1e-6
 
 
 # ---------------------------------------------------------------------------
@@ -90,7 +96,8 @@ def range_basis(A, ncols, candidates, against=None, scale=None, tol=None, greedy
     kept so far.  ``greedy=False`` (spatial bases, many candidates): take the
     candidates in order, skipping one whose residual norm^2 is <= 1/2.
     ``greedy=True`` (small bases): take the candidate of largest residual,
-    lowest index on ties.  Returns an m x ncols orthonormal matrix.
+    lowest index among those within ``COMPLETION_TIE`` of it.  Returns an
+    m x ncols orthonormal matrix.
     """
     if tol is None:
         tol = RANK_TOL
@@ -133,9 +140,12 @@ def range_basis(A, ncols, candidates, against=None, scale=None, tol=None, greedy
             for _ in range(2):
                 R = R - base @ (base.T @ R)
             nr = np.einsum("ij,ij->j", R, R)
-            t = int(np.argmax(nr))
-            if nr[t] <= 1e-8:
+            top = nr.max() if nr.size else 0.0
+            if top <= 1e-8:
                 break
+            # residuals within COMPLETION_TIE (relative) of the largest are
+            # ties, won by the lowest index
+            t = int(np.nonzero(nr >= top * (1.0 - COMPLETION_TIE))[0][0])
             v = (R[:, t] / np.sqrt(nr[t]))[:, None]
             cols.append(v)
             base = np.hstack([base, v])

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libtitus_hip.so")
 
 ABI_VERSION = 1
 INTEGRATOR_FIXED, INTEGRATOR_BUG = 0, 1
-PRECISION_FP64, PRECISION_MIXED = 0, 1
+PRECISION_FP64, PRECISION_MIXED, PRECISION_MIXED_FULL_FP32 = 0, 1, 2
 
 _P = ctypes.POINTER(ctypes.c_double)
 _PI = ctypes.POINTER(ctypes.c_int32)

@@ -14,6 +14,7 @@ import numpy as np
 from . import _lib
 from .problem import INTEGRATOR_BUG, build_problem
 from .settings import Settings, ellipse_phantom
+from .solver import MIXED_RANK_TOL, RANK_TOL
 
 
 def beamlet_plan(n_beamlets=64, n=128, N=15, seed=0, rank_range=(5, 40), e_range=(6.0, 20.0),
@@ -48,9 +49,16 @@ def beamlet_plan(n_beamlets=64, n=128, N=15, seed=0, rank_range=(5, 40), e_range
     return plan
 
 
-def solve_plan(plan, precision=_lib.PRECISION_MIXED, nthreads=0, device=0):
-    """Solve every beamlet concurrently; returns (doses C x B, steps per beamlet)."""
-    ctxs = [_lib.Context(P, INTEGRATOR_BUG, precision=precision, device=device) for _, P in plan]
+def solve_plan(plan, precision=_lib.PRECISION_MIXED, nthreads=0, device=0, rank_tol=None):
+    """Solve every beamlet concurrently; returns (doses C x B, steps per beamlet).
+
+    ``rank_tol`` None: the default cut of the precision (``solver.RANK_TOL`` or
+    ``solver.MIXED_RANK_TOL``).
+    """
+    if rank_tol is None:
+        rank_tol = MIXED_RANK_TOL if precision == _lib.PRECISION_MIXED else RANK_TOL
+    ctxs = [_lib.Context(P, INTEGRATOR_BUG, precision=precision, device=device, rank_tol=rank_tol)
+            for _, P in plan]
     try:
         _lib.solve_batch(ctxs, nthreads)
         doses = np.stack([c.state()["dose"] for c in ctxs], axis=1)

@@ -27,22 +27,32 @@ struct Grid {
   int nx, ny, C;
   double dx, dy;
   const double* rho;            // C
+  const double* irho;           // C, 1 / rho
 };
 
-// psi[bcells[b] + q*C] = bc_base[b + q*nb] * scale     (SolverCSD.jl:977-988)
-void launch_bc(hipStream_t st, void* psi, DT tpsi, int C, int nq, const int* bcells, int nb,
-               const double* bc_base, double scale);
+// The l = 0 moment u0[c] = sum_q psi[c, q] m0[q] is kept as G partial sums
+// u0p[g*C + c] (g < G = sweep_groups(...), at most kMaxSweepGroups), written
+// by the sweep and refreshed at the boundary cells by launch_bc; the dose
+// adds them up in order g = 0..G-1.
+constexpr int kMaxSweepGroups = 8;
+int sweep_groups(const Grid& g, int nq, DT tpsi);
 
-// dose[c] += coef * (X[c,:] . v + psi[c,:] . m0) / rho[c]   (SolverCSD.jl:990-997), FP64
+// psi[bcells[b] + q*C] = bc_base[b + q*nb] * scale     (SolverCSD.jl:977-988),
+// and u0p of the boundary cells (partial 0 = their moment, the others 0)
+void launch_bc(hipStream_t st, void* psi, DT tpsi, int C, int nq, const int* bcells, int nb,
+               const double* bc_base, double scale, const double* m0, double* u0p, int G);
+
+// dose[c] += coef * (X[c,:] . v + u0[c]) / rho[c]   (SolverCSD.jl:990-997), FP64
 void launch_dose(hipStream_t st, const Grid& g, double* dose, const double* X, int ldx, int r,
-                 const double* v, const void* psi, DT tpsi, int nq, const double* m0, double coef);
+                 const double* v, const double* u0p, int G, double coef);
 
 // Uncollided upwind update with the boundary restore (SolverCSD.jl:388-463,
 // 1000-1005): pout = (pin - dE * flux(pin/rho)) / denom on interior cells,
 // pin / denom on ghost cells, pin itself for ordinate 0 of `restore` cells.
-// Computes in the storage type of psi.
+// Computes in the storage type of psi; writes the moment partials of pout.
 void launch_sweep(hipStream_t st, const Grid& g, const void* pin, void* pout, DT tpsi, int nq,
-                  const double* omega, double dE, double denom, const unsigned char* restore);
+                  const double* omega, double dE, double denom, const unsigned char* restore,
+                  const double* m0, double* u0p);
 
 // Streaming K-step (SolverCSD.jl:1020-1027) with the stencils applied matrix
 // free: K[c,:] = X[c,:] Z0 - dE * sum_{d in {c,e,w,n,s}} X[d,:]/rho[d] Zd on
@@ -52,10 +62,12 @@ void launch_stream_k(hipStream_t st, const Grid& g, const double* X, int ldx, in
                      const double* Z, int n, double dE, double* K, int ldk, bool fp32);
 
 // Tall-skinny A^T op(B) (m x n) with deterministic two-stage reduction.
-// op = OP_NONE: B itself; otherwise the stencil operator(s) applied to B (the
-// Galerkin products X^T L X of SolverCSD.jl:1035-1038).  `fp32`: stage and
-// accumulate in FP32.  `partial` must hold tn_partial_size(C, m, n) doubles.
-size_t tn_partial_size(int C, int m, int n);
+// op = OP_NONE: B itself; OP_ALL4: the four stencil operators applied to B
+// (the Galerkin products X^T L X of SolverCSD.jl:1035-1038).  `fp32`:
+// compute and accumulate in FP32.  `partial` must hold tn_partial_size() doubles.
+// Products run on the matrix cores (16 x 16 x 4 FP64 or FP32 MFMA tiles);
+// op is OP_NONE or OP_ALL4.
+size_t tn_partial_size();
 void launch_tn(hipStream_t st, const Grid& g, const void* A, DT ta, int lda, int m, const void* B,
                DT tb, int ldb, int n, int op, bool fp32, double* partial, double* out);
 // FP64 convenience form

@@ -11,10 +11,10 @@ namespace dev {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMaxChunks = 256;      // blocks (row chunks) of the TN reductions
-constexpr int kSub = 32;             // rows staged in LDS at a time
-constexpr int kPairsPerThread = 8;   // outputs accumulated per thread
-constexpr int kPairsPerGroup = kBlock * kPairsPerThread;
+constexpr int kMaxTnSlots = 1024;    // workgroups (partial slots) of one TN reduction
+constexpr int kXcds = 8;             // accelerator complex dies of an MI355X
+constexpr int kSweepCols = 62;       // output columns of one sweep wavefront
+constexpr int kRowsKT = 256;         // rows of T staged in LDS at a time (k_gemm)
 
 inline void check_launch(const char* what) {
   hipError_t e = hipGetLastError();
@@ -31,184 +31,296 @@ __device__ inline bool interior(int c, int nx, int ny) {
 }
 
 // ---------------------------------------------------------------------------
-template <class TP>
-__global__ void k_bc(TP* __restrict__ psi, int C, int nq, const int* __restrict__ bcells, int nb,
-                     const double* __restrict__ base, double scale) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)nb * nq) return;
-  const int b = (int)(t % nb), q = (int)(t / nb);
-  psi[(size_t)q * C + bcells[b]] = (TP)(base[(size_t)q * nb + b] * scale);
+// Hardware dispatch places workgroup b on XCD b % 8.  Renumber the workgroups
+// so that each XCD runs one contiguous range of logical blocks: spatially
+// adjacent tiles then share that XCD's L2 (halo rows, density).
+__device__ inline int xcd_block(int b, int nb) {
+  const int per = nb / kXcds, rem = nb % kXcds;
+  const int x = b % kXcds, k = b / kXcds;
+  return x * per + min(x, rem) + k;
 }
 
+// psi at the boundary cells for every ordinate, and the l = 0 moment of those
+// cells: u0p[0][cell] = sum_q psi[cell, q] m0[q] (the other partials are 0).
 template <class TP>
+__global__ void k_bc(TP* __restrict__ psi, int C, int nq, const int* __restrict__ bcells, int nb,
+                     const double* __restrict__ base, double scale,
+                     const double* __restrict__ m0, double* __restrict__ u0p, int G) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int cell = bcells[b];
+  double u0 = 0.0;
+  for (int q = 0; q < nq; ++q) {
+    const TP v = (TP)(base[(size_t)q * nb + b] * scale);
+    psi[(size_t)q * C + cell] = v;
+    u0 += (double)v * m0[q];
+  }
+  u0p[cell] = u0;
+  for (int gq = 1; gq < G; ++gq) u0p[(size_t)gq * C + cell] = 0.0;
+}
+
+// dose[c] += coef * (X[c,:] v + sum_g u0p[g][c]) / rho[c]
 __global__ void k_dose(double* __restrict__ dose, const double* __restrict__ X, int ldx, int r,
-                       const double* __restrict__ v, const TP* __restrict__ psi, int C, int nq,
-                       const double* __restrict__ m0, const double* __restrict__ rho,
-                       double coef) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+                       const double* __restrict__ v, const double* __restrict__ u0p, int G, int C,
+                       const double* __restrict__ rho, double coef) {
+  const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (c >= C) return;
   double acc = 0.0;
   for (int a = 0; a < r; ++a) acc += X[(size_t)a * ldx + c] * v[a];
   double u0 = 0.0;
-  for (int q = 0; q < nq; ++q) u0 += (double)psi[(size_t)q * C + c] * m0[q];
+  for (int gq = 0; gq < G; ++gq) u0 += u0p[(size_t)gq * C + c];
   dose[c] += coef * (acc + u0) / rho[c];
 }
 
-template <class T>
-__global__ void k_sweep(const T* __restrict__ pin, T* __restrict__ pout, int nx, int ny, int nq,
-                        const double* __restrict__ omega, T dx, T dy,
-                        const double* __restrict__ rho, T dE, T denom,
-                        const unsigned char* __restrict__ restore) {
+// Uncollided upwind update.  Each wavefront owns a strip of H rows (index i)
+// by kSweepCols columns (index j, lanes 1..kSweepCols; lanes 0 and 63 carry the
+// neighbouring columns) and walks the ordinates of its group q = gq*qper ...
+// For every ordinate it loads the strip plus the one upwind halo row, so each
+// psi value is fetched once; the j-neighbours come from the adjacent lanes.
+// The new psi's l = 0 moment accumulates per cell and is written to
+// u0p[gq][c] at the end (the dose needs it after the sweep).
+template <class T, int H>
+__global__ __launch_bounds__(kBlock) void k_sweep(
+    const T* __restrict__ pin, T* __restrict__ pout, int nx, int ny, int nq, int qper,
+    const double* __restrict__ omega, T dx, T dy, const double* __restrict__ rho, T dE, T denom,
+    const unsigned char* __restrict__ restore, const double* __restrict__ m0,
+    double* __restrict__ u0p, int ntj, int nsb) {
   const int C = nx * ny;
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)C * nq) return;
-  const int c = (int)(t % C), q = (int)(t / C);
-  const T* p = pin + (size_t)q * C;
-  const T pc = p[c];
-  T out;
-  if (interior(c, nx, ny)) {
+  const int L = xcd_block(blockIdx.x, gridDim.x);
+  const int jt = L % ntj, sb = (L / ntj) % nsb, gq = L / (ntj * nsb);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i0 = (sb * (kBlock / 64) + wave) * H;
+  const int j = jt * kSweepCols - 1 + lane;
+  const bool jv = j >= 0 && j < ny;
+  const bool owner = lane >= 1 && lane <= kSweepCols && jv;
+  const bool jint = j >= 1 && j <= ny - 2;
+  // 1/rho of rows i0-1 .. i0+H of this column
+  T ir[H + 2];
+#pragma unroll
+  for (int h = 0; h < H + 2; ++h) {
+    const int i = i0 - 1 + h;
+    ir[h] = (jv && i >= 0 && i < nx) ? (T)(1.0 / rho[(size_t)i * ny + j]) : T(0);
+  }
+  double acc[H];
+#pragma unroll
+  for (int h = 0; h < H; ++h) acc[h] = 0.0;
+  const int qa = gq * qper, qb = min(nq, qa + qper);
+  // loads of ordinate q: strip rows into P[1..H], upwind halo into P[0] or P[H+1]
+  auto load = [&](int q, T* P) {
+    const T* p = pin + (size_t)q * C;
+    const bool fwd = (T)omega[q] >= T(0);       // the flux's own upwind test
+#pragma unroll
+    for (int h = 0; h < H + 2; ++h) {
+      const int i = i0 - 1 + h;
+      const bool need = (h >= 1 && h <= H) || (h == 0 && fwd) || (h == H + 1 && !fwd);
+      P[h] = (need && jv && i >= 0 && i < nx) ? p[(size_t)i * ny + j] : T(0);
+    }
+  };
+  T P[H + 2], Pn[H + 2] = {};
+  if (qa < qb) load(qa, P);
+  for (int q = qa; q < qb; ++q) {
+    if (q + 1 < qb) load(q + 1, Pn);
     const T ox = (T)omega[q], oz = (T)omega[2 * nq + q];
-    const T vc = pc / (T)rho[c];
-    const T fx = (ox >= T(0)) ? (vc - p[c - ny] / (T)rho[c - ny]) : (p[c + ny] / (T)rho[c + ny] - vc);
-    const T fy = (oz >= T(0)) ? (vc - p[c - 1] / (T)rho[c - 1]) : (p[c + 1] / (T)rho[c + 1] - vc);
-    const T flux = fx * (ox / dx) + fy * (oz / dy);
-    out = (pc - dE * flux) / denom;
-  } else {
-    out = (q == 0 && restore[c]) ? pc : pc / denom;
-  }
-  pout[t] = out;
-}
-
-// K[c,b] = X[c,:] Z0[:,b] - dE * sum_d (X[d,:]/rho[d]) Zd[:,b] (interior c)
-template <class TC>
-__global__ void k_stream_k(const double* __restrict__ X, int ldx, int r, int nx, int ny,
-                           const double* __restrict__ rho, const double* __restrict__ Z, int n,
-                           double dE, double* __restrict__ K, int ldk) {
-  const int C = nx * ny;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = blockIdx.y;
-  if (c >= C) return;
-  const double* z0 = Z + (size_t)b * r;
-  TC acc0 = 0;
-  for (int a = 0; a < r; ++a) acc0 += (TC)X[(size_t)a * ldx + c] * (TC)z0[a];
-  TC out = acc0;
-  if (interior(c, nx, ny)) {
-    const double* zc = Z + (size_t)(1 * n + b) * r;
-    const double* ze = Z + (size_t)(2 * n + b) * r;
-    const double* zw = Z + (size_t)(3 * n + b) * r;
-    const double* zn = Z + (size_t)(4 * n + b) * r;
-    const double* zs = Z + (size_t)(5 * n + b) * r;
-    TC sc = 0, se = 0, sw = 0, sn = 0, ss = 0;
-    for (int a = 0; a < r; ++a) {
-      const double* xa = X + (size_t)a * ldx;
-      sc += (TC)xa[c] * (TC)zc[a];
-      se += (TC)xa[c + ny] * (TC)ze[a];
-      sw += (TC)xa[c - ny] * (TC)zw[a];
-      sn += (TC)xa[c + 1] * (TC)zn[a];
-      ss += (TC)xa[c - 1] * (TC)zs[a];
-    }
-    const TC s = sc / (TC)rho[c] + se / (TC)rho[c + ny] + sw / (TC)rho[c - ny] +
-                 sn / (TC)rho[c + 1] + ss / (TC)rho[c - 1];
-    out = acc0 - (TC)dE * s;
-  }
-  K[(size_t)b * ldk + c] = (double)out;
-}
-
-template <class TB, class TC>
-__device__ inline TC stencil_value(const TB* __restrict__ B, int ldb, int b, int c, int op, int nx,
-                                   int ny, TC dx, TC dy, const double* __restrict__ rho) {
-  if (!interior(c, nx, ny)) return TC(0);
-  const TB* col = B + (size_t)b * ldb;
-  const TC h = TC(0.5);
-  switch (op) {
-    case OP_L1X:
-      return ((TC)col[c] / (TC)rho[c] - h * ((TC)col[c - ny] / (TC)rho[c - ny]) -
-              h * ((TC)col[c + ny] / (TC)rho[c + ny])) / dx;
-    case OP_L1Y:
-      return ((TC)col[c] / (TC)rho[c] - h * ((TC)col[c - 1] / (TC)rho[c - 1]) -
-              h * ((TC)col[c + 1] / (TC)rho[c + 1])) / dy;
-    case OP_L2X:
-      return ((TC)col[c + ny] / (TC)rho[c + ny] - (TC)col[c - ny] / (TC)rho[c - ny]) / (TC(2) * dx);
-    default:
-      return ((TC)col[c + 1] / (TC)rho[c + 1] - (TC)col[c - 1] / (TC)rho[c - 1]) / (TC(2) * dy);
-  }
-}
-
-// partial[chunk][o*mn + p] = sum over the chunk's rows of A[c, a] * op_o(B)[c, b],
-// p = a + b*m.  NOPS = 1: the single operator `op`; NOPS = 4: the four
-// stencils L1x, L1y, L2x, L2y at once (A staged once, B's images side by side).
-// Staging and accumulation in TC; partials written in FP64.
-template <int NOPS, class TA, class TB, class TC>
-__global__ void k_tn_partial(const TA* __restrict__ A, int lda, int m, const TB* __restrict__ B,
-                             int ldb, int n, int C, int op, int nx, int ny, double dx_, double dy_,
-                             const double* __restrict__ rho, int rows_per_chunk,
-                             double* __restrict__ partial) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  TC* As = reinterpret_cast<TC*>(lds_raw);          // m x (kSub+1)
-  TC* Bs = As + (size_t)m * (kSub + 1);              // NOPS x n x (kSub+1)
-  const TC dx = (TC)dx_, dy = (TC)dy_;
-  const int chunk = blockIdx.x;
-  const int c0 = chunk * rows_per_chunk;
-  const int c1 = min(C, c0 + rows_per_chunk);
-  const int mn = m * n;
-  const int tid = threadIdx.x;
-  int pa[kPairsPerThread], pb[kPairsPerThread];
-  TC acc[NOPS][kPairsPerThread];
-  for (int k = 0; k < kPairsPerThread; ++k) {
-    const int p = blockIdx.y * kPairsPerGroup + k * kBlock + tid;
-    pa[k] = p < mn ? p % m : -1;
-    pb[k] = p < mn ? p / m : -1;
-    for (int o = 0; o < NOPS; ++o) acc[o][k] = TC(0);
-  }
-  for (int s0 = c0; s0 < c1; s0 += kSub) {
-    const int rows = min(kSub, c1 - s0);
-    for (int e = tid; e < kSub * m; e += kBlock) {
-      const int rr = e % kSub, a = e / kSub;
-      As[a * (kSub + 1) + rr] = rr < rows ? (TC)A[(size_t)a * lda + s0 + rr] : TC(0);
-    }
-    for (int e = tid; e < kSub * n; e += kBlock) {
-      const int rr = e % kSub, b = e / kSub;
-      const int c = s0 + rr;
-      if (NOPS == 1) {
-        TC v = TC(0);
-        if (rr < rows)
-          v = (op == OP_NONE) ? (TC)B[(size_t)b * ldb + c]
-                              : stencil_value<TB, TC>(B, ldb, b, c, op, nx, ny, dx, dy, rho);
-        Bs[b * (kSub + 1) + rr] = v;
-      } else {
-        TC v[4] = {TC(0), TC(0), TC(0), TC(0)};
-        if (rr < rows && interior(c, nx, ny)) {
-          const TB* col = B + (size_t)b * ldb;
-          const TC vc = (TC)col[c] / (TC)rho[c];
-          const TC vw = (TC)col[c - ny] / (TC)rho[c - ny], ve = (TC)col[c + ny] / (TC)rho[c + ny];
-          const TC vs = (TC)col[c - 1] / (TC)rho[c - 1], vn = (TC)col[c + 1] / (TC)rho[c + 1];
-          const TC h = TC(0.5);
-          v[OP_L1X] = (vc - h * vw - h * ve) / dx;
-          v[OP_L1Y] = (vc - h * vs - h * vn) / dy;
-          v[OP_L2X] = (ve - vw) / (TC(2) * dx);
-          v[OP_L2Y] = (vn - vs) / (TC(2) * dy);
+    const T cx = ox / dx, cz = oz / dy;
+    const double mq = m0[q];
+    T* po = pout + (size_t)q * C;
+    T v[H + 2];
+#pragma unroll
+    for (int h = 0; h < H + 2; ++h) v[h] = P[h] * ir[h];
+#pragma unroll
+    for (int h = 1; h <= H; ++h) {
+      const int i = i0 - 1 + h;
+      const T vw = __shfl_up(v[h], 1, 64);
+      const T ve = __shfl_down(v[h], 1, 64);
+      if (owner && i < nx) {
+        const size_t c = (size_t)i * ny + j;
+        T out;
+        if (jint && i >= 1 && i <= nx - 2) {
+          const T fx = (ox >= T(0)) ? (v[h] - v[h - 1]) : (v[h + 1] - v[h]);
+          const T fy = (oz >= T(0)) ? (v[h] - vw) : (ve - v[h]);
+          const T flux = fx * cx + fy * cz;
+          out = (P[h] - dE * flux) / denom;
+        } else {
+          out = (q == 0 && restore[c]) ? P[h] : P[h] / denom;
         }
-        for (int o = 0; o < NOPS; ++o) Bs[((size_t)o * n + b) * (kSub + 1) + rr] = v[o];
+        po[c] = out;
+        acc[h - 1] += (double)out * mq;
       }
     }
-    __syncthreads();
-    for (int k = 0; k < kPairsPerThread; ++k) {
-      if (pa[k] < 0) continue;
-      const TC* ar = As + pa[k] * (kSub + 1);
-      for (int o = 0; o < NOPS; ++o) {
-        const TC* br = Bs + ((size_t)o * n + pb[k]) * (kSub + 1);
-        TC s = acc[o][k];
-        for (int rr = 0; rr < kSub; ++rr) s += ar[rr] * br[rr];
-        acc[o][k] = s;
-      }
-    }
-    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < H + 2; ++h) P[h] = Pn[h];
   }
-  for (int k = 0; k < kPairsPerThread; ++k)
-    if (pa[k] >= 0)
-      for (int o = 0; o < NOPS; ++o)
-        partial[(size_t)chunk * NOPS * mn + (size_t)o * mn + pa[k] + (size_t)pb[k] * m] =
-            (double)acc[o][k];
+#pragma unroll
+  for (int h = 1; h <= H; ++h) {
+    const int i = i0 - 1 + h;
+    if (owner && i < nx) u0p[(size_t)gq * C + (size_t)i * ny + j] = acc[h - 1];
+  }
+}
+
+// K[c,b] = X[c,:] Z0[:,b] - dE * sum_d (X[d,:]/rho[d]) Zd[:,b] (stencil part on
+// interior c only).  One thread per cell computes NB output columns b0.., so
+// every X value is loaded once per column block; the six r x NB coefficient
+// blocks sit in LDS as Zs[(a*6 + d)*NB + k] and are read as broadcasts.
+// Ghost cells use themselves as neighbours with 1/rho = 0, so their stencil
+// sum is exactly zero and K = X Z0 there.
+template <class TC, int NB>
+__global__ __launch_bounds__(kBlock) void k_stream_k(
+    const double* __restrict__ X, int ldx, int r, int nx, int ny, const double* __restrict__ rho,
+    const double* __restrict__ Z, int n, double dE, double* __restrict__ K, int ldk) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  TC* Zs = reinterpret_cast<TC*>(lds_raw);
+  const int C = nx * ny;
+  const int b0 = blockIdx.y * NB;
+  const int nbv = min(NB, n - b0);
+  for (int e = threadIdx.x; e < r * 6 * NB; e += blockDim.x) {
+    const int k = e % NB, d = (e / NB) % 6, a = e / (6 * NB);
+    Zs[e] = k < nbv ? (TC)Z[((size_t)d * n + b0 + k) * r + a] : TC(0);
+  }
+  __syncthreads();
+  const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  const int i = c / ny, j = c - (c / ny) * ny;
+  const bool in = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
+  const int ce = in ? c + ny : c, cw = in ? c - ny : c, cn = in ? c + 1 : c, cs = in ? c - 1 : c;
+  const TC ic = in ? (TC)(1.0 / rho[c]) : TC(0), ie = in ? (TC)(1.0 / rho[ce]) : TC(0);
+  const TC iw = in ? (TC)(1.0 / rho[cw]) : TC(0), in_ = in ? (TC)(1.0 / rho[cn]) : TC(0);
+  const TC is = in ? (TC)(1.0 / rho[cs]) : TC(0);
+  TC a0[NB], as[NB];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) { a0[k] = TC(0); as[k] = TC(0); }
+  for (int a = 0; a < r; ++a) {
+    const double* xa = X + (size_t)a * ldx;
+    const TC x0 = (TC)xa[c];
+    const TC xc = x0 * ic, xe = (TC)xa[ce] * ie, xw = (TC)xa[cw] * iw;
+    const TC xn = (TC)xa[cn] * in_, xs = (TC)xa[cs] * is;
+    const TC* z = Zs + (size_t)a * 6 * NB;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      a0[k] += x0 * z[k];
+      TC s = as[k];
+      s += xc * z[NB + k];
+      s += xe * z[2 * NB + k];
+      s += xw * z[3 * NB + k];
+      s += xn * z[4 * NB + k];
+      s += xs * z[5 * NB + k];
+      as[k] = s;
+    }
+  }
+  const TC tdE = (TC)dE;
+#pragma unroll
+  for (int k = 0; k < NB; ++k)
+    if (k < nbv) K[(size_t)(b0 + k) * ldk + c] = (double)(a0[k] - tdE * as[k]);
+}
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// 16 x 16 x 4 matrix-core step D += a (16 x 4) b (4 x 16) in the compute type.
+// Operand lanes: lane l carries row / column l % 16 at K-slot l / 16.  The
+// result lane l holds column l % 16 and four rows row(l / 16, v), v = 0..3:
+// 4 (l / 16) + v for FP32, l / 16 + 4 v for FP64 (measured on gfx950).
+template <class TC> struct Mfma;
+template <> struct Mfma<double> {
+  using acc_t = f64x4;
+  __device__ static acc_t step(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int kg, int v) { return kg + 4 * v; }
+};
+template <> struct Mfma<float> {
+  using acc_t = f32x4;
+  __device__ static acc_t step(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int kg, int v) { return 4 * kg + v; }
+};
+
+// Tall-skinny A^T op(B) on the matrix cores.
+// partial[blk][o*mn + a + b*m] = sum over the cells of workgroup column blk of
+// A[c, a] * op_o(B)[c, b] (mn = m*n).  NOPS = 1: op(B) = B; NOPS = 4: the
+// stencils L1x, L1y, L2x, L2y of B/rho (zero on ghost cells) side by side.
+// A workgroup is four wavefronts on one 16 x 16 tile (blockIdx.y) of the
+// output, each over its own run of `cpw` cells; K-slot k of MFMA step s is
+// cell c0 + 4k + s, so a lane walks four consecutive cells of its column.
+// The four wavefronts' sums are added in LDS in a fixed order.
+template <int NOPS, class TB, class TC>
+__global__ __launch_bounds__(kBlock) void k_tn_mfma(
+    const double* __restrict__ A, int lda, int m, const TB* __restrict__ B, int ldb, int n,
+    int C, int nx, int ny, double idx_, double idy_, const double* __restrict__ irho, int cpw,
+    int tiles_m, double* __restrict__ partial) {
+  using MF = Mfma<TC>;
+  using acc_t = typename MF::acc_t;
+  constexpr int NACC = NOPS == 1 ? 2 : NOPS;       // two chains for the plain product
+  __shared__ acc_t red[3][NACC][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tm = blockIdx.y % tiles_m, tn = blockIdx.y / tiles_m;
+  const int col = lane & 15, kg = lane >> 4;
+  const int ca = tm * 16 + col, cb = tn * 16 + col;
+  const bool va = ca < m, vb = cb < n;
+  const double* Acol = A + (size_t)(va ? ca : 0) * lda;
+  const TB* Bcol = B + (size_t)(vb ? cb : 0) * ldb;
+  const int cbeg = (blockIdx.x * 4 + wave) * cpw;
+  const int cend = min(C, cbeg + cpw);
+  const TC idx = (TC)idx_, idy = (TC)idy_, h = TC(0.5);
+  acc_t acc[NACC];
+#pragma unroll
+  for (int o = 0; o < NACC; ++o) acc[o] = acc_t{};
+  for (int c0 = cbeg; c0 < cend; c0 += 16) {
+    const int cs = c0 + 4 * kg;
+    TC av[4], bv[NOPS][4];
+    int ii = cs / ny, jj = cs - (cs / ny) * ny;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int c = cs + s;
+      const bool live = c < cend;
+      av[s] = (va && live) ? (TC)Acol[c] : TC(0);
+      if (NOPS == 1) {
+        bv[0][s] = (vb && live) ? (TC)Bcol[c] : TC(0);
+      } else {
+        while (jj >= ny) { jj -= ny; ++ii; }
+        const bool in = vb && live && ii >= 1 && ii <= nx - 2 && jj >= 1 && jj <= ny - 2;
+        TC vc = TC(0), vw = TC(0), ve = TC(0), vs = TC(0), vn = TC(0);
+        if (in) {
+          vc = (TC)Bcol[c] * (TC)irho[c];
+          vw = (TC)Bcol[c - ny] * (TC)irho[c - ny];
+          ve = (TC)Bcol[c + ny] * (TC)irho[c + ny];
+          vs = (TC)Bcol[c - 1] * (TC)irho[c - 1];
+          vn = (TC)Bcol[c + 1] * (TC)irho[c + 1];
+        }
+        bv[OP_L1X % NOPS][s] = (vc - h * vw - h * ve) * idx;
+        bv[OP_L1Y % NOPS][s] = (vc - h * vs - h * vn) * idy;
+        bv[OP_L2X % NOPS][s] = (ve - vw) * (h * idx);
+        bv[OP_L2Y % NOPS][s] = (vn - vs) * (h * idy);
+        ++jj;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int o = 0; o < NOPS; ++o) {
+        const int t = NOPS == 1 ? (s & 1) : o;
+        acc[t] = MF::step(av[s], bv[o][s], acc[t]);
+      }
+  }
+  if (wave > 0)
+#pragma unroll
+    for (int o = 0; o < NACC; ++o) red[wave - 1][o][lane] = acc[o];
+  __syncthreads();
+  if (wave != 0) return;
+#pragma unroll
+  for (int w = 0; w < 3; ++w)
+#pragma unroll
+    for (int o = 0; o < NACC; ++o) acc[o] += red[w][o][lane];
+  if (NOPS == 1) acc[0] += acc[1];
+  const size_t mn = (size_t)m * n;
+#pragma unroll
+  for (int o = 0; o < NOPS; ++o)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int a = tm * 16 + MF::row(kg, v);
+      if (a < m && vb)
+        partial[((size_t)blockIdx.x * NOPS + o) * mn + a + (size_t)cb * m] = (double)acc[o][v];
+    }
 }
 
 // One wavefront per output: lane l sums chunks l, l+64, ...; then a fixed
@@ -291,21 +403,51 @@ __global__ void k_chol_inv(const double* __restrict__ G, int k, int allow_plain,
   if (tid == 0) status[0] = good ? (double)(attempt + 1) : -1.0;
 }
 
-template <class TA, class TC>
-__global__ void k_gemm(int C, const TA* __restrict__ A, int lda, int k,
-                       const double* __restrict__ T, double alpha, double beta,
-                       double* __restrict__ out, int ldo, const unsigned char* __restrict__ mask) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = blockIdx.y;
+// out[c, b0+k] (k < NB) from one thread per cell: A[c, :] is loaded once per
+// column block and multiplied into NB accumulators; T's rows a0..a0+kRowsKT
+// are staged in LDS as Ts[a*NB + k] (broadcast reads).  Sums run over a in
+// ascending order.
+template <class TA, class TC, int NB>
+__global__ __launch_bounds__(kBlock) void k_gemm(int C, const TA* __restrict__ A, int lda, int k,
+                                                 const double* __restrict__ T, int n, double alpha,
+                                                 double beta, double* __restrict__ out, int ldo,
+                                                 const unsigned char* __restrict__ mask) {
+  __shared__ TC Ts[kRowsKT * NB];
+  const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int b0 = blockIdx.y * NB;
+  const int nbv = min(NB, n - b0);
+  TC acc[NB];
+#pragma unroll
+  for (int kk = 0; kk < NB; ++kk) acc[kk] = TC(0);
+  for (int a0 = 0; a0 < k; a0 += kRowsKT) {
+    const int ka = min(kRowsKT, k - a0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < kRowsKT * NB; e += blockDim.x) {
+      const int kk = e % NB, a = e / NB;
+      Ts[e] = (a < ka && kk < nbv) ? (TC)T[(size_t)(b0 + kk) * k + a0 + a] : TC(0);
+    }
+    __syncthreads();
+    if (c < C) {
+      const TA* ap = A + (size_t)a0 * lda + c;
+#pragma unroll 4
+      for (int a = 0; a < ka; ++a) {
+        const TC x = (TC)ap[(size_t)a * lda];
+        const TC* t = Ts + a * NB;
+#pragma unroll
+        for (int kk = 0; kk < NB; ++kk) acc[kk] += x * t[kk];
+      }
+    }
+  }
   if (c >= C) return;
-  const double* t = T + (size_t)b * k;
-  TC acc = 0;
-  for (int a = 0; a < k; ++a) acc += (TC)A[(size_t)a * lda + c] * (TC)t[a];
-  double* o = out + (size_t)b * ldo + c;
-  double v = alpha * (double)acc;
-  if (beta != 0.0) v += beta * (*o);
-  if (mask && mask[c]) v = 0.0;
-  *o = v;
+  const bool zero = mask && mask[c];
+#pragma unroll
+  for (int kk = 0; kk < NB; ++kk) {
+    if (kk >= nbv) break;
+    double* o = out + (size_t)(b0 + kk) * ldo + c;
+    double v = alpha * (double)acc[kk];
+    if (beta != 0.0) v += beta * (*o);
+    *o = zero ? 0.0 : v;
+  }
 }
 
 __global__ void k_gather_rows(const double* __restrict__ A, int lda, int ncols,
@@ -324,26 +466,25 @@ __global__ void k_scatter_add(double* __restrict__ A, int lda, const int* __rest
   A[(size_t)cols[t] * lda + rows[t]] += vals[t];
 }
 
-template <int NOPS, class TA, class TB, class TC>
-void tn_partial_launch(hipStream_t st, dim3 grid, size_t lds, const void* A, int lda, int m,
-                       const void* B, int ldb, int n, const Grid& g, int op, int rpc,
-                       double* partial) {
-  hipLaunchKernelGGL((k_tn_partial<NOPS, TA, TB, TC>), grid, dim3(kBlock), lds, st,
-                     static_cast<const TA*>(A), lda, m, static_cast<const TB*>(B), ldb, n, g.C, op,
-                     g.nx, g.ny, g.dx, g.dy, g.rho, rpc, partial);
+template <int NOPS, class TB, class TC>
+void tn_mfma_launch(hipStream_t st, dim3 grid, const void* A, int lda, int m, const void* B,
+                    int ldb, int n, const Grid& g, int cpw, int tiles_m, double* partial) {
+  hipLaunchKernelGGL((k_tn_mfma<NOPS, TB, TC>), grid, dim3(kBlock), 0, st,
+                     static_cast<const double*>(A), lda, m, static_cast<const TB*>(B), ldb, n, g.C,
+                     g.nx, g.ny, 1.0 / g.dx, 1.0 / g.dy, g.irho, cpw, tiles_m, partial);
 }
 
 template <int NOPS>
-void tn_dispatch(hipStream_t st, dim3 grid, size_t lds, const void* A, DT ta, int lda, int m,
-                 const void* B, DT tb, int ldb, int n, const Grid& g, int op, bool fp32, int rpc,
+void tn_dispatch(hipStream_t st, dim3 grid, const void* A, DT ta, int lda, int m, const void* B,
+                 DT tb, int ldb, int n, const Grid& g, bool fp32, int cpw, int tiles_m,
                  double* partial) {
   if (ta != DT::F64) throw std::runtime_error("tn: the left block is always FP64");
   if (tb == DT::F64) {
-    if (fp32) tn_partial_launch<NOPS, double, double, float>(st, grid, lds, A, lda, m, B, ldb, n, g, op, rpc, partial);
-    else tn_partial_launch<NOPS, double, double, double>(st, grid, lds, A, lda, m, B, ldb, n, g, op, rpc, partial);
+    if (fp32) tn_mfma_launch<NOPS, double, float>(st, grid, A, lda, m, B, ldb, n, g, cpw, tiles_m, partial);
+    else tn_mfma_launch<NOPS, double, double>(st, grid, A, lda, m, B, ldb, n, g, cpw, tiles_m, partial);
   } else {
-    if (fp32) tn_partial_launch<NOPS, double, float, float>(st, grid, lds, A, lda, m, B, ldb, n, g, op, rpc, partial);
-    else tn_partial_launch<NOPS, double, float, double>(st, grid, lds, A, lda, m, B, ldb, n, g, op, rpc, partial);
+    if (fp32) tn_mfma_launch<NOPS, float, float>(st, grid, A, lda, m, B, ldb, n, g, cpw, tiles_m, partial);
+    else tn_mfma_launch<NOPS, float, double>(st, grid, A, lda, m, B, ldb, n, g, cpw, tiles_m, partial);
   }
 }
 
@@ -351,85 +492,115 @@ void tn_dispatch(hipStream_t st, dim3 grid, size_t lds, const void* A, DT ta, in
 
 // ---------------------------------------------------------------------------
 void launch_bc(hipStream_t st, void* psi, DT tpsi, int C, int nq, const int* bcells, int nb,
-               const double* bc_base, double scale) {
-  const size_t n = (size_t)nb * nq;
-  if (n == 0) return;
+               const double* bc_base, double scale, const double* m0, double* u0p, int G) {
+  if (nb == 0) return;
   if (tpsi == DT::F64)
-    hipLaunchKernelGGL(k_bc<double>, dim3(blocks_for(n)), dim3(kBlock), 0, st,
-                       static_cast<double*>(psi), C, nq, bcells, nb, bc_base, scale);
+    hipLaunchKernelGGL(k_bc<double>, dim3(blocks_for(nb)), dim3(kBlock), 0, st,
+                       static_cast<double*>(psi), C, nq, bcells, nb, bc_base, scale, m0, u0p, G);
   else
-    hipLaunchKernelGGL(k_bc<float>, dim3(blocks_for(n)), dim3(kBlock), 0, st,
-                       static_cast<float*>(psi), C, nq, bcells, nb, bc_base, scale);
+    hipLaunchKernelGGL(k_bc<float>, dim3(blocks_for(nb)), dim3(kBlock), 0, st,
+                       static_cast<float*>(psi), C, nq, bcells, nb, bc_base, scale, m0, u0p, G);
   check_launch("bc");
 }
 
 void launch_dose(hipStream_t st, const Grid& g, double* dose, const double* X, int ldx, int r,
-                 const double* v, const void* psi, DT tpsi, int nq, const double* m0, double coef) {
-  if (tpsi == DT::F64)
-    hipLaunchKernelGGL(k_dose<double>, dim3(blocks_for(g.C)), dim3(kBlock), 0, st, dose, X, ldx, r,
-                       v, static_cast<const double*>(psi), g.C, nq, m0, g.rho, coef);
-  else
-    hipLaunchKernelGGL(k_dose<float>, dim3(blocks_for(g.C)), dim3(kBlock), 0, st, dose, X, ldx, r,
-                       v, static_cast<const float*>(psi), g.C, nq, m0, g.rho, coef);
+                 const double* v, const double* u0p, int G, double coef) {
+  hipLaunchKernelGGL(k_dose, dim3(blocks_for(g.C)), dim3(kBlock), 0, st, dose, X, ldx, r, v, u0p,
+                     G, g.C, g.rho, coef);
   check_launch("dose");
 }
 
+namespace {
+template <class T>
+constexpr int sweep_rows() { return sizeof(T) == 8 ? 8 : 16; }
+
+struct SweepShape { int ntj, nsb, qper, G; };
+
+SweepShape sweep_shape(const Grid& g, int nq, DT tpsi) {
+  const int H = tpsi == DT::F64 ? sweep_rows<double>() : sweep_rows<float>();
+  SweepShape s;
+  s.ntj = (g.ny + kSweepCols - 1) / kSweepCols;
+  s.nsb = (g.nx + (kBlock / 64) * H - 1) / ((kBlock / 64) * H);
+  // ordinate groups: enough workgroups to fill the chip (about four per CU),
+  // at most kMaxSweepGroups moment partials
+  const int base = s.ntj * s.nsb;
+  int G = std::max(1, std::min(nq, (1024 + base - 1) / base));
+  G = std::min(G, kMaxSweepGroups);
+  s.qper = (nq + G - 1) / G;
+  s.G = std::max(1, (nq + s.qper - 1) / s.qper);
+  return s;
+}
+}  // namespace
+
+int sweep_groups(const Grid& g, int nq, DT tpsi) { return sweep_shape(g, nq, tpsi).G; }
+
 void launch_sweep(hipStream_t st, const Grid& g, const void* pin, void* pout, DT tpsi, int nq,
-                  const double* omega, double dE, double denom, const unsigned char* restore) {
-  const size_t n = (size_t)g.C * nq;
+                  const double* omega, double dE, double denom, const unsigned char* restore,
+                  const double* m0, double* u0p) {
+  if (nq == 0) return;
+  const SweepShape s = sweep_shape(g, nq, tpsi);
+  const dim3 grid((unsigned)(s.ntj * s.nsb * s.G));
   if (tpsi == DT::F64)
-    hipLaunchKernelGGL(k_sweep<double>, dim3(blocks_for(n)), dim3(kBlock), 0, st,
+    hipLaunchKernelGGL((k_sweep<double, sweep_rows<double>()>), grid, dim3(kBlock), 0, st,
                        static_cast<const double*>(pin), static_cast<double*>(pout), g.nx, g.ny, nq,
-                       omega, g.dx, g.dy, g.rho, dE, denom, restore);
+                       s.qper, omega, g.dx, g.dy, g.rho, dE, denom, restore, m0, u0p, s.ntj, s.nsb);
   else
-    hipLaunchKernelGGL(k_sweep<float>, dim3(blocks_for(n)), dim3(kBlock), 0, st,
+    hipLaunchKernelGGL((k_sweep<float, sweep_rows<float>()>), grid, dim3(kBlock), 0, st,
                        static_cast<const float*>(pin), static_cast<float*>(pout), g.nx, g.ny, nq,
-                       omega, (float)g.dx, (float)g.dy, g.rho, (float)dE, (float)denom, restore);
+                       s.qper, omega, (float)g.dx, (float)g.dy, g.rho, (float)dE, (float)denom,
+                       restore, m0, u0p, s.ntj, s.nsb);
   check_launch("sweep");
 }
 
 void launch_stream_k(hipStream_t st, const Grid& g, const double* X, int ldx, int r,
                      const double* Z, int n, double dE, double* K, int ldk, bool fp32) {
   if (n == 0) return;
-  if (fp32)
-    hipLaunchKernelGGL(k_stream_k<float>, dim3(blocks_for(g.C), n), dim3(kBlock), 0, st, X, ldx, r,
-                       g.nx, g.ny, g.rho, Z, n, dE, K, ldk);
-  else
-    hipLaunchKernelGGL(k_stream_k<double>, dim3(blocks_for(g.C), n), dim3(kBlock), 0, st, X, ldx,
-                       r, g.nx, g.ny, g.rho, Z, n, dE, K, ldk);
+  // column blocks of NB = 16 (8 when r is wide, to bound the LDS footprint)
+  const bool narrow = r > 48;
+  const int NB = narrow ? 8 : 16;
+  const dim3 grid(blocks_for(g.C), (n + NB - 1) / NB);
+  const size_t lds = (size_t)r * 6 * NB * (fp32 ? sizeof(float) : sizeof(double));
+  if (fp32) {
+    if (narrow)
+      hipLaunchKernelGGL((k_stream_k<float, 8>), grid, dim3(kBlock), lds, st, X, ldx, r, g.nx,
+                         g.ny, g.rho, Z, n, dE, K, ldk);
+    else
+      hipLaunchKernelGGL((k_stream_k<float, 16>), grid, dim3(kBlock), lds, st, X, ldx, r, g.nx,
+                         g.ny, g.rho, Z, n, dE, K, ldk);
+  } else {
+    if (narrow)
+      hipLaunchKernelGGL((k_stream_k<double, 8>), grid, dim3(kBlock), lds, st, X, ldx, r, g.nx,
+                         g.ny, g.rho, Z, n, dE, K, ldk);
+    else
+      hipLaunchKernelGGL((k_stream_k<double, 16>), grid, dim3(kBlock), lds, st, X, ldx, r, g.nx,
+                         g.ny, g.rho, Z, n, dE, K, ldk);
+  }
   check_launch("stream_K");
 }
 
-static void tn_chunks(int C, int* nchunks, int* rpc) {
-  int nc = std::min(kMaxChunks, (C + kSub - 1) / kSub);
-  int r = (C + nc - 1) / nc;
-  r = ((r + kSub - 1) / kSub) * kSub;          // whole LDS sub-chunks
-  *rpc = r;
-  *nchunks = (C + r - 1) / r;
-}
-
-size_t tn_partial_size(int C, int m, int n) {
-  int nc, rpc;
-  tn_chunks(C, &nc, &rpc);
-  return (size_t)nc * m * n * 4;       // room for the four-operator form
-}
+size_t tn_partial_size() { return (size_t)kMaxTnSlots * 4 * 256; }
 
 void launch_tn(hipStream_t st, const Grid& g, const void* A, DT ta, int lda, int m, const void* B,
                DT tb, int ldb, int n, int op, bool fp32, double* partial, double* out) {
   if (m == 0 || n == 0) return;
-  int nchunks, rpc;
-  tn_chunks(g.C, &nchunks, &rpc);
-  const int mn = m * n;
-  const int groups = (mn + kPairsPerGroup - 1) / kPairsPerGroup;
+  if (op != OP_NONE && op != OP_ALL4) throw std::runtime_error("tn: unsupported operator");
   const int nops = (op == OP_ALL4) ? 4 : 1;
-  const size_t lds = (size_t)(m + nops * n) * (kSub + 1) * (fp32 ? sizeof(float) : sizeof(double));
-  if (lds > 160 * 1024) throw std::runtime_error("tn: block too wide for LDS");
-  const dim3 grid(nchunks, groups);
+  const int tiles_m = (m + 15) / 16, tiles = tiles_m * ((n + 15) / 16);
+  // cell runs of at least 64 cells (a multiple of 16) per wavefront; at most
+  // kMaxTnSlots workgroups in all, so the partials fit tn_partial_size()
+  int nchunks = std::max(1, std::min((g.C + 255) / 256, kMaxTnSlots / tiles));
+  int cpw = (g.C + 4 * nchunks - 1) / (4 * nchunks);
+  cpw = (cpw + 15) / 16 * 16;
+  nchunks = (g.C + 4 * cpw - 1) / (4 * cpw);
+  if ((size_t)nchunks * nops * m * n > tn_partial_size())
+    throw std::runtime_error("tn: output too wide for the partial buffer");
+  const dim3 grid(nchunks, tiles);
   if (nops == 4)
-    tn_dispatch<4>(st, grid, lds, A, ta, lda, m, B, tb, ldb, n, g, op, fp32, rpc, partial);
+    tn_dispatch<4>(st, grid, A, ta, lda, m, B, tb, ldb, n, g, fp32, cpw, tiles_m, partial);
   else
-    tn_dispatch<1>(st, grid, lds, A, ta, lda, m, B, tb, ldb, n, g, op, fp32, rpc, partial);
+    tn_dispatch<1>(st, grid, A, ta, lda, m, B, tb, ldb, n, g, fp32, cpw, tiles_m, partial);
   check_launch("tn_partial");
+  const int mn = m * n;
   const int per_block = kBlock / 64;
   const int outs = nops * mn;
   hipLaunchKernelGGL(k_tn_reduce, dim3((outs + per_block - 1) / per_block), dim3(kBlock), 0, st,
@@ -446,25 +617,34 @@ void launch_chol_inv(hipStream_t st, const double* G, int k, bool allow_plain, d
   check_launch("chol_inv");
 }
 
+namespace {
+// column-block width by n: 4, 8 or 16 accumulators per thread
+template <class TA, class TC>
+void gemm_nb(hipStream_t st, int C, const void* A, int lda, int k, const double* T, int n,
+             double alpha, double beta, double* out, int ldo, const unsigned char* mask) {
+  const TA* a = static_cast<const TA*>(A);
+  if (n <= 4)
+    hipLaunchKernelGGL((k_gemm<TA, TC, 4>), dim3(blocks_for(C), 1), dim3(kBlock), 0, st, C, a, lda,
+                       k, T, n, alpha, beta, out, ldo, mask);
+  else if (n <= 8)
+    hipLaunchKernelGGL((k_gemm<TA, TC, 8>), dim3(blocks_for(C), 1), dim3(kBlock), 0, st, C, a, lda,
+                       k, T, n, alpha, beta, out, ldo, mask);
+  else
+    hipLaunchKernelGGL((k_gemm<TA, TC, 16>), dim3(blocks_for(C), (n + 15) / 16), dim3(kBlock), 0,
+                       st, C, a, lda, k, T, n, alpha, beta, out, ldo, mask);
+}
+}  // namespace
+
 void launch_gemm(hipStream_t st, int C, const void* A, DT ta, int lda, int k, const double* T,
                  int n, double alpha, double beta, double* out, int ldo,
                  const unsigned char* zero_mask, bool fp32) {
   if (n == 0) return;
-  const dim3 grid(blocks_for(C), n);
   if (ta == DT::F64) {
-    if (fp32)
-      hipLaunchKernelGGL((k_gemm<double, float>), grid, dim3(kBlock), 0, st, C,
-                         static_cast<const double*>(A), lda, k, T, alpha, beta, out, ldo, zero_mask);
-    else
-      hipLaunchKernelGGL((k_gemm<double, double>), grid, dim3(kBlock), 0, st, C,
-                         static_cast<const double*>(A), lda, k, T, alpha, beta, out, ldo, zero_mask);
+    if (fp32) gemm_nb<double, float>(st, C, A, lda, k, T, n, alpha, beta, out, ldo, zero_mask);
+    else gemm_nb<double, double>(st, C, A, lda, k, T, n, alpha, beta, out, ldo, zero_mask);
   } else {
-    if (fp32)
-      hipLaunchKernelGGL((k_gemm<float, float>), grid, dim3(kBlock), 0, st, C,
-                         static_cast<const float*>(A), lda, k, T, alpha, beta, out, ldo, zero_mask);
-    else
-      hipLaunchKernelGGL((k_gemm<float, double>), grid, dim3(kBlock), 0, st, C,
-                         static_cast<const float*>(A), lda, k, T, alpha, beta, out, ldo, zero_mask);
+    if (fp32) gemm_nb<float, float>(st, C, A, lda, k, T, n, alpha, beta, out, ldo, zero_mask);
+    else gemm_nb<float, double>(st, C, A, lda, k, T, n, alpha, beta, out, ldo, zero_mask);
   }
   check_launch("gemm");
 }

@@ -319,19 +319,24 @@ bool canonical_completion(const HMat& Bc, const std::vector<int>& cand_index, in
     }
   }
   // greedy selection; also the fallback of the in-order rule when the
-  // candidates run out
+  // candidates run out.  Residuals within kCompletionTie (relative) of the
+  // largest count as tied and the lowest index wins, so that perturbations of
+  // the basis far below the tie width (FP32 arithmetic upstream) cannot change
+  // which canonical vector completes it.
   {
+    std::vector<double> pv(nc);
     while ((int)chosen.size() < need) {
-      int best = -1;
-      double bp = 1e-8;
+      double pmax = 1e-8;
       for (int t = 0; t < nc; ++t) {
-        if (used[t]) continue;
-        const double p = pivot(t);
-        if (p > bp) { bp = p; best = t; }
+        pv[t] = used[t] ? -1.0 : pivot(t);
+        if (pv[t] > pmax) pmax = pv[t];
       }
+      int best = -1;
+      for (int t = 0; t < nc; ++t)
+        if (!used[t] && pv[t] > 1e-8 && pv[t] >= pmax * (1.0 - kCompletionTie)) { best = t; break; }
       if (best < 0) return false;
       used[best] = 1;
-      accept(best, bp);
+      accept(best, pv[best]);
     }
   }
   const int k = (int)chosen.size();

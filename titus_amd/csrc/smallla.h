@@ -62,13 +62,15 @@ HMat inv_upper(const HMat& R);
 // orthogonalised against `base` and against each other.
 //   greedy == false: candidates in order, a candidate is taken when its
 //                    residual norm^2 exceeds 1/2;
-//   greedy == true : the candidate of largest residual norm^2 (lowest index
-//                    on ties) is taken each time.
+//   greedy == true : the candidate of largest residual norm^2 is taken each
+//                    time; residuals within kCompletionTie (relative) of the
+//                    largest are ties, won by the lowest index.
 // Only the rows base[c, :] of the candidates are needed: the residual Gram
 // matrix is I - Bc Bc^T.  Returns the chosen candidates and the upper factor
 // Rinv (need x need) such that the new columns are
 //     (E_chosen - base * Bc_chosen^T) * Rinv.
 // Returns false when the candidates run out.
+constexpr double kCompletionTie = 1e-6;
 bool canonical_completion(const HMat& Bc, const std::vector<int>& cand_index,
                           int need, bool greedy, std::vector<int>& chosen,
                           HMat& Rinv);

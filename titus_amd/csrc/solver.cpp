@@ -136,11 +136,14 @@ struct titus_ctx {
   HMat Ax, Az, aAx, aAz, M;
   titus::SpMat sAx[4];      // |Ax|, |Az|, Ax, Az in compressed rows
   // device buffers
-  double *rho = nullptr, *omega = nullptr, *bc_base = nullptr, *m0 = nullptr;
+  double *rho = nullptr, *irho = nullptr, *omega = nullptr, *bc_base = nullptr, *m0 = nullptr;
   void *psi = nullptr, *psi2 = nullptr;    // C x nq, FP64 (plain) or FP32 (mixed)
   dev::DT tpsi = dev::DT::F64;
-  bool mixed = false;                       // FP32 sweeps, stencils, Galerkin, psi products
+  bool mixed = false;                       // psi stored and swept in FP32
+  bool full32 = false;                      // streaming stencil, psi and Galerkin products in FP32
   double* dose = nullptr;
+  double* u0p = nullptr;                    // l = 0 moment of psi, G partials x C
+  int u0_groups = 1;
   double *X = nullptr, *Xn = nullptr, *K = nullptr, *Q1 = nullptr, *Q2 = nullptr;
   double *partial = nullptr, *arena = nullptr, *upl = nullptr, *svals = nullptr;
   int *bcells = nullptr, *cand_d = nullptr, *srows = nullptr, *scols = nullptr;
@@ -159,7 +162,7 @@ struct titus_ctx {
   Timing timing;
 
   ~titus_ctx() {
-    void* ptrs[] = {rho, omega, bc_base, m0, psi, psi2, dose, X, Xn, K, Q1, Q2, partial, arena,
+    void* ptrs[] = {rho, irho, omega, bc_base, m0, psi, psi2, dose, u0p, X, Xn, K, Q1, Q2, partial, arena,
                     upl, svals, bcells, cand_d, srows, scols, bmask, restore};
     for (void* p : ptrs) if (p) hipFree(p);
     void* hptrs[] = {h_arena, h_upl, h_svals, h_srows, h_scols};
@@ -222,9 +225,9 @@ struct titus_ctx {
     const int nops = op == dev::OP_ALL4 ? 4 : 1;
     const size_t off = aalloc((size_t)nops * m * n);
     const double by = 8.0 * C * (A == B ? m : m + n) + (op == dev::OP_NONE ? 0.0 : 8.0 * C);
-    // Galerkin products run in FP32 in the mixed form; Gram and projection
-    // products of the QR passes stay FP64
-    const bool fp32 = mixed && op != dev::OP_NONE;
+    // Galerkin products run in FP32 only in the MIXED_FULL_FP32 form; Gram
+    // and projection products of the QR passes are always FP64
+    const bool fp32 = full32 && op != dev::OP_NONE;
     timed(op == dev::OP_NONE ? "gram" : "galerkin", by, 2.0 * nops * C * m * n, [&] {
       dev::launch_tn(st, g, A, dev::DT::F64, C, m, B, dev::DT::F64, C, n, op, fp32, partial,
                      arena + off);
@@ -302,7 +305,7 @@ struct titus_ctx {
     std::vector<HMat> WAW = galerkin_w(W, W);
     const double* Zd = upload(stream_Z(S, WAW));
     timed("stream_K", 8.0 * C * (2 * r + 1), 12.0 * C * r * r,
-          [&] { dev::launch_stream_k(st, g, X, C, r, Zd, r, dE, K, C, mixed); });
+          [&] { dev::launch_stream_k(st, g, X, C, r, Zd, r, dE, K, C, full32); });
   }
 
   // ---- device range basis --------------------------------------------------------
@@ -439,20 +442,22 @@ struct titus_ctx {
       for (int b = 0; b < rank; ++b) v[a] += S(a, b) * W(0, b);
     const double* vd = upload_raw(v.data(), v.size());
     const double coef = 0.5 * dE * Sn;
-    const double ps = mixed ? 4.0 : 8.0;     // bytes per psi entry
-    timed("dose", 8.0 * C * (rank + 3) + ps * C * nq, 2.0 * C * (rank + nq), [&] {
-      dev::launch_dose(st, g, dose, X, C, rank, vd, psi, tpsi, nq, m0, coef);
+    timed("dose", 8.0 * C * (rank + 3 + u0_groups), 2.0 * C * (rank + u0_groups), [&] {
+      dev::launch_dose(st, g, dose, X, C, rank, vd, u0p, u0_groups, coef);
     });
   }
   void begin_step(int s, std::vector<double>& Dvec, std::vector<double>& Dout) {
     const std::vector<double> sg = sig(s);
     const double ps = mixed ? 4.0 : 8.0;
-    timed("bc", (8.0 + ps) * nb * nq + 4.0 * nb, 1.0 * nb * nq,
-          [&] { dev::launch_bc(st, psi, tpsi, C, nq, bcells, nb, bc_base, bc_scale[s - 1]); });
+    timed("bc", (8.0 + ps) * nb * nq + 4.0 * nb + 8.0 * nb * u0_groups, 3.0 * nb * nq, [&] {
+      dev::launch_bc(st, psi, tpsi, C, nq, bcells, nb, bc_base, bc_scale[s - 1], m0, u0p,
+                     u0_groups);
+    });
     dose_half(stopping[s - 1]);
     const double denom = 1.0 + dE * sg[0];
-    timed("sweep", 2.0 * ps * C * nq + 9.0 * C, 12.0 * C * nq, [&] {
-      dev::launch_sweep(st, g, psi, psi2, tpsi, nq, omega, dE, denom, restore);
+    // psi read and written once, density, restore flags and moment partials
+    timed("sweep", 2.0 * ps * C * nq + 9.0 * C + 8.0 * C * u0_groups, 14.0 * C * nq, [&] {
+      dev::launch_sweep(st, g, psi, psi2, tpsi, nq, omega, dE, denom, restore, m0, u0p);
     });
     std::swap(psi, psi2);
     Dvec.resize(nm);
@@ -484,15 +489,15 @@ struct titus_ctx {
     const double* Td = upload(T);
     const double ps = mixed ? 4.0 : 8.0;
     timed("gemm", ps * C * nq + 16.0 * C * T.n, 2.0 * C * nq * T.n, [&] {
-      dev::launch_gemm(st, C, psi, tpsi, C, nq, Td, T.n, 1.0, 1.0, K, C, bmask, mixed);
+      dev::launch_gemm(st, C, psi, tpsi, C, nq, Td, T.n, 1.0, 1.0, K, C, bmask, full32);
     });
   }
-  // arena[off : off + m*nq] = Xa^T psi (the psi contractions; FP32 in the mixed form)
+  // arena[off : off + m*nq] = Xa^T psi (the psi contractions; FP32 in MIXED_FULL_FP32)
   size_t tn_psi(const double* Xa, int m) {
     const size_t off = aalloc((size_t)m * nq);
     const double ps = mixed ? 4.0 : 8.0;
     timed("gram", 8.0 * C * m + ps * C * nq, 2.0 * C * m * nq, [&] {
-      dev::launch_tn(st, g, Xa, dev::DT::F64, C, m, psi, tpsi, C, nq, dev::OP_NONE, mixed, partial,
+      dev::launch_tn(st, g, Xa, dev::DT::F64, C, m, psi, tpsi, C, nq, dev::OP_NONE, full32, partial,
                      arena + off);
     });
     return off;
@@ -714,7 +719,8 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
                 p->cand_cells, "missing operand");
     require(p->integrator == TITUS_INTEGRATOR_FIXED || p->integrator == TITUS_INTEGRATOR_BUG,
             "unknown integrator");
-    require(p->precision == TITUS_PRECISION_FP64 || p->precision == TITUS_PRECISION_MIXED,
+    require(p->precision == TITUS_PRECISION_FP64 || p->precision == TITUS_PRECISION_MIXED ||
+                p->precision == TITUS_PRECISION_MIXED_FULL_FP32,
             "unknown precision form");
     require(p->r >= 1, "rank must be >= 1");
     const int C = p->nx * p->ny;
@@ -738,7 +744,8 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     HIPCHK(hipStreamCreateWithFlags(&x.st, hipStreamNonBlocking));
     x.nx = p->nx; x.ny = p->ny; x.C = C; x.nm = p->nm; x.nq = p->nq; x.ne = p->ne;
     x.nsig = p->nsigma; x.integrator = p->integrator; x.precision = p->precision;
-    x.mixed = p->precision == TITUS_PRECISION_MIXED;
+    x.mixed = p->precision != TITUS_PRECISION_FP64;
+    x.full32 = p->precision == TITUS_PRECISION_MIXED_FULL_FP32;
     x.tpsi = x.mixed ? dev::DT::F32 : dev::DT::F64;
     x.dx = p->dx; x.dy = p->dy; x.dE = p->dE; x.theta = p->theta; x.theta_index = p->theta_index;
     x.rank_tol = p->rank_tol > 0.0 ? p->rank_tol : 1e-7;
@@ -768,6 +775,7 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
 
     const size_t cx = (size_t)C * x.capx;
     x.rho = dalloc<double>(C);
+    x.irho = dalloc<double>(C);
     x.omega = dalloc<double>((size_t)p->nq * 3);
     x.bc_base = dalloc<double>((size_t)p->nb * p->nq);
     x.m0 = dalloc<double>(p->nq);
@@ -775,13 +783,14 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     x.psi = dalloc<unsigned char>((size_t)C * p->nq * psz);
     x.psi2 = dalloc<unsigned char>((size_t)C * p->nq * psz);
     x.dose = dalloc<double>(C);
+    x.u0p = dalloc<double>((size_t)dev::kMaxSweepGroups * C);
     x.X = dalloc<double>(cx);
     x.Xn = dalloc<double>(cx);
     x.K = dalloc<double>(cx);
     x.Q1 = dalloc<double>(cx);
     x.Q2 = dalloc<double>(cx);
     const int wmax = std::max(x.capx, p->nq);
-    x.partial = dalloc<double>(dev::tn_partial_size(C, wmax, wmax));
+    x.partial = dalloc<double>(dev::tn_partial_size());
     const size_t w2 = (size_t)wmax * wmax;
     const size_t ncand_used = std::min<size_t>(p->ncand, (size_t)2 * x.capx + 64);
     // largest phase: Galerkin of the augmented basis plus one range job
@@ -802,9 +811,14 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     x.cand_d = dalloc<int>(p->ncand);
     x.bmask = dalloc<unsigned char>(C);
     x.restore = dalloc<unsigned char>(C);
-    x.g = dev::Grid{p->nx, p->ny, C, p->dx, p->dy, x.rho};
+    x.g = dev::Grid{p->nx, p->ny, C, p->dx, p->dy, x.rho, x.irho};
 
     HIPCHK(hipMemcpy(x.rho, p->density, sizeof(double) * C, hipMemcpyHostToDevice));
+    {
+      std::vector<double> ir(C);
+      for (int c = 0; c < C; ++c) ir[c] = 1.0 / p->density[c];
+      HIPCHK(hipMemcpy(x.irho, ir.data(), sizeof(double) * C, hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMemcpy(x.omega, p->omega, sizeof(double) * p->nq * 3, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(x.bc_base, p->bc_base, sizeof(double) * (size_t)p->nb * p->nq, hipMemcpyHostToDevice));
     std::vector<double> m0(p->nq);
@@ -820,6 +834,9 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     HIPCHK(hipMemset(x.psi, 0, psz * (size_t)C * p->nq));
     HIPCHK(hipMemset(x.psi2, 0, psz * (size_t)C * p->nq));
     HIPCHK(hipMemset(x.dose, 0, sizeof(double) * C));
+    // psi starts at zero, so does its moment
+    HIPCHK(hipMemset(x.u0p, 0, sizeof(double) * dev::kMaxSweepGroups * C));
+    x.u0_groups = dev::sweep_groups(x.g, p->nq, x.tpsi);
     HIPCHK(hipMemset(x.X, 0, sizeof(double) * cx));
 
     // initial factors (SolverCSD.jl:932-957): svd!(zeros) -> identity columns,

@@ -21,19 +21,36 @@ from . import _lib
 from .problem import INTEGRATOR_BUG, INTEGRATOR_FIXED, build_problem
 
 
-class SolverCSD:
-    """Operands of the CSD transport solve (``SolverCSD.jl:20-147``), FP64.
+# Rank cut of the range rule.  FP64: 1e-7 of the scale.  Mixed form: FP32
+# arithmetic perturbs singular values by ~1e-7 relative, and the dose depends
+# on which near-null directions enter the basis (both integrators keep at
+# least two directions, and the fixed-rank one keeps all r), so the cut sits
+# three decades above that noise.  Compare a mixed solve with an FP64 solve
+# at the same cut.
+RANK_TOL = 1e-7
+MIXED_RANK_TOL = 1e-4
 
-    ``precision``: "fp64" (plain form).  ``moments``: "subset" (k >= 0 real
-    harmonics, the 2D-exact reduction) or "full" ((N+1)^2 as the reference).
+_PRECISIONS = {"fp64": _lib.PRECISION_FP64, "mixed": _lib.PRECISION_MIXED,
+               "mixed_full_fp32": _lib.PRECISION_MIXED_FULL_FP32}
+
+
+class SolverCSD:
+    """Operands of the CSD transport solve (``SolverCSD.jl:20-147``).
+
+    ``precision``: "fp64" (plain form, FP64 throughout), "mixed" (psi stored
+    and swept in FP32, the rest FP64) or "mixed_full_fp32" (streaming stencil,
+    psi contractions and Galerkin products in FP32 as well).  ``moments``: "subset" (k >= 0 real harmonics, the 2D-exact
+    reduction) or "full" ((N+1)^2 as the reference).  ``rank_tol``: None picks
+    the defaults above.
     """
 
-    def __init__(self, settings, device=0, precision="fp64", moments="subset", rank_tol=1e-7):
-        if precision != "fp64":
-            raise ValueError("only the FP64 form is available")
+    def __init__(self, settings, device=0, precision="fp64", moments="subset", rank_tol=None):
+        if precision not in _PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}")
         self.settings = settings
         self.device = device
         self.moments = moments
+        self.precision = precision
         self.rank_tol = rank_tol
         nx, ny = settings.NCellsX, settings.NCellsY
         self.dose = np.zeros(nx * ny)
@@ -53,10 +70,16 @@ class SolverCSD:
     def MReduced(self):
         return self.problem(INTEGRATOR_FIXED)["M"]
 
+    def tolerance(self, integrator):
+        if self.rank_tol is not None:
+            return self.rank_tol
+        return RANK_TOL if self.precision == "fp64" else MIXED_RANK_TOL
+
     def context(self, integrator):
         """A fresh device context for ``integrator`` (steps run through it)."""
         return _lib.Context(self.problem(integrator), integrator, device=self.device,
-                            rank_tol=self.rank_tol)
+                            precision=_PRECISIONS[self.precision],
+                            rank_tol=self.tolerance(integrator))
 
 
 def _finish(obj, P, st):
