@@ -76,24 +76,32 @@ def cpu_baseline(P, seconds):
             "sample": f"FP64 numpy oracle, steps 1..{steps} of the same solve, {elapsed:.1f} s"}
 
 
-def run_form(P, precision, W, K, events, sync, rank_tol):
+def run_form(P, precision, W, K, period, sync, rank_tol):
+    """Solve steps 1..W untimed, then time steps W+1..W+K.
+
+    `period` > 0: HIP events and host-phase clocks on every period-th step of
+    the timed region (0: none); `sampled` counts those steps.
+    """
     from titus_amd import _lib
     ctx = _lib.Context(P, _lib.INTEGRATOR_BUG, precision=precision, rank_tol=rank_tol)
     try:
         ctx.solve(W)
         sync()
-        ctx.set_timing(events)
+        ctx.set_timing(period)
         t0 = time.perf_counter()
         ctx.solve(W + K)
         sync()
         elapsed = time.perf_counter() - t0
         stats = {name: ctx.kernel_stats(name) for name in ctx.KERNELS}
+        host = {name: ctx.kernel_stats(name)[0] for name in ctx.HOST_PHASES}
         done, _ = ctx.progress()
         ranks, _ = ctx.history(done)
         dose = ctx.state()["dose"]
     finally:
         ctx.close()
-    return dict(elapsed=elapsed, rate=K / elapsed, stats=stats, ranks=ranks, dose=dose)
+    sampled = sum(1 for s in range(W + 1, W + K + 1) if period and s % period == 0)
+    return dict(elapsed=elapsed, rate=K / elapsed, stats=stats, host=host, sampled=sampled,
+                ranks=ranks, dose=dose)
 
 
 def beamlet_batch(n_beamlets, nthreads, sync):
@@ -188,11 +196,18 @@ def stencil_scale(sync, steps=4, n=1024, calibrate=False):
     return out
 
 
-def roofline(stats, fp32):
+def per_step(run, ms):
+    """Milliseconds per sampled step (0 when nothing was sampled)."""
+    return ms / run["sampled"] if run["sampled"] else 0.0
+
+
+def roofline(stats, fp32_kernels=()):
+    """Roofline of the dominant kernel (largest device time); `fp32_kernels`
+    compute in FP32 (their flop peak is the FP32 one)."""
     dom = max(stats, key=lambda k: stats[k][0])
     ms, launches, by, fl = stats[dom]
     t_avg = ms / max(launches, 1) * 1e-3
-    peak_f = PEAK_TFLOPS["f32" if fp32 and dom in ("sweep", "stream_K", "galerkin") else "f64"]
+    peak_f = PEAK_TFLOPS["f32" if dom in fp32_kernels else "f64"]
     ridge = peak_f * 1e12 / (HBM_PEAK_GBS * 1e9)
     if by > 0 and fl / by > ridge:
         roof = {"bound": "mfma", "achieved": fl / max(launches, 1) / t_avg / 1e12,
@@ -217,6 +232,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-events", action="store_true",
                     help="time without per-kernel HIP events (no roofline)")
+    ap.add_argument("--event-period", type=int, default=10,
+                    help="HIP events on every n-th step of the timed region")
     ap.add_argument("--beamlets", type=int, default=64,
                     help="beamlets of the batched plan (0: skip)")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1),
@@ -243,15 +260,15 @@ def main():
     last = P["nE"] - 2
     W = min(args.warmup, last - 1)
     K = min(args.steps, last - W)
-    events = not args.no_events
+    period = 0 if args.no_events else args.event_period
     # both forms at the mixed form's rank cut (FP32 noise must not decide which
     # near-null directions enter the basis); the FP64 form at the FP64 cut is
     # run as well, untimed, to report what the cut itself changes
-    f64 = run_form(P, _lib.PRECISION_FP64, W, K, events, sync, MIXED_RANK_TOL)
-    mix = run_form(P, _lib.PRECISION_MIXED, W, K, events, sync, MIXED_RANK_TOL)
-    f64_cut = run_form(P, _lib.PRECISION_FP64, W, K, False, sync, RANK_TOL)
+    f64 = run_form(P, _lib.PRECISION_FP64, W, K, period, sync, MIXED_RANK_TOL)
+    mix = run_form(P, _lib.PRECISION_MIXED, W, K, period, sync, MIXED_RANK_TOL)
+    f64_cut = run_form(P, _lib.PRECISION_FP64, W, K, 0, sync, RANK_TOL)
     cut_err = float(np.linalg.norm(f64["dose"] - f64_cut["dose"]) / np.linalg.norm(f64_cut["dose"]))
-    g32 = run_form(P, _lib.PRECISION_MIXED_FULL_FP32, W, K, False, sync, MIXED_RANK_TOL)
+    g32 = run_form(P, _lib.PRECISION_MIXED_FULL_FP32, W, K, 0, sync, MIXED_RANK_TOL)
     g32_err = float(np.linalg.norm(g32["dose"] - f64["dose"]) / np.linalg.norm(f64["dose"]))
     derr = float(np.linalg.norm(mix["dose"] - f64["dose"]) / np.linalg.norm(f64["dose"]))
     n_r = min(len(mix["ranks"]), len(f64["ranks"]))
@@ -280,7 +297,7 @@ def main():
             "parallelism": "single GPU",
         },
         "fp64": {"value": f64["rate"], "ms_per_step": f64["elapsed"] / K * 1e3,
-                 "device_ms_per_step": sum(v[0] for v in f64["stats"].values()) / K},
+                 "device_ms_per_step": per_step(f64, sum(v[0] for v in f64["stats"].values()))},
         "mixed_over_fp64": mix["rate"] / f64["rate"],
         "dose_rel_l2_mixed_vs_fp64": derr,
         "rank_history_mixed_vs_fp64": {"equal": bool(len(diff) == 0),
@@ -288,13 +305,14 @@ def main():
         "rank_tol": MIXED_RANK_TOL,
         "mixed_full_fp32": {"value": g32["rate"], "dose_rel_l2_vs_fp64": g32_err},
         "fp64_dose_rel_l2_rank_tol_1e-4_vs_1e-7": cut_err,
-        "device_ms_per_step": sum(v[0] for v in mix["stats"].values()) / K,
-        "kernel_ms": {k: round(v[0], 3) for k, v in mix["stats"].items()},
-        "events_in_timed_region": events,
+        "device_ms_per_step": per_step(mix, sum(v[0] for v in mix["stats"].values())),
+        "kernel_ms_per_step": {k: round(per_step(mix, v[0]), 4) for k, v in mix["stats"].items()},
+        "host_ms_per_step": {k.split(":")[1]: round(per_step(mix, v), 4) for k, v in mix["host"].items()},
+        "events_in_timed_region": {"period": period, "sampled_steps": mix["sampled"]},
     }
-    if events:
-        out["roofline"] = roofline(mix["stats"], fp32=True)
-        out["roofline_fp64"] = roofline(f64["stats"], fp32=False)
+    if period:
+        out["roofline"] = roofline(mix["stats"], fp32_kernels=("sweep",))
+        out["roofline_fp64"] = roofline(f64["stats"])
     if args.beamlets > 0:
         out["beamlets"] = beamlet_batch(args.beamlets, args.threads, sync)
     out["stencil_scale"] = stencil_scale(sync)

@@ -136,8 +136,13 @@ int titus_get_history(const struct titus_ctx* ctx, int32_t* ranks, double* etas,
 /* Device time of the library's kernels since timing was (re)enabled, measured
  * with HIP events on the context's stream: total milliseconds, launch count,
  * and the algorithmic HBM bytes and flops of those launches, for kernel
- * `name` ("bc", "dose", "sweep", "stream_K", "galerkin", "gram", "gemm"). */
-int titus_set_timing(struct titus_ctx* ctx, int32_t enabled);
+ * `name` ("bc", "dose", "sweep", "stream_K", "galerkin", "gram", "gemm",
+ * "chol").  `period`: 0 off; n > 0 times the launches of every n-th energy
+ * step (1: all of them; a larger period keeps the event overhead out of most
+ * steps).  Host-side phases are reported under the names "host:step" (wall
+ * time of the sampled steps), "host:wait" (blocked on the stream) and
+ * "host:<phase>" for the host algebra, in wall milliseconds. */
+int titus_set_timing(struct titus_ctx* ctx, int32_t period);
 int titus_kernel_stats(const struct titus_ctx* ctx, const char* name, double* total_ms,
                        int64_t* launches, double* bytes, double* flops);
 

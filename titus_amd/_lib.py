@@ -201,10 +201,13 @@ class Context:
         check(self._lib.titus_get_history(self.h, _ip(ranks), _dp(etas), n))
         return ranks, etas
 
-    def set_timing(self, on):
-        check(self._lib.titus_set_timing(self.h, 1 if on else 0))
+    def set_timing(self, period):
+        """Time every `period`-th step (True: every step; 0 / False: off)."""
+        check(self._lib.titus_set_timing(self.h, int(period)))
 
     KERNELS = ("bc", "dose", "sweep", "stream_K", "galerkin", "gram", "gemm", "chol")
+    HOST_PHASES = ("host:step", "host:wait", "host:range_finish", "host:truncate", "host:range_w",
+                   "host:galerkin_w", "host:stream_LS")
 
     def kernel_stats(self, name):
         """(total ms, launches, algorithmic bytes, flops) since timing was enabled."""

@@ -22,6 +22,9 @@ HIP_FLAGS = [
     # packed FP32 instructions are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
+    # host-side small algebra: AVX2 + FMA (every x86-64 server CPU of an
+    # MI355X node has them)
+    "-Xarch_host", "-mavx2", "-Xarch_host", "-mfma",
 ]
 
 

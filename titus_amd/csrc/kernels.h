@@ -98,9 +98,9 @@ inline void launch_gemm(hipStream_t st, int C, const double* A, int lda, int k, 
 void launch_gather_rows(hipStream_t st, const double* A, int lda, int ncols, const int* rows,
                         int nrows, double* out);
 
-// A[rows[t] + cols[t]*lda] += vals[t].
-void launch_scatter_add(hipStream_t st, double* A, int lda, const int* rows, const int* cols,
-                        const double* vals, int n);
+// A[rows[t] + cols[t]*lda] += vals[t] with packed = [rows, cols, vals] (n
+// each, the indices stored as doubles).
+void launch_scatter_add(hipStream_t st, double* A, int lda, const double* packed, int n);
 
 }  // namespace dev
 }  // namespace titus

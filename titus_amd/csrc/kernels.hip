@@ -458,12 +458,13 @@ __global__ void k_gather_rows(const double* __restrict__ A, int lda, int ncols,
   out[t] = A[(size_t)k * lda + rows[r]];
 }
 
-__global__ void k_scatter_add(double* __restrict__ A, int lda, const int* __restrict__ rows,
-                              const int* __restrict__ cols, const double* __restrict__ vals,
+// packed = [rows (n), cols (n), vals (n)], indices stored as doubles
+__global__ void k_scatter_add(double* __restrict__ A, int lda, const double* __restrict__ packed,
                               int n) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  A[(size_t)cols[t] * lda + rows[t]] += vals[t];
+  const int row = (int)packed[t], col = (int)packed[n + t];
+  A[(size_t)col * lda + row] += packed[2 * n + t];
 }
 
 template <int NOPS, class TB, class TC>
@@ -658,11 +659,9 @@ void launch_gather_rows(hipStream_t st, const double* A, int lda, int ncols, con
   check_launch("gather_rows");
 }
 
-void launch_scatter_add(hipStream_t st, double* A, int lda, const int* rows, const int* cols,
-                        const double* vals, int n) {
+void launch_scatter_add(hipStream_t st, double* A, int lda, const double* packed, int n) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_scatter_add, dim3(blocks_for(n)), dim3(kBlock), 0, st, A, lda, rows, cols,
-                     vals, n);
+  hipLaunchKernelGGL(k_scatter_add, dim3(blocks_for(n)), dim3(kBlock), 0, st, A, lda, packed, n);
   check_launch("scatter_add");
 }
 

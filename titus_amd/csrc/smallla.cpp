@@ -32,15 +32,22 @@ SpMat SpMat::from_dense(const HMat& A) {
 
 HMat spmm(const SpMat& A, const HMat& B) {
   if (A.n != B.m) throw std::runtime_error("spmm: inner dimensions differ");
-  HMat C(A.m, B.n);
-  for (int j = 0; j < B.n; ++j) {
-    const double* b = B.col(j);
-    double* c = C.col(j);
-    for (int i = 0; i < A.m; ++i) {
-      double s = 0.0;
-      for (int t = A.rowptr[i]; t < A.rowptr[i + 1]; ++t) s += A.val[t] * b[A.col[t]];
-      c[i] = s;
+  // one pass over the nonzeros: each row of A combines whole rows of B (B held
+  // row-major, so the inner loop runs over contiguous columns); the sum per
+  // output entry runs over the row's nonzeros in order
+  const int n = B.n;
+  HMat C(A.m, n);
+  std::vector<double> Bt((std::size_t)B.m * n), acc(n);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < B.m; ++i) Bt[(std::size_t)i * n + j] = B(i, j);
+  for (int i = 0; i < A.m; ++i) {
+    std::fill(acc.begin(), acc.end(), 0.0);
+    for (int t = A.rowptr[i]; t < A.rowptr[i + 1]; ++t) {
+      const double v = A.val[t];
+      const double* b = Bt.data() + (std::size_t)A.col[t] * n;
+      for (int j = 0; j < n; ++j) acc[j] += v * b[j];
     }
+    for (int j = 0; j < n; ++j) C(i, j) = acc[j];
   }
   return C;
 }

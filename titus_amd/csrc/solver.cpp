@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -72,7 +74,8 @@ T* halloc(size_t n) {
 
 // Kernel timing with HIP events on the context's stream.
 struct Timing {
-  bool enabled = false;
+  bool enabled = false;     // sampling this step
+  int period = 0;           // time every period-th step (0: never)
   struct Rec { int id; hipEvent_t a, b; double bytes, flops; };
   std::vector<Rec> pending;
   std::vector<hipEvent_t> pool;
@@ -95,6 +98,12 @@ struct Timing {
     total_bytes.push_back(0.0); total_flops.push_back(0.0);
     return k;
   }
+  // host wall time of a phase (milliseconds), counted as one "launch"
+  void host(const char* name, double ms) {
+    const int k = id(name);
+    total_ms[k] += ms;
+    launches[k] += 1;
+  }
   void resolve() {
     for (auto& r : pending) {
       float ms = 0.f;
@@ -111,6 +120,18 @@ struct Timing {
   ~Timing() {
     for (auto& r : pending) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
     for (auto e : pool) hipEventDestroy(e);
+  }
+};
+
+// Wall time of a host phase, recorded when the step is sampled.
+struct HostScope {
+  Timing& t;
+  const char* name;
+  std::chrono::steady_clock::time_point t0;
+  HostScope(Timing& tm, const char* n) : t(tm), name(n), t0(std::chrono::steady_clock::now()) {}
+  ~HostScope() {
+    if (t.enabled)
+      t.host(name, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
   }
 };
 
@@ -145,13 +166,12 @@ struct titus_ctx {
   double* u0p = nullptr;                    // l = 0 moment of psi, G partials x C
   int u0_groups = 1;
   double *X = nullptr, *Xn = nullptr, *K = nullptr, *Q1 = nullptr, *Q2 = nullptr;
-  double *partial = nullptr, *arena = nullptr, *upl = nullptr, *svals = nullptr;
-  int *bcells = nullptr, *cand_d = nullptr, *srows = nullptr, *scols = nullptr;
+  double *partial = nullptr, *arena = nullptr, *upl = nullptr;
+  int *bcells = nullptr, *cand_d = nullptr;
   unsigned char *bmask = nullptr, *restore = nullptr;
   // pinned host mirrors
-  double *h_arena = nullptr, *h_upl = nullptr, *h_svals = nullptr;
-  int *h_srows = nullptr, *h_scols = nullptr;
-  size_t arena_n = 0, arena_top = 0, upl_n = 0, upl_top = 0, scat_n = 0;
+  double *h_arena = nullptr, *h_upl = nullptr;
+  size_t arena_n = 0, arena_top = 0, upl_n = 0, upl_top = 0;
   dev::Grid g{};
   int nb = 0;
   // state
@@ -159,13 +179,13 @@ struct titus_ctx {
   int rank = 0, steps_done = 0;
   std::vector<int> ranks;
   std::vector<double> etas;
-  Timing timing;
+  mutable Timing timing;
 
   ~titus_ctx() {
     void* ptrs[] = {rho, irho, omega, bc_base, m0, psi, psi2, dose, u0p, X, Xn, K, Q1, Q2, partial, arena,
-                    upl, svals, bcells, cand_d, srows, scols, bmask, restore};
+                    upl, bcells, cand_d, bmask, restore};
     for (void* p : ptrs) if (p) hipFree(p);
-    void* hptrs[] = {h_arena, h_upl, h_svals, h_srows, h_scols};
+    void* hptrs[] = {h_arena, h_upl};
     for (void* p : hptrs) if (p) hipHostFree(p);
     if (st) hipStreamDestroy(st);
   }
@@ -196,6 +216,7 @@ struct titus_ctx {
   void fetch() {
     if (arena_top)
       HIPCHK(hipMemcpyAsync(h_arena, arena, arena_top * sizeof(double), hipMemcpyDeviceToHost, st));
+    HostScope hs(timing, "host:wait");
     HIPCHK(hipStreamSynchronize(st));
     upl_top = 0;          // every queued upload has been consumed
   }
@@ -255,22 +276,44 @@ struct titus_ctx {
   }
 
   // ---- host building blocks (angular side) -------------------------------------
-  std::vector<HMat> galerkin_w(const HMat& Wa, const HMat& Wb) const {
-    return {titus::matmul(Wa, titus::spmm(sAx[0], Wb), true), titus::matmul(Wa, titus::spmm(sAx[1], Wb), true),
-            titus::matmul(Wa, titus::spmm(sAx[2], Wb), true), titus::matmul(Wa, titus::spmm(sAx[3], Wb), true)};
+  // [|Ax| Wm, |Az| Wm, Ax Wm, Az Wm]: computed once per basis and reused by
+  // the Galerkin products, the L-step (A (W S^T) = (A W) S^T) and the
+  // augmented basis (A [W Wt] = [A W, A Wt])
+  using AW4 = std::array<HMat, 4>;
+  AW4 apply_a(const HMat& Wm) const {
+    HostScope hs(timing, "host:galerkin_w");
+    return {titus::spmm(sAx[0], Wm), titus::spmm(sAx[1], Wm), titus::spmm(sAx[2], Wm),
+            titus::spmm(sAx[3], Wm)};
   }
-  // L - dE (Ax L XL2xX^T + Az L XL2yX^T + |Ax| L XL1xX^T + |Az| L XL1yX^T)
-  HMat stream_L(const HMat& L, const std::vector<HMat>& XLX) const {
-    HMat out = L;
+  std::vector<HMat> galerkin_from(const HMat& Wa, const AW4& awb) const {
+    HostScope hs(timing, "host:galerkin_w");
+    return {titus::matmul(Wa, awb[0], true), titus::matmul(Wa, awb[1], true),
+            titus::matmul(Wa, awb[2], true), titus::matmul(Wa, awb[3], true)};
+  }
+  std::vector<HMat> galerkin_w(const HMat& Wa, const HMat& Wb) const {
+    return galerkin_from(Wa, apply_a(Wb));
+  }
+  static AW4 hcat4(const AW4& a, const AW4& b) {
+    return {titus::hcat(a[0], b[0]), titus::hcat(a[1], b[1]), titus::hcat(a[2], b[2]),
+            titus::hcat(a[3], b[3])};
+  }
+  // L - dE (Ax L XL2xX^T + Az L XL2yX^T + |Ax| L XL1xX^T + |Az| L XL1yX^T) for
+  // L = W S^T, with A L = (A W) S^T from aw = apply_a(W)
+  HMat stream_L(const AW4& aw, const std::vector<HMat>& XLX) const {
+    HostScope hs(timing, "host:stream_LS");
+    HMat out = titus::matmul(W, S, false, true);
     for (int k : {2, 3, 0, 1}) {
-      HMat t = titus::matmul(titus::spmm(sAx[k], L), XLX[k], false, true);
+      HMat t = titus::matmul(titus::matmul(aw[k], S, false, true), XLX[k], false, true);
       for (size_t i = 0; i < out.a.size(); ++i) out.a[i] -= dE * t.a[i];
     }
     return out;
   }
   HMat stream_S(const HMat& S0, const std::vector<HMat>& XLX, const std::vector<HMat>& WAW) const {
+    HostScope hs(timing, "host:stream_LS");
     HMat out = S0;
     for (int k : {2, 3, 0, 1}) {
+      if (XLX[k].m != S0.m || XLX[k].n != S0.m || WAW[k].m != S0.n || WAW[k].n != S0.n)
+        throw TitusError(TITUS_E_ARG, "S-step: Galerkin blocks do not match the coefficients");
       HMat t = titus::matmul(titus::matmul(XLX[k], S0), WAW[k]);
       for (size_t i = 0; i < out.a.size(); ++i) out.a[i] -= dE * t.a[i];
     }
@@ -298,11 +341,11 @@ struct titus_ctx {
     return Z;
   }
   HMat range_w(const HMat& L, int ncols, const HMat* against, double scale) const {
+    HostScope hs(timing, "host:range_w");
     static const std::vector<int> all;
     return titus::range_basis(L, ncols, against, scale, rank_tol, true, all);
   }
-  void stream_k(int r) {
-    std::vector<HMat> WAW = galerkin_w(W, W);
+  void stream_k(int r, const std::vector<HMat>& WAW) {
     const double* Zd = upload(stream_Z(S, WAW));
     timed("stream_K", 8.0 * C * (2 * r + 1), 12.0 * C * r * r,
           [&] { dev::launch_stream_k(st, g, X, C, r, Zd, r, dE, K, C, full32); });
@@ -354,6 +397,7 @@ struct titus_ctx {
   // Host part after fetch(): SVD of R = R3 R2 R1, rank, canonical completion,
   // and the device launches that write the basis (C x ncols) to `out`.
   int range_finish(const RangeJob& j, int ncols, double* out) {
+    HostScope hs(timing, "host:range_finish");
     const int k = j.k, q = j.q;
     HMat G1 = view(j.g1, k, k);
     double tr1 = 0.0;
@@ -416,18 +460,16 @@ struct titus_ctx {
         for (double& v : Tx.a) v = -v;
         gemm(j.Xa, q, Tx, 1.0, 1.0, out + (size_t)p * C, nullptr);
       }
-      int ns = 0;
+      // (row, column, value) triples of the upper-triangular Rinv at the
+      // chosen cells, packed as doubles through the upload ring
+      const int ns = need * (need + 1) / 2;
+      std::vector<double> pk(3 * (size_t)ns);
+      int e = 0;
       for (int s = 0; s < need; ++s)
-        for (int t = s; t < need; ++t) {
-          if ((size_t)ns >= scat_n) throw TitusError(TITUS_E_ARG, "scatter area too small");
-          h_srows[ns] = chosen[s]; h_scols[ns] = p + t; h_svals[ns] = Rinv(s, t); ++ns;
+        for (int t = s; t < need; ++t, ++e) {
+          pk[e] = chosen[s]; pk[ns + e] = p + t; pk[2 * ns + e] = Rinv(s, t);
         }
-      HIPCHK(hipMemcpyAsync(srows, h_srows, ns * sizeof(int), hipMemcpyHostToDevice, st));
-      HIPCHK(hipMemcpyAsync(scols, h_scols, ns * sizeof(int), hipMemcpyHostToDevice, st));
-      HIPCHK(hipMemcpyAsync(svals, h_svals, ns * sizeof(double), hipMemcpyHostToDevice, st));
-      dev::launch_scatter_add(st, out, C, srows, scols, svals, ns);
-      // the pinned scatter lists are reused next time: wait for these copies
-      HIPCHK(hipStreamSynchronize(st));
+      dev::launch_scatter_add(st, out, C, upload_raw(pk.data(), pk.size()), ns);
     }
     return p;
   }
@@ -511,13 +553,14 @@ struct titus_ctx {
       // K-step and old Galerkin products (SolverCSD.jl:1019-1038)
       areset();
       const size_t gx = galerkin_x(X, r, X, r);
-      stream_k(r);
+      const AW4 aw = apply_a(W);
+      stream_k(r, galerkin_from(W, aw));
       RangeJob j = range_launch(K, r, nullptr, 0);
       fetch();
       std::vector<HMat> XLX = galerkin_view(gx, r, r);
       range_finish(j, r, Xn);
       // L-step on the host while the device writes XNew (:1033-1046)
-      HMat L = stream_L(titus::matmul(W, S, false, true), XLX);
+      HMat L = stream_L(aw, XLX);
       HMat WNew = range_w(L, r, nullptr, -1.0);
       HMat NUp = titus::matmul(WNew, W, true, false);
       areset();
@@ -564,6 +607,7 @@ struct titus_ctx {
   // truncate! (SolverCSD.jl:3369-3399) of the 2r x 2r Sup over the augmented
   // bases X1 (device, C x 2r) and W1; writes X (device), W, S, rank.
   void truncate_to(const HMat& Sup, const double* X1, const HMat& W1) {
+    HostScope hs(timing, "host:truncate");
     HMat U, V;
     std::vector<double> D;
     titus::svd_completed(Sup, rank_tol, U, D, V);
@@ -600,13 +644,14 @@ struct titus_ctx {
       const int r = rank;
       areset();
       const size_t gx = galerkin_x(X, r, X, r);
-      stream_k(r);
+      const AW4 aw = apply_a(W);
+      stream_k(r, galerkin_from(W, aw));
       copy_x_to_aug(r);
       RangeJob j = range_launch(K, r, X, r);
       fetch();
       std::vector<HMat> XLX = galerkin_view(gx, r, r);
       range_finish(j, r, Xn + (size_t)r * C);               // Xn = X1 = [X | Xt]
-      HMat W1 = augment_w(stream_L(titus::matmul(W, S, false, true), XLX));
+      HMat W1 = augment_w(stream_L(aw, XLX));
       HMat Nu = titus::matmul(W1, W, true, false);
       areset();
       const size_t mu = tn(Xn, 2 * r, X, r, dev::OP_NONE);
@@ -615,7 +660,9 @@ struct titus_ctx {
       HMat Mu = view(mu, 2 * r, r);
       XLX = galerkin_view(g1, 2 * r, 2 * r);
       HMat S1 = titus::matmul(titus::matmul(Mu, S), Nu, false, true);   // 2r x 2r
-      std::vector<HMat> WAW = galerkin_w(W1, W1);
+      // A W1 = [A W, A Wt]
+      std::vector<HMat> WAW =
+          galerkin_from(W1, hcat4(aw, apply_a(titus::cols(W1, r, W1.n))));
       HMat Sup = stream_S(S1, XLX, WAW);
       // eta diagnostic (:2210-2220): rows r.. of X1^T L X1, columns r.. of W1^T A W1
       HMat acc(r, r);
@@ -663,6 +710,8 @@ struct titus_ctx {
   void step(int s) {
     if (s != steps_done + 1) throw TitusError(TITUS_E_STATE, "steps must run in order");
     if (s >= ne) throw TitusError(TITUS_E_STATE, "step beyond the energy grid");
+    timing.enabled = timing.period > 0 && s % timing.period == 0;
+    HostScope hs(timing, "host:step");
     if (integrator == TITUS_INTEGRATOR_FIXED) step_fixed(s);
     else step_bug(s);
     steps_done = s;
@@ -800,13 +849,6 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     x.upl_n = 64 * w2 + 8192;
     x.upl = dalloc<double>(x.upl_n);
     x.h_upl = halloc<double>(x.upl_n);
-    x.scat_n = (size_t)x.capx * x.capx;
-    x.srows = dalloc<int>(x.scat_n);
-    x.scols = dalloc<int>(x.scat_n);
-    x.svals = dalloc<double>(x.scat_n);
-    x.h_srows = halloc<int>(x.scat_n);
-    x.h_scols = halloc<int>(x.scat_n);
-    x.h_svals = halloc<double>(x.scat_n);
     x.bcells = dalloc<int>(p->nb);
     x.cand_d = dalloc<int>(p->ncand);
     x.bmask = dalloc<unsigned char>(C);
@@ -956,16 +998,18 @@ int titus_get_history(const titus_ctx* ctx, int32_t* ranks, double* etas, int32_
   });
 }
 
-int titus_set_timing(titus_ctx* ctx, int32_t enabled) {
+int titus_set_timing(titus_ctx* ctx, int32_t period) {
   return guarded([&] {
     require(ctx != nullptr, "null context");
+    require(period >= 0, "timing period must be >= 0");
     HIPCHK(hipStreamSynchronize(ctx->st));
     ctx->timing.resolve();
     for (auto& t : ctx->timing.total_ms) t = 0.0;
     for (auto& l : ctx->timing.launches) l = 0;
     for (auto& b : ctx->timing.total_bytes) b = 0.0;
     for (auto& f : ctx->timing.total_flops) f = 0.0;
-    ctx->timing.enabled = enabled != 0;
+    ctx->timing.period = period;
+    ctx->timing.enabled = false;
   });
 }
 
