@@ -183,6 +183,7 @@ def stencil_scale(sync, steps=4, n=1024, calibrate=False):
                     res[k] = {"GB/s": round(gbs, 1), "frac_of_8TB/s": round(gbs / HBM_PEAK_GBS, 3),
                               "avg_launch_us": round(ms / launches * 1e3, 2),
                               "bytes_per_launch": by / launches}
+            res["sweep_plan"] = ctx.tuning("sweep")
             out[name] = res
         finally:
             ctx.close()

@@ -146,6 +146,14 @@ int titus_set_timing(struct titus_ctx* ctx, int32_t period);
 int titus_kernel_stats(const struct titus_ctx* ctx, const char* name, double* total_ms,
                        int64_t* launches, double* bytes, double* flops);
 
+/* Launch plan the library uses for one of this context's kernels (no
+ * counterpart in the reference: its loops have no launch parameters).
+ * kernel "sweep": values[0] = rows per wavefront strip, values[1] = ordinate
+ * groups, values[2] = 1 when the plan was chosen by timing the candidates at
+ * titus_create (0: the size heuristic, for problems too small to tune).
+ * Writes at most n values. */
+int titus_get_tuning(const struct titus_ctx* ctx, const char* kernel, int32_t* values, int32_t n);
+
 void titus_destroy(struct titus_ctx* ctx);
 const char* titus_last_error(void);
 int32_t titus_abi_version(void);

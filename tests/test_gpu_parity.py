@@ -176,6 +176,30 @@ def test_mixed_full_fp32_dose_error(name, monkeypatch):
     assert ed <= MIXED_TOL
 
 
+def test_tuned_sweep_plan():
+    """A context large enough to tune its sweep plan still matches the oracle."""
+    from titus_amd.problem import INTEGRATOR_FIXED, build_problem
+    from titus_amd.settings import Settings
+    s = Settings(161, 161, 8, "waterBeamElectrons", "Electrons", a=0.0, b=8.0, c=0.0, d=8.0,
+                 x0=4.0, y0=0.0, nPN=5, eMin=1.0, eMax=10.5, sigmaX=1.0, sigmaY=1.0,
+                 sigmaO1Inv=2.0, sigmaO3Inv=2.0)
+    P = build_problem(s, "subset", INTEGRATOR_FIXED)
+    announce("tuned sweep", P)
+    assert P["C"] * P["Q"] >= 2 ** 18
+    ctx = _lib.Context(P, _lib.INTEGRATOR_FIXED)
+    try:
+        plan = ctx.tuning("sweep")
+        print(f"[tuned sweep] plan {plan}", flush=True)
+        assert plan["tuned"] and plan["rows"] in (2, 4, 8, 16) and 1 <= plan["groups"] <= 8
+        ctx.solve(4)
+        g = ctx.state()
+    finally:
+        ctx.close()
+    o = dlra.solve_fixed(P, steps=4)
+    assert rel(g["dose"], o["dose"]) <= 1e-10
+    assert rel(g["psi"], o["psi"]) <= 1e-12
+
+
 def test_step_order_errors():
     P = cases.small_fixed()
     ctx = _lib.Context(P, _lib.INTEGRATOR_FIXED)

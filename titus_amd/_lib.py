@@ -26,8 +26,8 @@ ERRORS = {
 # every symbol include/titus_hip.h declares
 EXPORTED = ("titus_create", "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
             "titus_get_state",
-            "titus_get_history", "titus_set_timing", "titus_kernel_stats", "titus_destroy",
-            "titus_last_error", "titus_abi_version")
+            "titus_get_history", "titus_set_timing", "titus_kernel_stats", "titus_get_tuning",
+            "titus_destroy", "titus_last_error", "titus_abi_version")
 
 
 class TitusError(RuntimeError):
@@ -78,13 +78,14 @@ def load():
     lib.titus_set_timing.argtypes = [vp, ctypes.c_int32]
     lib.titus_kernel_stats.argtypes = [vp, ctypes.c_char_p, _P, ctypes.POINTER(ctypes.c_int64),
                                        _P, _P]
+    lib.titus_get_tuning.argtypes = [vp, ctypes.c_char_p, _PI, ctypes.c_int32]
     lib.titus_destroy.argtypes = [vp]
     lib.titus_destroy.restype = None
     lib.titus_last_error.restype = ctypes.c_char_p
     lib.titus_abi_version.restype = ctypes.c_int32
     for name in ("titus_create", "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
                  "titus_get_state", "titus_get_history", "titus_set_timing",
-                 "titus_kernel_stats"):
+                 "titus_kernel_stats", "titus_get_tuning"):
         getattr(lib, name).restype = ctypes.c_int
     if lib.titus_abi_version() != ABI_VERSION:
         raise RuntimeError("libtitus_hip.so ABI version mismatch")
@@ -216,6 +217,12 @@ class Context:
         check(self._lib.titus_kernel_stats(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n),
                                            ctypes.byref(by), ctypes.byref(fl)))
         return ms.value, n.value, by.value, fl.value
+
+    def tuning(self, kernel="sweep"):
+        """Launch plan of `kernel`: dict(rows, groups, tuned) for the sweep."""
+        v = np.zeros(3, dtype=np.int32)
+        check(self._lib.titus_get_tuning(self.h, kernel.encode(), _ip(v), 3))
+        return {"rows": int(v[0]), "groups": int(v[1]), "tuned": bool(v[2])}
 
     def close(self):
         if getattr(self, "h", None):

@@ -12,6 +12,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstddef>
+#include <vector>
 
 namespace titus {
 namespace dev {
@@ -30,12 +31,20 @@ struct Grid {
   const double* irho;           // C, 1 / rho
 };
 
+// Launch plan of the uncollided sweep: `rows` per wavefront strip (2, 4, 8
+// or 16) and `groups` of ordinates (1..kMaxSweepGroups) that run side by side.
 // The l = 0 moment u0[c] = sum_q psi[c, q] m0[q] is kept as G partial sums
-// u0p[g*C + c] (g < G = sweep_groups(...), at most kMaxSweepGroups), written
-// by the sweep and refreshed at the boundary cells by launch_bc; the dose
-// adds them up in order g = 0..G-1.
+// u0p[g*C + c] (g < G = sweep_groups(...)), written by the sweep and
+// refreshed at the boundary cells by launch_bc; the dose adds them up in
+// order g = 0..G-1.
 constexpr int kMaxSweepGroups = 8;
-int sweep_groups(const Grid& g, int nq, DT tpsi);
+struct SweepPlan {
+  int rows = 8;
+  int groups = 1;
+};
+SweepPlan sweep_default(const Grid& g, int nq, DT tpsi);
+std::vector<SweepPlan> sweep_candidates(const Grid& g, int nq, DT tpsi);
+int sweep_groups(const Grid& g, int nq, const SweepPlan& plan);
 
 // psi[bcells[b] + q*C] = bc_base[b + q*nb] * scale     (SolverCSD.jl:977-988),
 // and u0p of the boundary cells (partial 0 = their moment, the others 0)
@@ -52,7 +61,7 @@ void launch_dose(hipStream_t st, const Grid& g, double* dose, const double* X, i
 // Computes in the storage type of psi; writes the moment partials of pout.
 void launch_sweep(hipStream_t st, const Grid& g, const void* pin, void* pout, DT tpsi, int nq,
                   const double* omega, double dE, double denom, const unsigned char* restore,
-                  const double* m0, double* u0p);
+                  const double* m0, double* u0p, const SweepPlan& plan);
 
 // Streaming K-step (SolverCSD.jl:1020-1027) with the stencils applied matrix
 // free: K[c,:] = X[c,:] Z0 - dE * sum_{d in {c,e,w,n,s}} X[d,:]/rho[d] Zd on

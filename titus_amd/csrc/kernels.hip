@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace titus {
 namespace dev {
@@ -12,6 +13,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxTnSlots = 1024;    // workgroups (partial slots) of one TN reduction
+constexpr int kTnBTiles = 4;         // B tiles of one A^T B wavefront (shared A loads)
 constexpr int kXcds = 8;             // accelerator complex dies of an MI355X
 constexpr int kSweepCols = 62;       // output columns of one sweep wavefront
 constexpr int kRowsKT = 256;         // rows of T staged in LDS at a time (k_gemm)
@@ -236,30 +238,43 @@ template <> struct Mfma<float> {
 };
 
 // Tall-skinny A^T op(B) on the matrix cores.
-// partial[blk][o*mn + a + b*m] = sum over the cells of workgroup column blk of
+// partial[chunk][o*mn + a + b*m] = sum over the cells of `chunk` of
 // A[c, a] * op_o(B)[c, b] (mn = m*n).  NOPS = 1: op(B) = B; NOPS = 4: the
 // stencils L1x, L1y, L2x, L2y of B/rho (zero on ghost cells) side by side.
-// A workgroup is four wavefronts on one 16 x 16 tile (blockIdx.y) of the
-// output, each over its own run of `cpw` cells; K-slot k of MFMA step s is
-// cell c0 + 4k + s, so a lane walks four consecutive cells of its column.
-// The four wavefronts' sums are added in LDS in a fixed order.
-template <int NOPS, class TB, class TC>
+// A workgroup is four wavefronts on one tile group -- one 16-column A tile
+// and up to NBT 16-column B tiles -- each wavefront over its own run of `cpw`
+// cells of the chunk; K-slot k of MFMA step s is cell c0 + 4k + s, so a lane
+// walks four consecutive cells of its column, and the A values are loaded
+// once for all B tiles.  Workgroups are numbered tile group fastest and
+// XCD-contiguous, so neighbouring chunks (stencil halo rows) and the tile
+// groups of one chunk (shared A) run on the same XCD.  The four wavefronts'
+// sums are added in LDS in a fixed order.
+template <int NOPS, int NBT, class TB, class TC>
 __global__ __launch_bounds__(kBlock) void k_tn_mfma(
     const double* __restrict__ A, int lda, int m, const TB* __restrict__ B, int ldb, int n,
     int C, int nx, int ny, double idx_, double idy_, const double* __restrict__ irho, int cpw,
-    int tiles_m, double* __restrict__ partial) {
+    int ngroups, int groups_n, double* __restrict__ partial) {
   using MF = Mfma<TC>;
   using acc_t = typename MF::acc_t;
-  constexpr int NACC = NOPS == 1 ? 2 : NOPS;       // two chains for the plain product
+  constexpr int NACC = NOPS * NBT;
   __shared__ acc_t red[3][NACC][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int tm = blockIdx.y % tiles_m, tn = blockIdx.y / tiles_m;
+  const int L = xcd_block(blockIdx.x, gridDim.x);
+  const int grp = L % ngroups, chunk = L / ngroups;
+  const int tm = grp / groups_n, tn0 = (grp % groups_n) * NBT;
   const int col = lane & 15, kg = lane >> 4;
-  const int ca = tm * 16 + col, cb = tn * 16 + col;
-  const bool va = ca < m, vb = cb < n;
+  const int ca = tm * 16 + col;
+  const bool va = ca < m;
   const double* Acol = A + (size_t)(va ? ca : 0) * lda;
-  const TB* Bcol = B + (size_t)(vb ? cb : 0) * ldb;
-  const int cbeg = (blockIdx.x * 4 + wave) * cpw;
+  bool vb[NBT];
+  const TB* Bcol[NBT];
+#pragma unroll
+  for (int t = 0; t < NBT; ++t) {
+    const int cb = (tn0 + t) * 16 + col;
+    vb[t] = cb < n;
+    Bcol[t] = B + (size_t)(vb[t] ? cb : 0) * ldb;
+  }
+  const int cbeg = (chunk * 4 + wave) * cpw;
   const int cend = min(C, cbeg + cpw);
   const TC idx = (TC)idx_, idy = (TC)idy_, h = TC(0.5);
   acc_t acc[NACC];
@@ -267,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void k_tn_mfma(
   for (int o = 0; o < NACC; ++o) acc[o] = acc_t{};
   for (int c0 = cbeg; c0 < cend; c0 += 16) {
     const int cs = c0 + 4 * kg;
-    TC av[4], bv[NOPS][4];
+    TC av[4], bv[NACC][4];
     int ii = cs / ny, jj = cs - (cs / ny) * ny;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
@@ -275,32 +290,31 @@ __global__ __launch_bounds__(kBlock) void k_tn_mfma(
       const bool live = c < cend;
       av[s] = (va && live) ? (TC)Acol[c] : TC(0);
       if (NOPS == 1) {
-        bv[0][s] = (vb && live) ? (TC)Bcol[c] : TC(0);
+#pragma unroll
+        for (int t = 0; t < NBT; ++t) bv[t][s] = (vb[t] && live) ? (TC)Bcol[t][c] : TC(0);
       } else {
         while (jj >= ny) { jj -= ny; ++ii; }
-        const bool in = vb && live && ii >= 1 && ii <= nx - 2 && jj >= 1 && jj <= ny - 2;
+        const bool in = vb[0] && live && ii >= 1 && ii <= nx - 2 && jj >= 1 && jj <= ny - 2;
         TC vc = TC(0), vw = TC(0), ve = TC(0), vs = TC(0), vn = TC(0);
         if (in) {
-          vc = (TC)Bcol[c] * (TC)irho[c];
-          vw = (TC)Bcol[c - ny] * (TC)irho[c - ny];
-          ve = (TC)Bcol[c + ny] * (TC)irho[c + ny];
-          vs = (TC)Bcol[c - 1] * (TC)irho[c - 1];
-          vn = (TC)Bcol[c + 1] * (TC)irho[c + 1];
+          const TB* b = Bcol[0];
+          vc = (TC)b[c] * (TC)irho[c];
+          vw = (TC)b[c - ny] * (TC)irho[c - ny];
+          ve = (TC)b[c + ny] * (TC)irho[c + ny];
+          vs = (TC)b[c - 1] * (TC)irho[c - 1];
+          vn = (TC)b[c + 1] * (TC)irho[c + 1];
         }
-        bv[OP_L1X % NOPS][s] = (vc - h * vw - h * ve) * idx;
-        bv[OP_L1Y % NOPS][s] = (vc - h * vs - h * vn) * idy;
-        bv[OP_L2X % NOPS][s] = (ve - vw) * (h * idx);
-        bv[OP_L2Y % NOPS][s] = (vn - vs) * (h * idy);
+        bv[OP_L1X % NACC][s] = (vc - h * vw - h * ve) * idx;
+        bv[OP_L1Y % NACC][s] = (vc - h * vs - h * vn) * idy;
+        bv[OP_L2X % NACC][s] = (ve - vw) * (h * idx);
+        bv[OP_L2Y % NACC][s] = (vn - vs) * (h * idy);
         ++jj;
       }
     }
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
-      for (int o = 0; o < NOPS; ++o) {
-        const int t = NOPS == 1 ? (s & 1) : o;
-        acc[t] = MF::step(av[s], bv[o][s], acc[t]);
-      }
+      for (int o = 0; o < NACC; ++o) acc[o] = MF::step(av[s], bv[o][s], acc[o]);
   }
   if (wave > 0)
 #pragma unroll
@@ -311,16 +325,20 @@ __global__ __launch_bounds__(kBlock) void k_tn_mfma(
   for (int w = 0; w < 3; ++w)
 #pragma unroll
     for (int o = 0; o < NACC; ++o) acc[o] += red[w][o][lane];
-  if (NOPS == 1) acc[0] += acc[1];
   const size_t mn = (size_t)m * n;
 #pragma unroll
-  for (int o = 0; o < NOPS; ++o)
+  for (int o = 0; o < NACC; ++o) {
+    // NOPS = 1: accumulator o is B tile o; NOPS = 4 (one B tile): operator o
+    const int op = NOPS == 1 ? 0 : o, t = NOPS == 1 ? o : 0;
+    const int cb = (tn0 + t) * 16 + col;
+    if (!vb[t]) continue;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int a = tm * 16 + MF::row(kg, v);
-      if (a < m && vb)
-        partial[((size_t)blockIdx.x * NOPS + o) * mn + a + (size_t)cb * m] = (double)acc[o][v];
+      if (a < m)
+        partial[((size_t)chunk * NOPS + op) * mn + a + (size_t)cb * m] = (double)acc[o][v];
     }
+  }
 }
 
 // One wavefront per output: lane l sums chunks l, l+64, ...; then a fixed
@@ -467,25 +485,26 @@ __global__ void k_scatter_add(double* __restrict__ A, int lda, const double* __r
   A[(size_t)col * lda + row] += packed[2 * n + t];
 }
 
-template <int NOPS, class TB, class TC>
-void tn_mfma_launch(hipStream_t st, dim3 grid, const void* A, int lda, int m, const void* B,
-                    int ldb, int n, const Grid& g, int cpw, int tiles_m, double* partial) {
-  hipLaunchKernelGGL((k_tn_mfma<NOPS, TB, TC>), grid, dim3(kBlock), 0, st,
+template <int NOPS, int NBT, class TB, class TC>
+void tn_mfma_launch(hipStream_t st, int blocks, const void* A, int lda, int m, const void* B,
+                    int ldb, int n, const Grid& g, int cpw, int ngroups, int groups_n,
+                    double* partial) {
+  hipLaunchKernelGGL((k_tn_mfma<NOPS, NBT, TB, TC>), dim3(blocks), dim3(kBlock), 0, st,
                      static_cast<const double*>(A), lda, m, static_cast<const TB*>(B), ldb, n, g.C,
-                     g.nx, g.ny, 1.0 / g.dx, 1.0 / g.dy, g.irho, cpw, tiles_m, partial);
+                     g.nx, g.ny, 1.0 / g.dx, 1.0 / g.dy, g.irho, cpw, ngroups, groups_n, partial);
 }
 
-template <int NOPS>
-void tn_dispatch(hipStream_t st, dim3 grid, const void* A, DT ta, int lda, int m, const void* B,
-                 DT tb, int ldb, int n, const Grid& g, bool fp32, int cpw, int tiles_m,
-                 double* partial) {
+template <int NOPS, int NBT>
+void tn_dispatch(hipStream_t st, int blocks, const void* A, DT ta, int lda, int m, const void* B,
+                 DT tb, int ldb, int n, const Grid& g, bool fp32, int cpw, int ngroups,
+                 int groups_n, double* partial) {
   if (ta != DT::F64) throw std::runtime_error("tn: the left block is always FP64");
   if (tb == DT::F64) {
-    if (fp32) tn_mfma_launch<NOPS, double, float>(st, grid, A, lda, m, B, ldb, n, g, cpw, tiles_m, partial);
-    else tn_mfma_launch<NOPS, double, double>(st, grid, A, lda, m, B, ldb, n, g, cpw, tiles_m, partial);
+    if (fp32) tn_mfma_launch<NOPS, NBT, double, float>(st, blocks, A, lda, m, B, ldb, n, g, cpw, ngroups, groups_n, partial);
+    else tn_mfma_launch<NOPS, NBT, double, double>(st, blocks, A, lda, m, B, ldb, n, g, cpw, ngroups, groups_n, partial);
   } else {
-    if (fp32) tn_mfma_launch<NOPS, float, float>(st, grid, A, lda, m, B, ldb, n, g, cpw, tiles_m, partial);
-    else tn_mfma_launch<NOPS, float, double>(st, grid, A, lda, m, B, ldb, n, g, cpw, tiles_m, partial);
+    if (fp32) tn_mfma_launch<NOPS, NBT, float, float>(st, blocks, A, lda, m, B, ldb, n, g, cpw, ngroups, groups_n, partial);
+    else tn_mfma_launch<NOPS, NBT, float, double>(st, blocks, A, lda, m, B, ldb, n, g, cpw, ngroups, groups_n, partial);
   }
 }
 
@@ -512,44 +531,78 @@ void launch_dose(hipStream_t st, const Grid& g, double* dose, const double* X, i
 }
 
 namespace {
-template <class T>
-constexpr int sweep_rows() { return sizeof(T) == 8 ? 8 : 16; }
-
 struct SweepShape { int ntj, nsb, qper, G; };
 
-SweepShape sweep_shape(const Grid& g, int nq, DT tpsi) {
-  const int H = tpsi == DT::F64 ? sweep_rows<double>() : sweep_rows<float>();
+SweepShape sweep_shape(const Grid& g, int nq, const SweepPlan& p) {
   SweepShape s;
   s.ntj = (g.ny + kSweepCols - 1) / kSweepCols;
-  s.nsb = (g.nx + (kBlock / 64) * H - 1) / ((kBlock / 64) * H);
-  // ordinate groups: enough workgroups to fill the chip (about four per CU),
-  // at most kMaxSweepGroups moment partials
-  const int base = s.ntj * s.nsb;
-  int G = std::max(1, std::min(nq, (1024 + base - 1) / base));
-  G = std::min(G, kMaxSweepGroups);
-  s.qper = (nq + G - 1) / G;
+  s.nsb = (g.nx + (kBlock / 64) * p.rows - 1) / ((kBlock / 64) * p.rows);
+  s.qper = (nq + p.groups - 1) / p.groups;
   s.G = std::max(1, (nq + s.qper - 1) / s.qper);
   return s;
 }
+
+template <class T, int H>
+void sweep_launch(hipStream_t st, const Grid& g, const void* pin, void* pout, int nq,
+                  const double* omega, double dE, double denom, const unsigned char* restore,
+                  const double* m0, double* u0p, const SweepShape& s) {
+  hipLaunchKernelGGL((k_sweep<T, H>), dim3((unsigned)(s.ntj * s.nsb * s.G)), dim3(kBlock), 0, st,
+                     static_cast<const T*>(pin), static_cast<T*>(pout), g.nx, g.ny, nq, s.qper,
+                     omega, (T)g.dx, (T)g.dy, g.rho, (T)dE, (T)denom, restore, m0, u0p, s.ntj,
+                     s.nsb);
+}
 }  // namespace
 
-int sweep_groups(const Grid& g, int nq, DT tpsi) { return sweep_shape(g, nq, tpsi).G; }
+SweepPlan sweep_default(const Grid& g, int nq, DT tpsi) {
+  SweepPlan p;
+  p.rows = tpsi == DT::F64 ? 8 : 16;
+  // ordinate groups: enough workgroups to fill the chip (about four per CU)
+  const int ntj = (g.ny + kSweepCols - 1) / kSweepCols;
+  const int base = ntj * ((g.nx + (kBlock / 64) * p.rows - 1) / ((kBlock / 64) * p.rows));
+  p.groups = std::max(1, std::min({nq, kMaxSweepGroups, (1024 + base - 1) / base}));
+  return p;
+}
+
+std::vector<SweepPlan> sweep_candidates(const Grid& g, int nq, DT tpsi) {
+  (void)g;
+  (void)tpsi;
+  std::vector<SweepPlan> out;
+  for (int rows : {2, 4, 8, 16})
+    for (int groups : {1, 2, 4, 8})
+      if (groups <= std::max(1, nq)) out.push_back(SweepPlan{rows, groups});
+  return out;
+}
+
+int sweep_groups(const Grid& g, int nq, const SweepPlan& p) { return sweep_shape(g, nq, p).G; }
 
 void launch_sweep(hipStream_t st, const Grid& g, const void* pin, void* pout, DT tpsi, int nq,
                   const double* omega, double dE, double denom, const unsigned char* restore,
-                  const double* m0, double* u0p) {
+                  const double* m0, double* u0p, const SweepPlan& plan) {
   if (nq == 0) return;
-  const SweepShape s = sweep_shape(g, nq, tpsi);
-  const dim3 grid((unsigned)(s.ntj * s.nsb * s.G));
-  if (tpsi == DT::F64)
-    hipLaunchKernelGGL((k_sweep<double, sweep_rows<double>()>), grid, dim3(kBlock), 0, st,
-                       static_cast<const double*>(pin), static_cast<double*>(pout), g.nx, g.ny, nq,
-                       s.qper, omega, g.dx, g.dy, g.rho, dE, denom, restore, m0, u0p, s.ntj, s.nsb);
-  else
-    hipLaunchKernelGGL((k_sweep<float, sweep_rows<float>()>), grid, dim3(kBlock), 0, st,
-                       static_cast<const float*>(pin), static_cast<float*>(pout), g.nx, g.ny, nq,
-                       s.qper, omega, (float)g.dx, (float)g.dy, g.rho, (float)dE, (float)denom,
-                       restore, m0, u0p, s.ntj, s.nsb);
+  if (plan.groups < 1 || plan.groups > kMaxSweepGroups)
+    throw std::runtime_error("sweep: ordinate groups out of range");
+  const SweepShape s = sweep_shape(g, nq, plan);
+  const bool f64 = tpsi == DT::F64;
+  switch (plan.rows) {
+    case 2:
+      if (f64) sweep_launch<double, 2>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s);
+      else sweep_launch<float, 2>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s);
+      break;
+    case 4:
+      if (f64) sweep_launch<double, 4>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s);
+      else sweep_launch<float, 4>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s);
+      break;
+    case 8:
+      if (f64) sweep_launch<double, 8>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s);
+      else sweep_launch<float, 8>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s);
+      break;
+    case 16:
+      if (f64) sweep_launch<double, 16>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s);
+      else sweep_launch<float, 16>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s);
+      break;
+    default:
+      throw std::runtime_error("sweep: rows per strip must be 2, 4, 8 or 16");
+  }
   check_launch("sweep");
 }
 
@@ -586,20 +639,33 @@ void launch_tn(hipStream_t st, const Grid& g, const void* A, DT ta, int lda, int
   if (m == 0 || n == 0) return;
   if (op != OP_NONE && op != OP_ALL4) throw std::runtime_error("tn: unsupported operator");
   const int nops = (op == OP_ALL4) ? 4 : 1;
-  const int tiles_m = (m + 15) / 16, tiles = tiles_m * ((n + 15) / 16);
+  // B tiles per wavefront: the four stencil images of one tile, or up to
+  // kTnBTiles plain tiles sharing one A tile
+  const int tiles_m = (m + 15) / 16, tiles_n = (n + 15) / 16;
+  const int nbt = nops == 4 ? 1 : std::min(tiles_n, kTnBTiles);
+  const int groups_n = (tiles_n + nbt - 1) / nbt;
+  const int ngroups = tiles_m * groups_n;
   // cell runs of at least 64 cells (a multiple of 16) per wavefront; at most
-  // kMaxTnSlots workgroups in all, so the partials fit tn_partial_size()
-  int nchunks = std::max(1, std::min((g.C + 255) / 256, kMaxTnSlots / tiles));
+  // kMaxTnSlots workgroups, and partials within tn_partial_size()
+  const size_t per_chunk = (size_t)nops * m * n;
+  int nchunks = std::max(1, std::min((g.C + 255) / 256, kMaxTnSlots / ngroups));
+  nchunks = std::max(1, (int)std::min<size_t>(nchunks, tn_partial_size() / per_chunk));
   int cpw = (g.C + 4 * nchunks - 1) / (4 * nchunks);
   cpw = (cpw + 15) / 16 * 16;
   nchunks = (g.C + 4 * cpw - 1) / (4 * cpw);
-  if ((size_t)nchunks * nops * m * n > tn_partial_size())
+  if ((size_t)nchunks * per_chunk > tn_partial_size())
     throw std::runtime_error("tn: output too wide for the partial buffer");
-  const dim3 grid(nchunks, tiles);
-  if (nops == 4)
-    tn_dispatch<4>(st, grid, A, ta, lda, m, B, tb, ldb, n, g, fp32, cpw, tiles_m, partial);
-  else
-    tn_dispatch<1>(st, grid, A, ta, lda, m, B, tb, ldb, n, g, fp32, cpw, tiles_m, partial);
+  const int blocks = nchunks * ngroups;
+  if (nops == 4) {
+    tn_dispatch<4, 1>(st, blocks, A, ta, lda, m, B, tb, ldb, n, g, fp32, cpw, ngroups, groups_n, partial);
+  } else {
+    switch (nbt) {
+      case 1: tn_dispatch<1, 1>(st, blocks, A, ta, lda, m, B, tb, ldb, n, g, fp32, cpw, ngroups, groups_n, partial); break;
+      case 2: tn_dispatch<1, 2>(st, blocks, A, ta, lda, m, B, tb, ldb, n, g, fp32, cpw, ngroups, groups_n, partial); break;
+      case 3: tn_dispatch<1, 3>(st, blocks, A, ta, lda, m, B, tb, ldb, n, g, fp32, cpw, ngroups, groups_n, partial); break;
+      default: tn_dispatch<1, kTnBTiles>(st, blocks, A, ta, lda, m, B, tb, ldb, n, g, fp32, cpw, ngroups, groups_n, partial); break;
+    }
+  }
   check_launch("tn_partial");
   const int mn = m * n;
   const int per_block = kBlock / 64;

@@ -25,6 +25,8 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
+#include <tuple>
 #include <memory>
 #include <numeric>
 #include <stdexcept>
@@ -165,6 +167,8 @@ struct titus_ctx {
   double* dose = nullptr;
   double* u0p = nullptr;                    // l = 0 moment of psi, G partials x C
   int u0_groups = 1;
+  dev::SweepPlan sweep_plan{};
+  bool sweep_tuned = false;
   double *X = nullptr, *Xn = nullptr, *K = nullptr, *Q1 = nullptr, *Q2 = nullptr;
   double *partial = nullptr, *arena = nullptr, *upl = nullptr;
   int *bcells = nullptr, *cand_d = nullptr;
@@ -499,7 +503,7 @@ struct titus_ctx {
     const double denom = 1.0 + dE * sg[0];
     // psi read and written once, density, restore flags and moment partials
     timed("sweep", 2.0 * ps * C * nq + 9.0 * C + 8.0 * C * u0_groups, 14.0 * C * nq, [&] {
-      dev::launch_sweep(st, g, psi, psi2, tpsi, nq, omega, dE, denom, restore, m0, u0p);
+      dev::launch_sweep(st, g, psi, psi2, tpsi, nq, omega, dE, denom, restore, m0, u0p, sweep_plan);
     });
     std::swap(psi, psi2);
     Dvec.resize(nm);
@@ -721,6 +725,59 @@ struct titus_ctx {
 };
 
 // ---------------------------------------------------------------------------
+// Launch-plan autotuner of the uncollided sweep (the kernel that moves the
+// most bytes).  Each candidate plan (rows per strip x ordinate groups) is
+// timed with HIP events on the context's own, still zero, psi buffers; the
+// fastest is kept for every context of the same grid, ordinate count and
+// precision in this process.  Problems below kTuneMinWork cells x ordinates
+// are launch-latency bound and keep the heuristic plan.
+namespace {
+constexpr size_t kTuneMinWork = size_t(1) << 18;
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int>, dev::SweepPlan> g_tuned;
+}  // namespace
+
+static void tune_sweep(titus_ctx& x) {
+  x.sweep_plan = dev::sweep_default(x.g, x.nq, x.tpsi);
+  x.sweep_tuned = false;
+  if ((size_t)x.C * x.nq < kTuneMinWork) return;
+  const auto key = std::make_tuple(x.nx, x.ny, x.nq, x.mixed ? 1 : 0);
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  auto it = g_tuned.find(key);
+  if (it != g_tuned.end()) {
+    x.sweep_plan = it->second;
+    x.sweep_tuned = true;
+    return;
+  }
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  const double denom = 1.0 + x.dE * x.sigma[0];
+  float best = 0.f;
+  for (const dev::SweepPlan& c : dev::sweep_candidates(x.g, x.nq, x.tpsi)) {
+    auto run = [&] {
+      dev::launch_sweep(x.st, x.g, x.psi, x.psi2, x.tpsi, x.nq, x.omega, x.dE, denom, x.restore,
+                        x.m0, x.u0p, c);
+    };
+    run();
+    HIPCHK(hipEventRecord(a, x.st));
+    for (int rep = 0; rep < 3; ++rep) run();
+    HIPCHK(hipEventRecord(b, x.st));
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    if (!x.sweep_tuned || ms < best) {
+      best = ms;
+      x.sweep_plan = c;
+      x.sweep_tuned = true;
+    }
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  g_tuned[key] = x.sweep_plan;
+}
+
+// ---------------------------------------------------------------------------
 namespace {
 
 template <class F>
@@ -878,7 +935,10 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     HIPCHK(hipMemset(x.dose, 0, sizeof(double) * C));
     // psi starts at zero, so does its moment
     HIPCHK(hipMemset(x.u0p, 0, sizeof(double) * dev::kMaxSweepGroups * C));
-    x.u0_groups = dev::sweep_groups(x.g, p->nq, x.tpsi);
+    tune_sweep(x);
+    x.u0_groups = dev::sweep_groups(x.g, p->nq, x.sweep_plan);
+    HIPCHK(hipMemset(x.u0p, 0, sizeof(double) * dev::kMaxSweepGroups * C));
+    HIPCHK(hipMemset(x.psi2, 0, psz * (size_t)C * p->nq));
     HIPCHK(hipMemset(x.X, 0, sizeof(double) * cx));
 
     // initial factors (SolverCSD.jl:932-957): svd!(zeros) -> identity columns,
@@ -1026,6 +1086,15 @@ int titus_kernel_stats(const titus_ctx* cctx, const char* name, double* total_ms
     if (launches) *launches = found ? ctx->timing.launches[it->second] : 0;
     if (bytes) *bytes = found ? ctx->timing.total_bytes[it->second] : 0.0;
     if (flops) *flops = found ? ctx->timing.total_flops[it->second] : 0.0;
+  });
+}
+
+int titus_get_tuning(const titus_ctx* ctx, const char* kernel, int32_t* values, int32_t n) {
+  return guarded([&] {
+    require(ctx != nullptr && kernel != nullptr && (values != nullptr || n == 0), "bad argument");
+    require(std::string(kernel) == "sweep", "unknown kernel (tuned kernels: sweep)");
+    const int32_t v[3] = {ctx->sweep_plan.rows, ctx->sweep_plan.groups, ctx->sweep_tuned ? 1 : 0};
+    for (int i = 0; i < n && i < 3; ++i) values[i] = v[i];
   });
 }
 
