@@ -187,13 +187,23 @@ def stencil_scale(sync, steps=4, n=1024, calibrate=False):
             out[name] = res
         finally:
             ctx.close()
-    if calibrate:
-        import torch
+    # a plain 256 MiB device copy: the practical HBM ceiling on this GPU, and
+    # the reference kernel for scaling the PMC traffic counters
+    import torch
+    if torch.cuda.is_available():
         a = torch.empty(256 * 2 ** 20 // 8, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
         for _ in range(3):
             b.copy_(a)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        reps = 1 if calibrate else 20
+        for _ in range(reps):
+            b.copy_(a)
+        e1.record()
         torch.cuda.synchronize()
+        gbs = 2 * a.numel() * 8 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+        out["copy_256MiB"] = {"GB/s": round(gbs, 1), "frac_of_8TB/s": round(gbs / HBM_PEAK_GBS, 3)}
     return out
 
 

@@ -150,7 +150,8 @@ int titus_kernel_stats(const struct titus_ctx* ctx, const char* name, double* to
  * counterpart in the reference: its loops have no launch parameters).
  * kernel "sweep": values[0] = rows per wavefront strip, values[1] = ordinate
  * groups, values[2] = 1 when the plan was chosen by timing the candidates at
- * titus_create (0: the size heuristic, for problems too small to tune).
+ * titus_create (0: the size heuristic, for problems too small to tune),
+ * values[3] = 1 when psi streams through non-temporal loads and stores.
  * Writes at most n values. */
 int titus_get_tuning(const struct titus_ctx* ctx, const char* kernel, int32_t* values, int32_t n);
 

@@ -219,10 +219,11 @@ class Context:
         return ms.value, n.value, by.value, fl.value
 
     def tuning(self, kernel="sweep"):
-        """Launch plan of `kernel`: dict(rows, groups, tuned) for the sweep."""
-        v = np.zeros(3, dtype=np.int32)
-        check(self._lib.titus_get_tuning(self.h, kernel.encode(), _ip(v), 3))
-        return {"rows": int(v[0]), "groups": int(v[1]), "tuned": bool(v[2])}
+        """Launch plan of `kernel`: dict(rows, groups, tuned, nontemporal) for the sweep."""
+        v = np.zeros(4, dtype=np.int32)
+        check(self._lib.titus_get_tuning(self.h, kernel.encode(), _ip(v), 4))
+        return {"rows": int(v[0]), "groups": int(v[1]), "tuned": bool(v[2]),
+                "nontemporal": bool(v[3])}
 
     def close(self):
         if getattr(self, "h", None):

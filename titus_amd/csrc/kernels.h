@@ -32,7 +32,8 @@ struct Grid {
 };
 
 // Launch plan of the uncollided sweep: `rows` per wavefront strip (2, 4, 8
-// or 16) and `groups` of ordinates (1..kMaxSweepGroups) that run side by side.
+// or 16), `groups` of ordinates (1..kMaxSweepGroups) that run side by side,
+// and whether psi streams through non-temporal loads and stores.
 // The l = 0 moment u0[c] = sum_q psi[c, q] m0[q] is kept as G partial sums
 // u0p[g*C + c] (g < G = sweep_groups(...)), written by the sweep and
 // refreshed at the boundary cells by launch_bc; the dose adds them up in
@@ -41,6 +42,7 @@ constexpr int kMaxSweepGroups = 8;
 struct SweepPlan {
   int rows = 8;
   int groups = 1;
+  bool nontemporal = false;   // psi streamed with non-temporal loads and stores
 };
 SweepPlan sweep_default(const Grid& g, int nq, DT tpsi);
 std::vector<SweepPlan> sweep_candidates(const Grid& g, int nq, DT tpsi);

@@ -81,7 +81,7 @@ __global__ void k_dose(double* __restrict__ dose, const double* __restrict__ X, 
 // psi value is fetched once; the j-neighbours come from the adjacent lanes.
 // The new psi's l = 0 moment accumulates per cell and is written to
 // u0p[gq][c] at the end (the dose needs it after the sweep).
-template <class T, int H>
+template <class T, int H, bool NT>
 __global__ __launch_bounds__(kBlock) void k_sweep(
     const T* __restrict__ pin, T* __restrict__ pout, int nx, int ny, int nq, int qper,
     const double* __restrict__ omega, T dx, T dy, const double* __restrict__ rho, T dE, T denom,
@@ -115,7 +115,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep(
     for (int h = 0; h < H + 2; ++h) {
       const int i = i0 - 1 + h;
       const bool need = (h >= 1 && h <= H) || (h == 0 && fwd) || (h == H + 1 && !fwd);
-      P[h] = (need && jv && i >= 0 && i < nx) ? p[(size_t)i * ny + j] : T(0);
+      const T* src = p + (size_t)i * ny + j;
+      P[h] = (need && jv && i >= 0 && i < nx) ? (NT ? __builtin_nontemporal_load(src) : *src) : T(0);
     }
   };
   T P[H + 2], Pn[H + 2] = {};
@@ -145,7 +146,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep(
         } else {
           out = (q == 0 && restore[c]) ? P[h] : P[h] / denom;
         }
-        po[c] = out;
+        if (NT) __builtin_nontemporal_store(out, po + c);
+        else po[c] = out;
         acc[h - 1] += (double)out * mq;
       }
     }
@@ -545,8 +547,9 @@ SweepShape sweep_shape(const Grid& g, int nq, const SweepPlan& p) {
 template <class T, int H>
 void sweep_launch(hipStream_t st, const Grid& g, const void* pin, void* pout, int nq,
                   const double* omega, double dE, double denom, const unsigned char* restore,
-                  const double* m0, double* u0p, const SweepShape& s) {
-  hipLaunchKernelGGL((k_sweep<T, H>), dim3((unsigned)(s.ntj * s.nsb * s.G)), dim3(kBlock), 0, st,
+                  const double* m0, double* u0p, const SweepShape& s, bool nt) {
+  auto kern = nt ? k_sweep<T, H, true> : k_sweep<T, H, false>;
+  hipLaunchKernelGGL(kern, dim3((unsigned)(s.ntj * s.nsb * s.G)), dim3(kBlock), 0, st,
                      static_cast<const T*>(pin), static_cast<T*>(pout), g.nx, g.ny, nq, s.qper,
                      omega, (T)g.dx, (T)g.dy, g.rho, (T)dE, (T)denom, restore, m0, u0p, s.ntj,
                      s.nsb);
@@ -567,9 +570,10 @@ std::vector<SweepPlan> sweep_candidates(const Grid& g, int nq, DT tpsi) {
   (void)g;
   (void)tpsi;
   std::vector<SweepPlan> out;
-  for (int rows : {2, 4, 8, 16})
-    for (int groups : {1, 2, 4, 8})
-      if (groups <= std::max(1, nq)) out.push_back(SweepPlan{rows, groups});
+  for (int nt : {0, 1})
+    for (int rows : {2, 4, 8, 16})
+      for (int groups : {1, 2, 4, 8})
+        if (groups <= std::max(1, nq)) out.push_back(SweepPlan{rows, groups, nt != 0});
   return out;
 }
 
@@ -585,20 +589,20 @@ void launch_sweep(hipStream_t st, const Grid& g, const void* pin, void* pout, DT
   const bool f64 = tpsi == DT::F64;
   switch (plan.rows) {
     case 2:
-      if (f64) sweep_launch<double, 2>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s);
-      else sweep_launch<float, 2>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s);
+      if (f64) sweep_launch<double, 2>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal);
+      else sweep_launch<float, 2>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal);
       break;
     case 4:
-      if (f64) sweep_launch<double, 4>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s);
-      else sweep_launch<float, 4>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s);
+      if (f64) sweep_launch<double, 4>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal);
+      else sweep_launch<float, 4>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal);
       break;
     case 8:
-      if (f64) sweep_launch<double, 8>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s);
-      else sweep_launch<float, 8>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s);
+      if (f64) sweep_launch<double, 8>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal);
+      else sweep_launch<float, 8>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal);
       break;
     case 16:
-      if (f64) sweep_launch<double, 16>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s);
-      else sweep_launch<float, 16>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s);
+      if (f64) sweep_launch<double, 16>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal);
+      else sweep_launch<float, 16>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal);
       break;
     default:
       throw std::runtime_error("sweep: rows per strip must be 2, 4, 8 or 16");

@@ -1093,8 +1093,9 @@ int titus_get_tuning(const titus_ctx* ctx, const char* kernel, int32_t* values, 
   return guarded([&] {
     require(ctx != nullptr && kernel != nullptr && (values != nullptr || n == 0), "bad argument");
     require(std::string(kernel) == "sweep", "unknown kernel (tuned kernels: sweep)");
-    const int32_t v[3] = {ctx->sweep_plan.rows, ctx->sweep_plan.groups, ctx->sweep_tuned ? 1 : 0};
-    for (int i = 0; i < n && i < 3; ++i) values[i] = v[i];
+    const int32_t v[4] = {ctx->sweep_plan.rows, ctx->sweep_plan.groups, ctx->sweep_tuned ? 1 : 0,
+                          ctx->sweep_plan.nontemporal ? 1 : 0};
+    for (int i = 0; i < n && i < 4; ++i) values[i] = v[i];
   });
 }
 
