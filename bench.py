@@ -249,6 +249,8 @@ def main():
                     help="beamlets of the batched plan (0: skip)")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="host workers of the beamlet batch")
+    ap.add_argument("--no-stencil", action="store_true",
+                    help="skip the large-grid stencil measurement")
     ap.add_argument("--stencil-only", action="store_true",
                     help="only the large-grid stencil measurement (+ calibration copy), for PMC runs")
     args = ap.parse_args()
@@ -326,7 +328,8 @@ def main():
         out["roofline_fp64"] = roofline(f64["stats"])
     if args.beamlets > 0:
         out["beamlets"] = beamlet_batch(args.beamlets, args.threads, sync)
-    out["stencil_scale"] = stencil_scale(sync)
+    if not args.no_stencil:
+        out["stencil_scale"] = stencil_scale(sync)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(P, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
