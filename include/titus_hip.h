@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define TITUS_ABI_VERSION 1
+#define TITUS_ABI_VERSION 2
 
 typedef struct titus_ctx titus_ctx;
 
@@ -58,10 +58,10 @@ enum { TITUS_PRECISION_FP64 = 0, TITUS_PRECISION_MIXED = 1, TITUS_PRECISION_MIXE
  * ordinate reduction and initial factors of the Solve* functions
  * (code/SolverCSD.jl:63-146, 904-961, 2025-2082). */
 typedef struct titus_problem_v1 {
-  int32_t version;            /* TITUS_ABI_VERSION */
+  int32_t version;            /* TITUS_ABI_VERSION (1 accepted for 2D grids) */
   int32_t nx, ny;             /* NCellsX, NCellsY */
   double dx, dy;              /* settings.dx, settings.dy */
-  const double* density;      /* C = nx*ny, densityVec (SolverCSD.jl:143) */
+  const double* density;      /* C = nx*ny*nz, densityVec (SolverCSD.jl:143) */
 
   int32_t nm;                 /* moments carried */
   const int32_t* degree;      /* nm: Legendre degree l of each moment */
@@ -98,6 +98,14 @@ typedef struct titus_problem_v1 {
   const double* X0;           /* C x r  (NULL: svd!(zeros) start, ghost rows 0) */
   const double* S0;           /* r x r  (NULL: zeros) */
   const double* W0;           /* nm x r (NULL: identity columns) */
+
+  /* version 2: a third spatial axis.  The reference is 2D; a 3D grid has
+   * cells c = (i*ny + j)*nz + k with the k axis paired with Omega_y (Ay).
+   * Read only when version >= 2; version-1 callers describe 2D grids. */
+  int32_t nz;                 /* cells along k (1: 2D) */
+  double dz;
+  const double* Ay;           /* nm x nm  pn.Ay (PNSystem.jl:142-194), nz > 1 only */
+  const double* absAy;        /* nm x nm  |Ay| (Roe, as SolverCSD.jl:83-97), nz > 1 only */
 } titus_problem_v1;
 
 /* SolverCSD(settings): validate, copy operands to device `device`. */

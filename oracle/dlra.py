@@ -12,6 +12,9 @@ It restates, in numpy with T = Float64 throughout:
 * ``truncate``     -- ``truncate!``                       (``code/SolverCSD.jl:3369-3399``)
 * ``upwind_flux``  -- ``solveFluxUpwind!``                (``code/SolverCSD.jl:388-463``)
 * ``stencils``     -- ``Stencils`` applied matrix-free    (``code/stencils.jl:12-130``)
+* ``stencils3``, ``upwind_flux3`` -- their 3D extension (no reference: the
+                      snapshot is 2D; pinned by dense assembly, axis-permutation
+                      symmetry and the nz = 1 limit in ``tests/test_oracle3d.py``)
 * ``solve_dense``  -- the same splitting on the full C x Nm moment matrix
                       (the collided part of ``SolveFirstCollisionSource``,
                       ``SolverCSD.jl:465-586``, with the DLR step's conventions)
@@ -63,6 +66,45 @@ def stencils(U, rho, nx, ny, dx, dy):
     out[2][1:-1, 1:-1] = (e - w) / (2 * dx)
     out[3][1:-1, 1:-1] = (n - s) / (2 * dy)
     return [o.reshape(nx * ny, m) for o in out]
+
+
+def stencils3(U, rho, nx, ny, nz, dx, dy, dz):
+    """3D extension: (L1x, L1y, L1z, L2x, L2y, L2z) U on cells c = (i*ny + j)*nz + k.
+
+    Same 3-point forms as ``stencils`` along each axis (x: i, y: j, z: k);
+    interior cells only (all three indices in 1..n-2), ghost rows 0.
+    """
+    m = U.shape[1]
+    V = (U / rho[:, None]).reshape(nx, ny, nz, m)
+    out = [np.zeros((nx, ny, nz, m)) for _ in range(6)]
+    c = V[1:-1, 1:-1, 1:-1]
+    xm, xp = V[:-2, 1:-1, 1:-1], V[2:, 1:-1, 1:-1]
+    ym, yp = V[1:-1, :-2, 1:-1], V[1:-1, 2:, 1:-1]
+    zm, zp = V[1:-1, 1:-1, :-2], V[1:-1, 1:-1, 2:]
+    out[0][1:-1, 1:-1, 1:-1] = (c - 0.5 * xm - 0.5 * xp) / dx
+    out[1][1:-1, 1:-1, 1:-1] = (c - 0.5 * ym - 0.5 * yp) / dy
+    out[2][1:-1, 1:-1, 1:-1] = (c - 0.5 * zm - 0.5 * zp) / dz
+    out[3][1:-1, 1:-1, 1:-1] = (xp - xm) / (2 * dx)
+    out[4][1:-1, 1:-1, 1:-1] = (yp - ym) / (2 * dy)
+    out[5][1:-1, 1:-1, 1:-1] = (zp - zm) / (2 * dz)
+    return [o.reshape(nx * ny * nz, m) for o in out]
+
+
+def upwind_flux3(psi, rho, omega, nx, ny, nz, dx, dy, dz):
+    """3D first-order upwind flux of psi/rho: axis x (i) with Omega_x, y (j)
+    with Omega_z, z (k) with Omega_y -- the 2D pairing plus the third axis."""
+    Q = psi.shape[1]
+    V = (psi / rho[:, None]).reshape(nx, ny, nz, Q)
+    F = np.zeros((nx, ny, nz, Q))
+    c = V[1:-1, 1:-1, 1:-1]
+    parts = []
+    for (lo, hi, om, h) in (
+            (V[:-2, 1:-1, 1:-1], V[2:, 1:-1, 1:-1], omega[:, 0], dx),
+            (V[1:-1, :-2, 1:-1], V[1:-1, 2:, 1:-1], omega[:, 2], dy),
+            (V[1:-1, 1:-1, :-2], V[1:-1, 1:-1, 2:], omega[:, 1], dz)):
+        parts.append(np.where(om >= 0, c - lo, hi - c) * (om / h))
+    F[1:-1, 1:-1, 1:-1] = parts[0] + parts[1] + parts[2]
+    return F.reshape(nx * ny * nz, Q)
 
 
 def upwind_flux(psi, rho, omega, nx, ny, dx, dy):
@@ -194,48 +236,82 @@ def truncate(X, S, W, theta, theta_index, rank_cap, r_min=2):
 # energy loop
 # ---------------------------------------------------------------------------
 class _Ops:
+    """Spatial and angular operators of one problem.
+
+    2D (``nz`` absent or 1): the four operators L1x, L1y, L2x, L2y paired
+    with |Ax|, |Az|, Ax, Az.  3D: L1x, L1y, L1z, L2x, L2y, L2z paired with
+    |Ax|, |Az|, |Ay|, Ax, Az, Ay (``titus_amd/problem3d.py``).  The ops lists
+    hold the dissipative L1 operators first, then the central L2 ones.
+    """
+
     def __init__(self, P):
         self.P = P
         self.nx, self.ny, self.dx, self.dy = P["nx"], P["ny"], P["dx"], P["dy"]
+        self.nz, self.dz = int(P.get("nz", 1)), P.get("dz", 1.0)
+        self.nd = 3 if self.nz > 1 else 2
         self.rho = np.asarray(P["density"], dtype=np.float64)
         self.bcells = np.asarray(P["bcells"])
         self.interior = np.asarray(P["interior"])
         self.restore = np.asarray(P["restore_cells"])
         self.Ax, self.Az = np.asarray(P["Ax"]), np.asarray(P["Az"])
         self.aAx, self.aAz = np.asarray(P["absAx"]), np.asarray(P["absAz"])
+        if self.nd == 3:
+            self.Ay, self.aAy = np.asarray(P["Ay"]), np.asarray(P["absAy"])
+            self.As = [self.aAx, self.aAz, self.aAy, self.Ax, self.Az, self.Ay]
+        else:
+            self.As = [self.aAx, self.aAz, self.Ax, self.Az]
         self.M = np.asarray(P["M"])
         self.deg = np.asarray(P["degree"])
         self.Nm = P["Nm"]
 
     def L(self, U):
+        if self.nd == 3:
+            return stencils3(U, self.rho, self.nx, self.ny, self.nz, self.dx, self.dy, self.dz)
         return stencils(U, self.rho, self.nx, self.ny, self.dx, self.dy)
 
+    def _order(self):
+        """Term order of the streaming sums: central terms, then dissipative."""
+        nd = self.nd
+        return [nd + d for d in range(nd)] + list(range(nd))
+
     def galerkin_x(self, Xa, Xb):
-        """[Xa^T L1x Xb, Xa^T L1y Xb, Xa^T L2x Xb, Xa^T L2y Xb]."""
+        """[Xa^T L_o Xb for every operator o]."""
         return [Xa.T @ LX for LX in self.L(Xb)]
 
     def galerkin_w(self, Wa, Wb):
-        return [Wa.T @ self.aAx @ Wb, Wa.T @ self.aAz @ Wb,
-                Wa.T @ self.Ax @ Wb, Wa.T @ self.Az @ Wb]
+        return [Wa.T @ A @ Wb for A in self.As]
 
     def stream_K(self, K, WAW, dE):
-        """K - dE (L2x K WAxW + L2y K WAzW + L1x K W|Ax|W + L1y K W|Az|W)."""
-        L1x, L1y, L2x, L2y = self.L(K)
-        return K - dE * (L2x @ WAW[2] + L2y @ WAW[3] + L1x @ WAW[0] + L1y @ WAW[1])
+        """K - dE sum_o L_o K (W^T A_o W) (2D: L2x, L2y, L1x, L1y)."""
+        Ls = self.L(K)
+        o = self._order()
+        acc = Ls[o[0]] @ WAW[o[0]]
+        for t in o[1:]:
+            acc = acc + Ls[t] @ WAW[t]
+        return K - dE * acc
 
     def stream_L(self, L, XLX, dE):
-        """L - dE (Ax L XL2xX^T + Az L XL2yX^T + |Ax| L XL1xX^T + |Az| L XL1yX^T)."""
-        return L - dE * (self.Ax @ L @ XLX[2].T + self.Az @ L @ XLX[3].T
-                         + self.aAx @ L @ XLX[0].T + self.aAz @ L @ XLX[1].T)
+        """L - dE sum_o A_o L (X^T L_o X)^T."""
+        o = self._order()
+        acc = self.As[o[0]] @ L @ XLX[o[0]].T
+        for t in o[1:]:
+            acc = acc + self.As[t] @ L @ XLX[t].T
+        return L - dE * acc
 
     def stream_S(self, S, XLX, WAW, dE):
-        return S - dE * (XLX[2] @ S @ WAW[2] + XLX[3] @ S @ WAW[3]
-                         + XLX[0] @ S @ WAW[0] + XLX[1] @ S @ WAW[1])
+        o = self._order()
+        acc = XLX[o[0]] @ S @ WAW[o[0]]
+        for t in o[1:]:
+            acc = acc + XLX[t] @ S @ WAW[t]
+        return S - dE * acc
 
 
 def _sweep(ops, psi, sig0, dE, omega):
     """Uncollided update with the reference's boundary restore (Appendix quirk Q2)."""
-    flux = upwind_flux(psi, ops.rho, omega, ops.nx, ops.ny, ops.dx, ops.dy)
+    if ops.nd == 3:
+        flux = upwind_flux3(psi, ops.rho, omega, ops.nx, ops.ny, ops.nz, ops.dx, ops.dy, ops.dz)
+    else:
+        flux = upwind_flux(psi, ops.rho, omega, ops.nx, ops.ny, ops.dx, ops.dy)
     keep = psi[ops.restore, 0].copy()
     psi = (psi - dE * flux) / (1.0 + dE * sig0)
     psi[ops.restore, 0] = keep
@@ -382,8 +458,11 @@ def solve_bug(P, steps=None):
             # eta diagnostic (:2210-2220)
             XLXt = ops.galerkin_x(Xt, X)
             WAWt = ops.galerkin_w(W, Wt)
-            eta = np.linalg.norm(XLXt[2] @ S @ WAWt[2] + XLXt[3] @ S @ WAWt[3]
-                                 + XLXt[0] @ S @ WAWt[0] + XLXt[1] @ S @ WAWt[1])
+            o = ops._order()
+            acc = XLXt[o[0]] @ S @ WAWt[o[0]]
+            for t in o[1:]:
+                acc = acc + XLXt[t] @ S @ WAWt[t]
+            eta = np.linalg.norm(acc)
             etas.append(eta)
             S, X, W = SUP, XUP, WUP
             r = S.shape[0]
@@ -435,8 +514,12 @@ def solve_dense(P, steps=None, stream=True):
         Dvec = sig[ops.deg]
         if s > 1 and stream:
             u[ops.bcells, :] = 0.0
-            L1x, L1y, L2x, L2y = ops.L(u)
-            u = u - dE * (L2x @ ops.Ax + L2y @ ops.Az + L1x @ ops.aAx + L1y @ ops.aAz)
+            Ls = ops.L(u)
+            o = ops._order()
+            acc = Ls[o[0]] @ ops.As[o[0]]
+            for t in o[1:]:
+                acc = acc + Ls[t] @ ops.As[t]
+            u = u - dE * acc
         u = u / (1.0 + dE * (sig[0] - Dvec))[None, :]
         u[ops.bcells, :] = 0.0
         u = u + dE * psi @ (M.T * Dvec[None, :])

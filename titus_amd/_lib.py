@@ -11,7 +11,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtitus_hip.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 INTEGRATOR_FIXED, INTEGRATOR_BUG = 0, 1
 PRECISION_FP64, PRECISION_MIXED, PRECISION_MIXED_FULL_FP32 = 0, 1, 2
 
@@ -53,6 +53,7 @@ class ProblemV1(ctypes.Structure):
         ("rank_cap", ctypes.c_int32), ("theta", ctypes.c_double),
         ("theta_index", ctypes.c_double), ("rank_tol", ctypes.c_double),
         ("X0", _P), ("S0", _P), ("W0", _P),
+        ("nz", ctypes.c_int32), ("dz", ctypes.c_double), ("Ay", _P), ("absAy", _P),
     ]
 
 
@@ -139,6 +140,10 @@ def make_problem(P, integrator, precision=PRECISION_FP64, rank_tol=1e-7, initial
     s.integrator, s.precision = integrator, precision
     s.r, s.rank_cap = P["r"], P["rank_cap"]
     s.theta, s.theta_index, s.rank_tol = P["theta"], P["thetaIndex"], rank_tol
+    s.nz, s.dz = int(P.get("nz", 1)), float(P.get("dz", 1.0))
+    if s.nz > 1:
+        keep["Ay"], keep["absAy"] = f64(P["Ay"]), f64(P["absAy"])
+        s.Ay, s.absAy = _dp(keep["Ay"]), _dp(keep["absAy"])
     if initial is not None:
         for name in ("X0", "S0", "W0"):
             keep[name] = f64(initial[name])

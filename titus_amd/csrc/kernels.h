@@ -17,18 +17,24 @@
 namespace titus {
 namespace dev {
 
-// OP_ALL4 computes the four stencil products at once (output: 4 consecutive
-// m x n blocks in the order L1x, L1y, L2x, L2y).
-enum StencilOp { OP_NONE = -1, OP_L1X = 0, OP_L1Y = 1, OP_L2X = 2, OP_L2Y = 3, OP_ALL4 = 4 };
+// OP_ALL computes all stencil products of the grid at once: 2D, four
+// consecutive m x n blocks L1x, L1y, L2x, L2y; 3D, six blocks L1x, L1y, L1z,
+// L2x, L2y, L2z (dissipative operators first, then central ones).
+enum StencilOp { OP_NONE = -1, OP_ALL = 4 };
 
 // element type of a device block
 enum class DT { F64, F32 };
 
+// Structured grid; cells c = (i*ny + j)*nz + k, axis strides x: ny*nz,
+// y: nz, z: 1.  A 2D grid has nz = 1.
 struct Grid {
   int nx, ny, C;
   double dx, dy;
   const double* rho;            // C
   const double* irho;           // C, 1 / rho
+  int nz = 1;
+  double dz = 1.0;
+  int ndim() const { return nz > 1 ? 3 : 2; }
 };
 
 // Launch plan of the uncollided sweep: `rows` per wavefront strip (2, 4, 8
@@ -73,11 +79,11 @@ void launch_stream_k(hipStream_t st, const Grid& g, const double* X, int ldx, in
                      const double* Z, int n, double dE, double* K, int ldk, bool fp32);
 
 // Tall-skinny A^T op(B) (m x n) with deterministic two-stage reduction.
-// op = OP_NONE: B itself; OP_ALL4: the four stencil operators applied to B
+// op = OP_NONE: B itself; OP_ALL: the 2*ndim stencil operators applied to B
 // (the Galerkin products X^T L X of SolverCSD.jl:1035-1038).  `fp32`:
 // compute and accumulate in FP32.  `partial` must hold tn_partial_size() doubles.
 // Products run on the matrix cores (16 x 16 x 4 FP64 or FP32 MFMA tiles);
-// op is OP_NONE or OP_ALL4.
+// op is OP_NONE or OP_ALL.
 size_t tn_partial_size();
 void launch_tn(hipStream_t st, const Grid& g, const void* A, DT ta, int lda, int m, const void* B,
                DT tb, int ldb, int n, int op, bool fp32, double* partial, double* out);

@@ -163,50 +163,63 @@ __global__ __launch_bounds__(kBlock) void k_sweep(
 
 // K[c,b] = X[c,:] Z0[:,b] - dE * sum_d (X[d,:]/rho[d]) Zd[:,b] (stencil part on
 // interior c only).  One thread per cell computes NB output columns b0.., so
-// every X value is loaded once per column block; the six r x NB coefficient
-// blocks sit in LDS as Zs[(a*6 + d)*NB + k] and are read as broadcasts.
-// Ghost cells use themselves as neighbours with 1/rho = 0, so their stencil
-// sum is exactly zero and K = X Z0 there.
-template <class TC, int NB>
+// every X value is loaded once per column block; the 2*ND + 2 coefficient
+// blocks (Z0, centre, then + and - neighbour of each axis) sit in LDS as
+// Zs[(a*NZB + d)*NB + k] and are read as broadcasts.  Axis strides: x = ny*nz,
+// y = nz, z = 1 (ND = 2: nz = 1).  Ghost cells use themselves as neighbours
+// with 1/rho = 0, so their stencil sum is exactly zero and K = X Z0 there.
+template <class TC, int NB, int ND>
 __global__ __launch_bounds__(kBlock) void k_stream_k(
-    const double* __restrict__ X, int ldx, int r, int nx, int ny, const double* __restrict__ rho,
-    const double* __restrict__ Z, int n, double dE, double* __restrict__ K, int ldk) {
+    const double* __restrict__ X, int ldx, int r, int nx, int ny, int nz,
+    const double* __restrict__ rho, const double* __restrict__ Z, int n, double dE,
+    double* __restrict__ K, int ldk) {
+  constexpr int NZB = 2 * ND + 2;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   TC* Zs = reinterpret_cast<TC*>(lds_raw);
-  const int C = nx * ny;
+  const int C = nx * ny * nz;
   const int b0 = blockIdx.y * NB;
   const int nbv = min(NB, n - b0);
-  for (int e = threadIdx.x; e < r * 6 * NB; e += blockDim.x) {
-    const int k = e % NB, d = (e / NB) % 6, a = e / (6 * NB);
+  for (int e = threadIdx.x; e < r * NZB * NB; e += blockDim.x) {
+    const int k = e % NB, d = (e / NB) % NZB, a = e / (NZB * NB);
     Zs[e] = k < nbv ? (TC)Z[((size_t)d * n + b0 + k) * r + a] : TC(0);
   }
   __syncthreads();
   const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (c >= C) return;
-  const int i = c / ny, j = c - (c / ny) * ny;
-  const bool in = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
-  const int ce = in ? c + ny : c, cw = in ? c - ny : c, cn = in ? c + 1 : c, cs = in ? c - 1 : c;
-  const TC ic = in ? (TC)(1.0 / rho[c]) : TC(0), ie = in ? (TC)(1.0 / rho[ce]) : TC(0);
-  const TC iw = in ? (TC)(1.0 / rho[cw]) : TC(0), in_ = in ? (TC)(1.0 / rho[cn]) : TC(0);
-  const TC is = in ? (TC)(1.0 / rho[cs]) : TC(0);
+  const int syz = ny * nz;
+  const int i = c / syz, rem = c - (c / syz) * syz;
+  const int j = rem / nz, kz = rem - (rem / nz) * nz;
+  bool in = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
+  if (ND == 3) in = in && kz >= 1 && kz <= nz - 2;
+  const int stride[3] = {syz, nz, 1};
+  int nb[2 * ND];
+  TC inv[2 * ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    nb[2 * d] = in ? c + stride[d] : c;
+    nb[2 * d + 1] = in ? c - stride[d] : c;
+    inv[2 * d] = in ? (TC)(1.0 / rho[nb[2 * d]]) : TC(0);
+    inv[2 * d + 1] = in ? (TC)(1.0 / rho[nb[2 * d + 1]]) : TC(0);
+  }
+  const TC ic = in ? (TC)(1.0 / rho[c]) : TC(0);
   TC a0[NB], as[NB];
 #pragma unroll
   for (int k = 0; k < NB; ++k) { a0[k] = TC(0); as[k] = TC(0); }
   for (int a = 0; a < r; ++a) {
     const double* xa = X + (size_t)a * ldx;
     const TC x0 = (TC)xa[c];
-    const TC xc = x0 * ic, xe = (TC)xa[ce] * ie, xw = (TC)xa[cw] * iw;
-    const TC xn = (TC)xa[cn] * in_, xs = (TC)xa[cs] * is;
-    const TC* z = Zs + (size_t)a * 6 * NB;
+    const TC xc = x0 * ic;
+    TC xn[2 * ND];
+#pragma unroll
+    for (int d = 0; d < 2 * ND; ++d) xn[d] = (TC)xa[nb[d]] * inv[d];
+    const TC* z = Zs + (size_t)a * NZB * NB;
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
       a0[k] += x0 * z[k];
       TC s = as[k];
       s += xc * z[NB + k];
-      s += xe * z[2 * NB + k];
-      s += xw * z[3 * NB + k];
-      s += xn * z[4 * NB + k];
-      s += xs * z[5 * NB + k];
+#pragma unroll
+      for (int d = 0; d < 2 * ND; ++d) s += xn[d] * z[(2 + d) * NB + k];
       as[k] = s;
     }
   }
@@ -214,6 +227,96 @@ __global__ __launch_bounds__(kBlock) void k_stream_k(
 #pragma unroll
   for (int k = 0; k < NB; ++k)
     if (k < nbv) K[(size_t)(b0 + k) * ldk + c] = (double)(a0[k] - tdE * as[k]);
+}
+
+// 3D uncollided upwind update.  A wavefront owns a strip of H rows along x
+// (index i, walked in the upwind order of Omega_x as in 2D) by kSweepCols
+// cells along z (index k, lanes 1..kSweepCols; lanes 0 and 63 carry the
+// neighbouring columns) at one y index j.  The x-neighbour comes from the
+// strip registers, the z-neighbours from adjacent lanes, and the upwind
+// y-neighbour (j -+ 1, stride nz) is loaded per row.  Axis pairing: x with
+// Omega_x (omega[q]), y with Omega_z (omega[2 nq + q]), z with Omega_y
+// (omega[nq + q]).  Ordinate groups and the fused l = 0 moment as in 2D.
+template <class T, int H>
+__global__ __launch_bounds__(kBlock) void k_sweep3(
+    const T* __restrict__ pin, T* __restrict__ pout, int nx, int ny, int nz, int nq, int qper,
+    const double* __restrict__ omega, T dx, T dy, T dz, const double* __restrict__ rho, T dE,
+    T denom, const unsigned char* __restrict__ restore, const double* __restrict__ m0,
+    double* __restrict__ u0p, int ntk, int njb, int nsb) {
+  const size_t C = (size_t)nx * ny * nz;
+  const int L = xcd_block(blockIdx.x, gridDim.x);
+  const int kt = L % ntk, jb = (L / ntk) % njb, sb = (L / (ntk * njb)) % nsb;
+  const int gq = L / (ntk * njb * nsb);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = jb * (kBlock / 64) + wave;
+  const int i0 = sb * H;
+  const int k = kt * kSweepCols - 1 + lane;
+  const bool jv = j < ny;
+  const bool kv = jv && k >= 0 && k < nz;
+  const bool owner = lane >= 1 && lane <= kSweepCols && kv;
+  const bool kint = k >= 1 && k <= nz - 2;
+  const bool jint = j >= 1 && j <= ny - 2;
+  const size_t syz = (size_t)ny * nz;
+  auto cell = [&](int ii, int jj) { return (size_t)ii * syz + (size_t)jj * nz + k; };
+  T ir[H + 2];
+#pragma unroll
+  for (int h = 0; h < H + 2; ++h) {
+    const int i = i0 - 1 + h;
+    ir[h] = (kv && i >= 0 && i < nx) ? (T)(1.0 / rho[cell(i, j)]) : T(0);
+  }
+  double acc[H];
+#pragma unroll
+  for (int h = 0; h < H; ++h) acc[h] = 0.0;
+  const int qa = gq * qper, qb = min(nq, qa + qper);
+  for (int q = qa; q < qb; ++q) {
+    const T ox = (T)omega[q], oy = (T)omega[2 * nq + q], oz = (T)omega[nq + q];
+    const T cx = ox / dx, cy = oy / dy, cz = oz / dz;
+    const bool fx = ox >= T(0), fy = oy >= T(0);
+    const int jn = fy ? j - 1 : j + 1;              // upwind y neighbour
+    const T* p = pin + (size_t)q * C;
+    T* po = pout + (size_t)q * C;
+    const double mq = m0[q];
+    T P[H + 2], v[H + 2], vyn[H];
+#pragma unroll
+    for (int h = 0; h < H + 2; ++h) {
+      const int i = i0 - 1 + h;
+      const bool need = (h >= 1 && h <= H) || (h == 0 && fx) || (h == H + 1 && !fx);
+      P[h] = (need && kv && i >= 0 && i < nx) ? p[cell(i, j)] : T(0);
+      v[h] = P[h] * ir[h];
+    }
+#pragma unroll
+    for (int h = 1; h <= H; ++h) {
+      const int i = i0 - 1 + h;
+      const bool ok = kv && jint && i >= 1 && i <= nx - 2;
+      vyn[h - 1] = ok ? p[cell(i, jn)] * (T)(1.0 / rho[cell(i, jn)]) : T(0);
+    }
+#pragma unroll
+    for (int h = 1; h <= H; ++h) {
+      const int i = i0 - 1 + h;
+      const T vlo = __shfl_up(v[h], 1, 64);         // k - 1
+      const T vhi = __shfl_down(v[h], 1, 64);       // k + 1
+      if (owner && i < nx) {
+        const size_t c = cell(i, j);
+        T out;
+        if (jint && kint && i >= 1 && i <= nx - 2) {
+          const T gx = fx ? (v[h] - v[h - 1]) : (v[h + 1] - v[h]);
+          const T gy = fy ? (v[h] - vyn[h - 1]) : (vyn[h - 1] - v[h]);
+          const T gz = (oz >= T(0)) ? (v[h] - vlo) : (vhi - v[h]);
+          const T flux = gx * cx + gy * cy + gz * cz;
+          out = (P[h] - dE * flux) / denom;
+        } else {
+          out = (q == 0 && restore[c]) ? P[h] : P[h] / denom;
+        }
+        po[c] = out;
+        acc[h - 1] += (double)out * mq;
+      }
+    }
+  }
+#pragma unroll
+  for (int h = 1; h <= H; ++h) {
+    const int i = i0 - 1 + h;
+    if (owner && i < nx) u0p[(size_t)gq * C + cell(i, j)] = acc[h - 1];
+  }
 }
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -254,8 +357,9 @@ template <> struct Mfma<float> {
 template <int NOPS, int NBT, class TB, class TC>
 __global__ __launch_bounds__(kBlock) void k_tn_mfma(
     const double* __restrict__ A, int lda, int m, const TB* __restrict__ B, int ldb, int n,
-    int C, int nx, int ny, double idx_, double idy_, const double* __restrict__ irho, int cpw,
-    int ngroups, int groups_n, double* __restrict__ partial) {
+    int C, int nx, int ny, int nz, double idx_, double idy_, double idz_,
+    const double* __restrict__ irho, int cpw, int ngroups, int groups_n,
+    double* __restrict__ partial) {
   using MF = Mfma<TC>;
   using acc_t = typename MF::acc_t;
   constexpr int NACC = NOPS * NBT;
@@ -278,39 +382,51 @@ __global__ __launch_bounds__(kBlock) void k_tn_mfma(
   }
   const int cbeg = (chunk * 4 + wave) * cpw;
   const int cend = min(C, cbeg + cpw);
-  const TC idx = (TC)idx_, idy = (TC)idy_, h = TC(0.5);
+  constexpr int ND = NOPS >= 4 ? NOPS / 2 : 1;      // stencil axes (NOPS = 4 or 6)
+  const TC h = TC(0.5);
+  const TC ih[3] = {(TC)idx_, (TC)idy_, (TC)idz_};
+  const int syz = ny * nz;
+  const int stride[3] = {syz, nz, 1};
   acc_t acc[NACC];
 #pragma unroll
   for (int o = 0; o < NACC; ++o) acc[o] = acc_t{};
   for (int c0 = cbeg; c0 < cend; c0 += 16) {
     const int cs = c0 + 4 * kg;
     TC av[4], bv[NACC][4];
-    int ii = cs / ny, jj = cs - (cs / ny) * ny;
+    // cell coordinates of the lane's first cell, advanced with carries
+    int ii = cs / syz, jj = (cs - (cs / syz) * syz) / nz, kk = cs - (cs / nz) * nz;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const int c = cs + s;
       const bool live = c < cend;
       av[s] = (va && live) ? (TC)Acol[c] : TC(0);
-      if (NOPS == 1) {
+      if constexpr (NOPS == 1) {
 #pragma unroll
         for (int t = 0; t < NBT; ++t) bv[t][s] = (vb[t] && live) ? (TC)Bcol[t][c] : TC(0);
       } else {
+        while (kk >= nz) { kk -= nz; ++jj; }
         while (jj >= ny) { jj -= ny; ++ii; }
-        const bool in = vb[0] && live && ii >= 1 && ii <= nx - 2 && jj >= 1 && jj <= ny - 2;
-        TC vc = TC(0), vw = TC(0), ve = TC(0), vs = TC(0), vn = TC(0);
+        bool in = vb[0] && live && ii >= 1 && ii <= nx - 2 && jj >= 1 && jj <= ny - 2;
+        if (ND == 3) in = in && kk >= 1 && kk <= nz - 2;
+        TC vc = TC(0), vm[ND], vp[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) { vm[d] = TC(0); vp[d] = TC(0); }
         if (in) {
           const TB* b = Bcol[0];
           vc = (TC)b[c] * (TC)irho[c];
-          vw = (TC)b[c - ny] * (TC)irho[c - ny];
-          ve = (TC)b[c + ny] * (TC)irho[c + ny];
-          vs = (TC)b[c - 1] * (TC)irho[c - 1];
-          vn = (TC)b[c + 1] * (TC)irho[c + 1];
+#pragma unroll
+          for (int d = 0; d < ND; ++d) {
+            vm[d] = (TC)b[c - stride[d]] * (TC)irho[c - stride[d]];
+            vp[d] = (TC)b[c + stride[d]] * (TC)irho[c + stride[d]];
+          }
         }
-        bv[OP_L1X % NACC][s] = (vc - h * vw - h * ve) * idx;
-        bv[OP_L1Y % NACC][s] = (vc - h * vs - h * vn) * idy;
-        bv[OP_L2X % NACC][s] = (ve - vw) * (h * idx);
-        bv[OP_L2Y % NACC][s] = (vn - vs) * (h * idy);
-        ++jj;
+        // L1 (dissipative) operators first, then L2 (central), axis by axis
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+          bv[d % NACC][s] = (vc - h * vm[d] - h * vp[d]) * ih[d];
+          bv[(ND + d) % NACC][s] = (vp[d] - vm[d]) * (h * ih[d]);
+        }
+        ++kk;
       }
     }
 #pragma unroll
@@ -493,7 +609,8 @@ void tn_mfma_launch(hipStream_t st, int blocks, const void* A, int lda, int m, c
                     double* partial) {
   hipLaunchKernelGGL((k_tn_mfma<NOPS, NBT, TB, TC>), dim3(blocks), dim3(kBlock), 0, st,
                      static_cast<const double*>(A), lda, m, static_cast<const TB*>(B), ldb, n, g.C,
-                     g.nx, g.ny, 1.0 / g.dx, 1.0 / g.dy, g.irho, cpw, ngroups, groups_n, partial);
+                     g.nx, g.ny, g.nz, 1.0 / g.dx, 1.0 / g.dy, 1.0 / g.dz, g.irho, cpw, ngroups,
+                     groups_n, partial);
 }
 
 template <int NOPS, int NBT>
@@ -533,12 +650,21 @@ void launch_dose(hipStream_t st, const Grid& g, double* dose, const double* X, i
 }
 
 namespace {
-struct SweepShape { int ntj, nsb, qper, G; };
+struct SweepShape { int ntj, njb, nsb, qper, G; };
 
+// 2D: ntj column tiles (j) x nsb strip blocks (4 strips of rows i);
+// 3D: ntj column tiles (k) x njb blocks of 4 y indices (j) x nsb strips (i).
 SweepShape sweep_shape(const Grid& g, int nq, const SweepPlan& p) {
   SweepShape s;
-  s.ntj = (g.ny + kSweepCols - 1) / kSweepCols;
-  s.nsb = (g.nx + (kBlock / 64) * p.rows - 1) / ((kBlock / 64) * p.rows);
+  if (g.ndim() == 3) {
+    s.ntj = (g.nz + kSweepCols - 1) / kSweepCols;
+    s.njb = (g.ny + (kBlock / 64) - 1) / (kBlock / 64);
+    s.nsb = (g.nx + p.rows - 1) / p.rows;
+  } else {
+    s.ntj = (g.ny + kSweepCols - 1) / kSweepCols;
+    s.njb = 1;
+    s.nsb = (g.nx + (kBlock / 64) * p.rows - 1) / ((kBlock / 64) * p.rows);
+  }
   s.qper = (nq + p.groups - 1) / p.groups;
   s.G = std::max(1, (nq + s.qper - 1) / s.qper);
   return s;
@@ -548,6 +674,13 @@ template <class T, int H>
 void sweep_launch(hipStream_t st, const Grid& g, const void* pin, void* pout, int nq,
                   const double* omega, double dE, double denom, const unsigned char* restore,
                   const double* m0, double* u0p, const SweepShape& s, bool nt) {
+  if (g.ndim() == 3) {
+    hipLaunchKernelGGL((k_sweep3<T, H>), dim3((unsigned)(s.ntj * s.njb * s.nsb * s.G)),
+                       dim3(kBlock), 0, st, static_cast<const T*>(pin), static_cast<T*>(pout),
+                       g.nx, g.ny, g.nz, nq, s.qper, omega, (T)g.dx, (T)g.dy, (T)g.dz, g.rho,
+                       (T)dE, (T)denom, restore, m0, u0p, s.ntj, s.njb, s.nsb);
+    return;
+  }
   auto kern = nt ? k_sweep<T, H, true> : k_sweep<T, H, false>;
   hipLaunchKernelGGL(kern, dim3((unsigned)(s.ntj * s.nsb * s.G)), dim3(kBlock), 0, st,
                      static_cast<const T*>(pin), static_cast<T*>(pout), g.nx, g.ny, nq, s.qper,
@@ -560,18 +693,18 @@ SweepPlan sweep_default(const Grid& g, int nq, DT tpsi) {
   SweepPlan p;
   p.rows = tpsi == DT::F64 ? 8 : 16;
   // ordinate groups: enough workgroups to fill the chip (about four per CU)
-  const int ntj = (g.ny + kSweepCols - 1) / kSweepCols;
-  const int base = ntj * ((g.nx + (kBlock / 64) * p.rows - 1) / ((kBlock / 64) * p.rows));
+  SweepShape one = sweep_shape(g, 1, p);
+  const int base = one.ntj * one.njb * one.nsb;
   p.groups = std::max(1, std::min({nq, kMaxSweepGroups, (1024 + base - 1) / base}));
   return p;
 }
 
 std::vector<SweepPlan> sweep_candidates(const Grid& g, int nq, DT tpsi) {
-  (void)g;
   (void)tpsi;
   std::vector<SweepPlan> out;
   for (int nt : {0, 1})
     for (int rows : {2, 4, 8, 16})
+      if (!(nt && g.ndim() == 3))   // the 3D sweep has no non-temporal variant
       for (int groups : {1, 2, 4, 8})
         if (groups <= std::max(1, nq)) out.push_back(SweepPlan{rows, groups, nt != 0});
   return out;
@@ -610,43 +743,49 @@ void launch_sweep(hipStream_t st, const Grid& g, const void* pin, void* pout, DT
   check_launch("sweep");
 }
 
+namespace {
+template <class TC, int NB, int ND>
+void stream_k_nd(hipStream_t st, const Grid& g, const double* X, int ldx, int r, const double* Z,
+                 int n, double dE, double* K, int ldk) {
+  const dim3 grid(blocks_for(g.C), (n + NB - 1) / NB);
+  const size_t lds = (size_t)r * (2 * ND + 2) * NB * sizeof(TC);
+  hipLaunchKernelGGL((k_stream_k<TC, NB, ND>), grid, dim3(kBlock), lds, st, X, ldx, r, g.nx, g.ny,
+                     g.nz, g.rho, Z, n, dE, K, ldk);
+}
+template <class TC, int NB>
+void stream_k_nb(hipStream_t st, const Grid& g, const double* X, int ldx, int r, const double* Z,
+                 int n, double dE, double* K, int ldk) {
+  if (g.ndim() == 3) stream_k_nd<TC, NB, 3>(st, g, X, ldx, r, Z, n, dE, K, ldk);
+  else stream_k_nd<TC, NB, 2>(st, g, X, ldx, r, Z, n, dE, K, ldk);
+}
+}  // namespace
+
 void launch_stream_k(hipStream_t st, const Grid& g, const double* X, int ldx, int r,
                      const double* Z, int n, double dE, double* K, int ldk, bool fp32) {
   if (n == 0) return;
   // column blocks of NB = 16 (8 when r is wide, to bound the LDS footprint)
   const bool narrow = r > 48;
-  const int NB = narrow ? 8 : 16;
-  const dim3 grid(blocks_for(g.C), (n + NB - 1) / NB);
-  const size_t lds = (size_t)r * 6 * NB * (fp32 ? sizeof(float) : sizeof(double));
   if (fp32) {
-    if (narrow)
-      hipLaunchKernelGGL((k_stream_k<float, 8>), grid, dim3(kBlock), lds, st, X, ldx, r, g.nx,
-                         g.ny, g.rho, Z, n, dE, K, ldk);
-    else
-      hipLaunchKernelGGL((k_stream_k<float, 16>), grid, dim3(kBlock), lds, st, X, ldx, r, g.nx,
-                         g.ny, g.rho, Z, n, dE, K, ldk);
+    if (narrow) stream_k_nb<float, 8>(st, g, X, ldx, r, Z, n, dE, K, ldk);
+    else stream_k_nb<float, 16>(st, g, X, ldx, r, Z, n, dE, K, ldk);
   } else {
-    if (narrow)
-      hipLaunchKernelGGL((k_stream_k<double, 8>), grid, dim3(kBlock), lds, st, X, ldx, r, g.nx,
-                         g.ny, g.rho, Z, n, dE, K, ldk);
-    else
-      hipLaunchKernelGGL((k_stream_k<double, 16>), grid, dim3(kBlock), lds, st, X, ldx, r, g.nx,
-                         g.ny, g.rho, Z, n, dE, K, ldk);
+    if (narrow) stream_k_nb<double, 8>(st, g, X, ldx, r, Z, n, dE, K, ldk);
+    else stream_k_nb<double, 16>(st, g, X, ldx, r, Z, n, dE, K, ldk);
   }
   check_launch("stream_K");
 }
 
-size_t tn_partial_size() { return (size_t)kMaxTnSlots * 4 * 256; }
+size_t tn_partial_size() { return (size_t)kMaxTnSlots * 6 * 256; }
 
 void launch_tn(hipStream_t st, const Grid& g, const void* A, DT ta, int lda, int m, const void* B,
                DT tb, int ldb, int n, int op, bool fp32, double* partial, double* out) {
   if (m == 0 || n == 0) return;
-  if (op != OP_NONE && op != OP_ALL4) throw std::runtime_error("tn: unsupported operator");
-  const int nops = (op == OP_ALL4) ? 4 : 1;
+  if (op != OP_NONE && op != OP_ALL) throw std::runtime_error("tn: unsupported operator");
+  const int nops = (op == OP_ALL) ? 2 * g.ndim() : 1;
   // B tiles per wavefront: the four stencil images of one tile, or up to
   // kTnBTiles plain tiles sharing one A tile
   const int tiles_m = (m + 15) / 16, tiles_n = (n + 15) / 16;
-  const int nbt = nops == 4 ? 1 : std::min(tiles_n, kTnBTiles);
+  const int nbt = nops > 1 ? 1 : std::min(tiles_n, kTnBTiles);
   const int groups_n = (tiles_n + nbt - 1) / nbt;
   const int ngroups = tiles_m * groups_n;
   // cell runs of at least 64 cells (a multiple of 16) per wavefront; at most
@@ -660,7 +799,9 @@ void launch_tn(hipStream_t st, const Grid& g, const void* A, DT ta, int lda, int
   if ((size_t)nchunks * per_chunk > tn_partial_size())
     throw std::runtime_error("tn: output too wide for the partial buffer");
   const int blocks = nchunks * ngroups;
-  if (nops == 4) {
+  if (nops == 6) {
+    tn_dispatch<6, 1>(st, blocks, A, ta, lda, m, B, tb, ldb, n, g, fp32, cpw, ngroups, groups_n, partial);
+  } else if (nops == 4) {
     tn_dispatch<4, 1>(st, blocks, A, ta, lda, m, B, tb, ldb, n, g, fp32, cpw, ngroups, groups_n, partial);
   } else {
     switch (nbt) {

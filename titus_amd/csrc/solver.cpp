@@ -157,7 +157,20 @@ struct titus_ctx {
   std::vector<double> stopping, sigma, bc_scale;
   std::vector<int> degree, cand;
   HMat Ax, Az, aAx, aAz, M;
-  titus::SpMat sAx[4];      // |Ax|, |Az|, Ax, Az in compressed rows
+  // angular operators paired with the spatial stencils, in compressed rows:
+  // |Ax|, |Az| (, |Ay|), Ax, Az (, Ay) -- dissipative first, then central
+  std::vector<titus::SpMat> sA;
+  int nd = 2;                 // spatial dimensions (3: k axis paired with Ay)
+  int nz = 1;
+  double dz = 1.0;
+  int nops() const { return 2 * nd; }
+  // term order of the streaming sums: central operators, then dissipative
+  std::vector<int> term_order() const {
+    std::vector<int> o;
+    for (int d = 0; d < nd; ++d) o.push_back(nd + d);
+    for (int d = 0; d < nd; ++d) o.push_back(d);
+    return o;
+  }
   // device buffers
   double *rho = nullptr, *irho = nullptr, *omega = nullptr, *bc_base = nullptr, *m0 = nullptr;
   void *psi = nullptr, *psi2 = nullptr;    // C x nq, FP64 (plain) or FP32 (mixed)
@@ -247,7 +260,7 @@ struct titus_ctx {
   // ---- device building blocks --------------------------------------------------
   // arena[off : off + m*n] = A^T op(B)
   size_t tn(const double* A, int m, const double* B, int n, int op) {
-    const int nops = op == dev::OP_ALL4 ? 4 : 1;
+    const int nops = op == dev::OP_ALL ? this->nops() : 1;
     const size_t off = aalloc((size_t)nops * m * n);
     const double by = 8.0 * C * (A == B ? m : m + n) + (op == dev::OP_NONE ? 0.0 : 8.0 * C);
     // Galerkin products run in FP32 only in the MIXED_FULL_FP32 form; Gram
@@ -259,13 +272,13 @@ struct titus_ctx {
     });
     return off;
   }
-  // [Xa^T L1x Xb, Xa^T L1y Xb, Xa^T L2x Xb, Xa^T L2y Xb] at the returned offset
+  // [Xa^T L_o Xb for the 2 nd stencil operators] at the returned offset
   size_t galerkin_x(const double* Xa, int ma, const double* Xb, int mb) {
-    return tn(Xa, ma, Xb, mb, dev::OP_ALL4);
+    return tn(Xa, ma, Xb, mb, dev::OP_ALL);
   }
   std::vector<HMat> galerkin_view(size_t off, int ma, int mb) const {
     std::vector<HMat> out;
-    for (int op = 0; op < 4; ++op) out.push_back(view(off + (size_t)op * ma * mb, ma, mb));
+    for (int op = 0; op < nops(); ++op) out.push_back(view(off + (size_t)op * ma * mb, ma, mb));
     return out;
   }
   void gemm_dev(const double* A, int k, const double* Td, int n, double alpha, double beta,
@@ -280,33 +293,36 @@ struct titus_ctx {
   }
 
   // ---- host building blocks (angular side) -------------------------------------
-  // [|Ax| Wm, |Az| Wm, Ax Wm, Az Wm]: computed once per basis and reused by
-  // the Galerkin products, the L-step (A (W S^T) = (A W) S^T) and the
+  // [A_o Wm for every angular operator]: computed once per basis and reused
+  // by the Galerkin products, the L-step (A (W S^T) = (A W) S^T) and the
   // augmented basis (A [W Wt] = [A W, A Wt])
-  using AW4 = std::array<HMat, 4>;
-  AW4 apply_a(const HMat& Wm) const {
+  using AWs = std::vector<HMat>;
+  AWs apply_a(const HMat& Wm) const {
     HostScope hs(timing, "host:galerkin_w");
-    return {titus::spmm(sAx[0], Wm), titus::spmm(sAx[1], Wm), titus::spmm(sAx[2], Wm),
-            titus::spmm(sAx[3], Wm)};
+    AWs out;
+    for (const titus::SpMat& A : sA) out.push_back(titus::spmm(A, Wm));
+    return out;
   }
-  std::vector<HMat> galerkin_from(const HMat& Wa, const AW4& awb) const {
+  std::vector<HMat> galerkin_from(const HMat& Wa, const AWs& awb) const {
     HostScope hs(timing, "host:galerkin_w");
-    return {titus::matmul(Wa, awb[0], true), titus::matmul(Wa, awb[1], true),
-            titus::matmul(Wa, awb[2], true), titus::matmul(Wa, awb[3], true)};
+    std::vector<HMat> out;
+    for (const HMat& aw : awb) out.push_back(titus::matmul(Wa, aw, true));
+    return out;
   }
   std::vector<HMat> galerkin_w(const HMat& Wa, const HMat& Wb) const {
     return galerkin_from(Wa, apply_a(Wb));
   }
-  static AW4 hcat4(const AW4& a, const AW4& b) {
-    return {titus::hcat(a[0], b[0]), titus::hcat(a[1], b[1]), titus::hcat(a[2], b[2]),
-            titus::hcat(a[3], b[3])};
+  static AWs hcat_all(const AWs& a, const AWs& b) {
+    AWs out;
+    for (size_t o = 0; o < a.size(); ++o) out.push_back(titus::hcat(a[o], b[o]));
+    return out;
   }
-  // L - dE (Ax L XL2xX^T + Az L XL2yX^T + |Ax| L XL1xX^T + |Az| L XL1yX^T) for
+  // L - dE sum_o A_o L (X^T L_o X)^T (2D: Ax, Az, |Ax|, |Az| terms) for
   // L = W S^T, with A L = (A W) S^T from aw = apply_a(W)
-  HMat stream_L(const AW4& aw, const std::vector<HMat>& XLX) const {
+  HMat stream_L(const AWs& aw, const std::vector<HMat>& XLX) const {
     HostScope hs(timing, "host:stream_LS");
     HMat out = titus::matmul(W, S, false, true);
-    for (int k : {2, 3, 0, 1}) {
+    for (int k : term_order()) {
       HMat t = titus::matmul(titus::matmul(aw[k], S, false, true), XLX[k], false, true);
       for (size_t i = 0; i < out.a.size(); ++i) out.a[i] -= dE * t.a[i];
     }
@@ -315,7 +331,7 @@ struct titus_ctx {
   HMat stream_S(const HMat& S0, const std::vector<HMat>& XLX, const std::vector<HMat>& WAW) const {
     HostScope hs(timing, "host:stream_LS");
     HMat out = S0;
-    for (int k : {2, 3, 0, 1}) {
+    for (int k : term_order()) {
       if (XLX[k].m != S0.m || XLX[k].n != S0.m || WAW[k].m != S0.n || WAW[k].n != S0.n)
         throw TitusError(TITUS_E_ARG, "S-step: Galerkin blocks do not match the coefficients");
       HMat t = titus::matmul(titus::matmul(XLX[k], S0), WAW[k]);
@@ -323,22 +339,28 @@ struct titus_ctx {
     }
     return out;
   }
-  // Z = [S, S(M1x/dx + M1y/dy), S(M2x-M1x)/(2dx), S(-M2x-M1x)/(2dx),
-  //      S(M2y-M1y)/(2dy), S(-M2y-M1y)/(2dy)] with Mk = W^T A_k W
+  // Z = [S, S C_c, S C_x+, S C_x-, S C_y+, S C_y- (, S C_z+, S C_z-)] with
+  // C_c = sum_d M1_d / h_d, C_d+ = (M2_d - M1_d) / (2 h_d) and
+  // C_d- = (-M2_d - M1_d) / (2 h_d); M1_d = W^T |A_d| W, M2_d = W^T A_d W
+  // (2D: C_c = M1x/dx + M1y/dy, then the east, west, north, south blocks)
   HMat stream_Z(const HMat& S0, const std::vector<HMat>& WAW) const {
     const int r = S0.m, n = S0.n;
-    const HMat &M1x = WAW[0], &M1y = WAW[1], &M2x = WAW[2], &M2y = WAW[3];
-    HMat comb[5] = {HMat(n, n), HMat(n, n), HMat(n, n), HMat(n, n), HMat(n, n)};
+    const double h[3] = {dx, dy, dz};
+    const int nb = 1 + 2 * nd;
+    std::vector<HMat> comb(nb, HMat(n, n));
     for (size_t i = 0; i < (size_t)n * n; ++i) {
-      comb[0].a[i] = M1x.a[i] / dx + M1y.a[i] / dy;
-      comb[1].a[i] = (M2x.a[i] - M1x.a[i]) / (2.0 * dx);
-      comb[2].a[i] = (-M2x.a[i] - M1x.a[i]) / (2.0 * dx);
-      comb[3].a[i] = (M2y.a[i] - M1y.a[i]) / (2.0 * dy);
-      comb[4].a[i] = (-M2y.a[i] - M1y.a[i]) / (2.0 * dy);
+      double c = WAW[0].a[i] / h[0];
+      for (int d = 1; d < nd; ++d) c += WAW[d].a[i] / h[d];
+      comb[0].a[i] = c;
+      for (int d = 0; d < nd; ++d) {
+        const double m1 = WAW[d].a[i], m2 = WAW[nd + d].a[i];
+        comb[1 + 2 * d].a[i] = (m2 - m1) / (2.0 * h[d]);
+        comb[2 + 2 * d].a[i] = (-m2 - m1) / (2.0 * h[d]);
+      }
     }
-    HMat Z(r, 6 * n);
+    HMat Z(r, (nb + 1) * n);
     std::copy(S0.a.begin(), S0.a.end(), Z.a.begin());
-    for (int d = 0; d < 5; ++d) {
+    for (int d = 0; d < nb; ++d) {
       HMat t = titus::matmul(S0, comb[d]);
       std::copy(t.a.begin(), t.a.end(), Z.a.begin() + (size_t)(d + 1) * r * n);
     }
@@ -351,7 +373,7 @@ struct titus_ctx {
   }
   void stream_k(int r, const std::vector<HMat>& WAW) {
     const double* Zd = upload(stream_Z(S, WAW));
-    timed("stream_K", 8.0 * C * (2 * r + 1), 12.0 * C * r * r,
+    timed("stream_K", 8.0 * C * (2 * r + 1), 2.0 * (2 * nd + 2) * C * r * r,
           [&] { dev::launch_stream_k(st, g, X, C, r, Zd, r, dE, K, C, full32); });
   }
 
@@ -557,7 +579,7 @@ struct titus_ctx {
       // K-step and old Galerkin products (SolverCSD.jl:1019-1038)
       areset();
       const size_t gx = galerkin_x(X, r, X, r);
-      const AW4 aw = apply_a(W);
+      const AWs aw = apply_a(W);
       stream_k(r, galerkin_from(W, aw));
       RangeJob j = range_launch(K, r, nullptr, 0);
       fetch();
@@ -648,7 +670,7 @@ struct titus_ctx {
       const int r = rank;
       areset();
       const size_t gx = galerkin_x(X, r, X, r);
-      const AW4 aw = apply_a(W);
+      const AWs aw = apply_a(W);
       stream_k(r, galerkin_from(W, aw));
       copy_x_to_aug(r);
       RangeJob j = range_launch(K, r, X, r);
@@ -666,11 +688,11 @@ struct titus_ctx {
       HMat S1 = titus::matmul(titus::matmul(Mu, S), Nu, false, true);   // 2r x 2r
       // A W1 = [A W, A Wt]
       std::vector<HMat> WAW =
-          galerkin_from(W1, hcat4(aw, apply_a(titus::cols(W1, r, W1.n))));
+          galerkin_from(W1, hcat_all(aw, apply_a(titus::cols(W1, r, W1.n))));
       HMat Sup = stream_S(S1, XLX, WAW);
       // eta diagnostic (:2210-2220): rows r.. of X1^T L X1, columns r.. of W1^T A W1
       HMat acc(r, r);
-      for (int k : {2, 3, 0, 1}) {
+      for (int k : term_order()) {
         HMat xl(r, 2 * r), wa(2 * r, r);
         for (int c = 0; c < 2 * r; ++c)
           for (int i = 0; i < r; ++i) xl(i, c) = XLX[k](r + i, c);
@@ -741,7 +763,7 @@ static void tune_sweep(titus_ctx& x) {
   x.sweep_plan = dev::sweep_default(x.g, x.nq, x.tpsi);
   x.sweep_tuned = false;
   if ((size_t)x.C * x.nq < kTuneMinWork) return;
-  const auto key = std::make_tuple(x.nx, x.ny, x.nq, x.mixed ? 1 : 0);
+  const auto key = std::make_tuple(x.nx, x.ny * x.nz + (x.nz > 1 ? 1 << 24 : 0), x.nq, x.mixed ? 1 : 0);
   std::lock_guard<std::mutex> lock(g_tune_mu);
   auto it = g_tuned.find(key);
   if (it != g_tuned.end()) {
@@ -816,8 +838,10 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
   return guarded([&] {
     require(out != nullptr && p != nullptr, "null argument");
     *out = nullptr;
-    require(p->version == TITUS_ABI_VERSION, "problem struct version mismatch");
-    require(p->nx >= 3 && p->ny >= 3, "need at least 3 cells per direction");
+    require(p->version == 1 || p->version == TITUS_ABI_VERSION, "problem struct version mismatch");
+    const int nz = p->version >= 2 ? p->nz : 1;
+    require(p->nx >= 3 && p->ny >= 3 && (nz == 1 || nz >= 3), "need at least 3 cells per direction");
+    if (nz > 1) require(p->dz > 0 && p->Ay && p->absAy, "3D grid needs dz, Ay and |Ay|");
     require(p->dx > 0 && p->dy > 0, "cell widths must be positive");
     require(p->nm >= 1 && p->nq >= 1 && p->ne >= 3 && p->nsigma >= 1, "empty dimension");
     require(p->density && p->degree && p->Ax && p->Az && p->absAx && p->absAz && p->omega &&
@@ -829,7 +853,8 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
                 p->precision == TITUS_PRECISION_MIXED_FULL_FP32,
             "unknown precision form");
     require(p->r >= 1, "rank must be >= 1");
-    const int C = p->nx * p->ny;
+    require((long long)p->nx * p->ny * nz < (1LL << 31), "grid too large");
+    const int C = p->nx * p->ny * nz;
     for (int c = 0; c < C; ++c)
       require(std::isfinite(p->density[c]) && p->density[c] > 0.0, "density must be positive and finite");
     for (int i = 0; i < p->nm; ++i)
@@ -873,10 +898,18 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     x.aAx = copy_mat(p->absAx, p->nm, p->nm);
     x.aAz = copy_mat(p->absAz, p->nm, p->nm);
     x.M = copy_mat(p->M, p->nm, p->nq);
-    x.sAx[0] = titus::SpMat::from_dense(x.aAx);
-    x.sAx[1] = titus::SpMat::from_dense(x.aAz);
-    x.sAx[2] = titus::SpMat::from_dense(x.Ax);
-    x.sAx[3] = titus::SpMat::from_dense(x.Az);
+    x.nz = nz;
+    x.nd = nz > 1 ? 3 : 2;
+    x.dz = nz > 1 ? p->dz : 1.0;
+    if (x.nd == 3) {
+      const HMat Ay = copy_mat(p->Ay, p->nm, p->nm), aAy = copy_mat(p->absAy, p->nm, p->nm);
+      x.sA = {titus::SpMat::from_dense(x.aAx), titus::SpMat::from_dense(x.aAz),
+              titus::SpMat::from_dense(aAy), titus::SpMat::from_dense(x.Ax),
+              titus::SpMat::from_dense(x.Az), titus::SpMat::from_dense(Ay)};
+    } else {
+      x.sA = {titus::SpMat::from_dense(x.aAx), titus::SpMat::from_dense(x.aAz),
+              titus::SpMat::from_dense(x.Ax), titus::SpMat::from_dense(x.Az)};
+    }
     x.nb = p->nb;
 
     const size_t cx = (size_t)C * x.capx;
@@ -910,7 +943,7 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     x.cand_d = dalloc<int>(p->ncand);
     x.bmask = dalloc<unsigned char>(C);
     x.restore = dalloc<unsigned char>(C);
-    x.g = dev::Grid{p->nx, p->ny, C, p->dx, p->dy, x.rho, x.irho};
+    x.g = dev::Grid{p->nx, p->ny, C, p->dx, p->dy, x.rho, x.irho, x.nz, x.dz};
 
     HIPCHK(hipMemcpy(x.rho, p->density, sizeof(double) * C, hipMemcpyHostToDevice));
     {

@@ -23,6 +23,7 @@ Cells.  Cell (i, j) (0-based, i along x) has linear index c = i*ny + j, the
 reference's ``vectorIndex`` (``utils.jl:98-100``) shifted to 0-based.
 """
 import json
+import functools
 import math
 import os
 
@@ -96,10 +97,10 @@ class CSDGrid:
         self.settings = s
 
     def sigma_at_energy(self, energy):
-        """sigma_l(E), l = 0..nPN (``CSD.jl:97-127``, electrons branch)."""
+        """sigma_l(E), l = 0..nPN (``CSD.jl:97-127``)."""
         s = self.settings
         if s.particle == "Protons":
-            raise NotImplementedError("proton scattering moments are not provided by this build")
+            return self._sigma_protons(energy)
         if energy <= 5e-5:
             energy = 5e-5 + 1e-9
         if s.nPN + 1 > self.sigma_tab.shape[1]:
@@ -107,9 +108,81 @@ class CSDGrid:
         return np.array([float(interp_throw(self.E_sigmaTab, self.sigma_tab[:, i], energy))
                          for i in range(s.nPN + 1)])
 
+    def _sigma_protons(self, energy):
+        """Proton branch (``CSD.jl:98-115``): O-16 elastic (water's oxygen mass
+        fraction 0.888106, dropped below 7 MeV kinetic), proton-proton nuclear
+        (hydrogen fraction 0.111894) and single Coulomb scattering, each
+        interpolated linearly in total energy."""
+        s = self.settings
+        if energy <= 0.001 + s.eRest:
+            energy = 0.001 + s.eRest
+        if s.nPN + 1 > self.sigma_tab.shape[1]:
+            raise ValueError("tables hold only 40 Legendre moments")
+        pp, ce = proton_moment_tables()
+        E_tab = self.E_tab
+        y = np.zeros(s.nPN + 1)
+        for i in range(s.nPN + 1):
+            v_pp = float(interp_throw(E_tab, pp[:, i], energy))
+            v_ce = float(interp_throw(E_tab, ce[:, i], energy))
+            if energy < 7.0 + s.eRest:
+                y[i] = 0.88810600 * 0 + 0.11189400 * v_pp + v_ce
+            else:
+                v_o = float(interp_throw(self.E_sigmaTab, self.sigma_tab[:, i], energy))
+                y[i] = 0.88810600 * v_o + 0.11189400 * v_pp + v_ce
+        return y
+
     def sigma_table(self):
         """sigma_l(eGrid[n]) for every grid point: nEnergies x (nPN+1)."""
         return np.stack([self.sigma_at_energy(e) for e in self.eGrid])
+
+
+# ---------------------------------------------------------------------------
+# proton cross-section moments (MaterialParametersProtons.jl)
+# ---------------------------------------------------------------------------
+# scattering angles (degrees) of the tabulated elastic cross sections
+# (MaterialParametersProtons.jl:26)
+OMEGA_SIGMA_TAB = np.arange(5.0, 181.0, 5.0)
+
+
+def legendre_table(lmax, x):
+    """P_l(x) for l = 0..lmax (rows) at the points x (columns), by recurrence."""
+    x = np.asarray(x, dtype=np.float64)
+    P = np.zeros((lmax + 1, x.size))
+    P[0] = 1.0
+    if lmax >= 1:
+        P[1] = x
+    for l in range(1, lmax):
+        P[l + 1] = ((2 * l + 1) * x * P[l] - l * P[l - 1]) / (l + 1)
+    return P
+
+
+def integrate_xs_poly(N, mu, sigma):
+    """Legendre moments of tabulated cross sections (``integrateXS_Poly``,
+    ``MaterialParametersProtons.jl:2812-2831``): xi[n, i] = -sum_k (mu_{k+1} -
+    mu_k)/2 (sigma[n, k] P_i(mu_k) + sigma[n, k+1] P_i(mu_{k+1})), i < N."""
+    P = legendre_table(N - 1, mu)                      # N x nMu
+    dmu = np.diff(mu)                                   # nMu - 1
+    t = sigma[:, :-1, None] * P.T[None, :-1, :] + sigma[:, 1:, None] * P.T[None, 1:, :]
+    return -0.5 * np.einsum("k,nki->ni", dmu, t)
+
+
+@functools.lru_cache(maxsize=1)
+def proton_moment_tables():
+    """(sigma_pp, sigma_ce): 40 Legendre moments on the PSTAR energies of the
+    proton-proton nuclear elastic cross section (``Proton_Proton_nuclear``,
+    ``MaterialParametersProtons.jl:2834-2849``: 0.315 E^-1.126 + 3.78e-6 E for
+    E >= 10 MeV kinetic, isotropic) and of single Coulomb scattering on water
+    (``Rutherford``, ``:2888-2902``, with the reference's unit conventions)."""
+    t = load_tables("Protons")
+    E = np.asarray(t["E_tab_PSTAR"], dtype=np.float64)      # kinetic MeV
+    mu = np.cos(np.radians(OMEGA_SIGMA_TAB))
+    s_pp = np.where(E >= 10, 0.315 * E ** (-1.126) + 3.78e-6 * E, 0.0)
+    sig_pp = np.repeat(s_pp[:, None], OMEGA_SIGMA_TAB.size, axis=1)
+    z_eff, z_p = 7.42, 1.0
+    e, eps0 = 1.602176634e-19, 8.8541878128e-12
+    sig_ce = ((4 * math.pi * eps0) ** -2 * (z_eff * z_p * e ** 2 / (4 * E[:, None])) ** 2
+              / np.sin(np.radians(OMEGA_SIGMA_TAB[None, :] / 2)) ** 4)
+    return integrate_xs_poly(40, mu, sig_pp), integrate_xs_poly(40, mu, sig_ce)
 
 
 # ---------------------------------------------------------------------------
