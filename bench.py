@@ -207,6 +207,64 @@ def stencil_scale(sync, steps=4, n=1024, calibrate=False):
     return out
 
 
+def proton_cube_bench(sync, cpu_steps=2):
+    """3D configuration: 90 MeV protons into a 64^3 grid of 0.1 cm voxels
+    (seeded smooth density in [0.3, 1.6] g/cc), P_13 with all 196 moments,
+    fixed rank 30, 300 energy steps, FP64 and mixed forms.  Whole solves are
+    timed (contexts created before the timed region); kernels are timed with
+    events on every 10th step of a second, identical FP64 solve.  The FP64
+    oracle's per-step time on a few steps of the same problem is the CPU
+    figure."""
+    from titus_amd import _lib
+    from titus_amd.problem import INTEGRATOR_FIXED
+    from titus_amd.problem3d import build_problem3d, proton_cube
+    from titus_amd.solver import MIXED_RANK_TOL
+    P = build_problem3d(proton_cube())
+    out = {"workload": f"3D proton beam 90 MeV, 64^3 x 0.1 cm, P_13 ({P['Nm']} moments, "
+                       f"{P['Q']} ordinates), fixed rank 30, {P['nE'] - 1} steps, seed 0"}
+    doses = {}
+    for name, prec in (("fp64", _lib.PRECISION_FP64), ("mixed", _lib.PRECISION_MIXED)):
+        ctx = _lib.Context(P, INTEGRATOR_FIXED, precision=prec, rank_tol=MIXED_RANK_TOL)
+        try:
+            sync()
+            t0 = time.perf_counter()
+            ctx.solve(0)
+            sync()
+            dt = time.perf_counter() - t0
+            done, _ = ctx.progress()
+            doses[name] = ctx.state()["dose"]
+            out[name] = {"energy_steps_per_s": done / dt, "ms_per_step": dt / done * 1e3,
+                         "sweep_plan": ctx.tuning("sweep")}
+        finally:
+            ctx.close()
+    out["dose_rel_l2_mixed_vs_fp64"] = float(np.linalg.norm(doses["mixed"] - doses["fp64"])
+                                             / np.linalg.norm(doses["fp64"]))
+    ctx = _lib.Context(P, INTEGRATOR_FIXED, rank_tol=MIXED_RANK_TOL)
+    try:
+        ctx.set_timing(10)
+        ctx.solve(0)
+        sync()
+        sampled = (P["nE"] - 1) // 10
+        k = {}
+        for name in ctx.KERNELS:
+            ms, n, by, fl = ctx.kernel_stats(name)
+            if n:
+                k[name] = {"ms_per_step": round(ms / sampled, 4),
+                           "GB/s": round(by / (ms * 1e-3) / 1e9, 1),
+                           "TFLOP/s": round(fl / (ms * 1e-3) / 1e12, 2)}
+        out["fp64_kernels"] = k
+        out["fp64_host_ms_per_step"] = {n.split(":")[1]: round(ctx.kernel_stats(n)[0] / sampled, 4)
+                                        for n in ctx.HOST_PHASES}
+    finally:
+        ctx.close()
+    if cpu_steps:
+        from oracle import dlra
+        t0 = time.perf_counter()
+        dlra.solve_fixed(P, steps=cpu_steps)
+        out["cpu_oracle_s_per_step"] = (time.perf_counter() - t0) / cpu_steps
+    return out
+
+
 def per_step(run, ms):
     """Milliseconds per sampled step (0 when nothing was sampled)."""
     return ms / run["sampled"] if run["sampled"] else 0.0
@@ -251,6 +309,10 @@ def main():
                     help="host workers of the beamlet batch")
     ap.add_argument("--no-stencil", action="store_true",
                     help="skip the large-grid stencil measurement")
+    ap.add_argument("--no-cube", action="store_true",
+                    help="skip the 3D proton-cube measurement")
+    ap.add_argument("--cube-only", action="store_true",
+                    help="only the 3D proton-cube measurement")
     ap.add_argument("--stencil-only", action="store_true",
                     help="only the large-grid stencil measurement (+ calibration copy), for PMC runs")
     args = ap.parse_args()
@@ -266,6 +328,9 @@ def main():
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
     except Exception:
         sync = lambda: None
+    if args.cube_only:
+        print(json.dumps({"proton_cube": proton_cube_bench(sync, 0 if args.no_cpu else 2)}), flush=True)
+        return
     if args.stencil_only:
         print(json.dumps({"stencil_scale": stencil_scale(sync, calibrate=True)}), flush=True)
         return
@@ -330,6 +395,8 @@ def main():
         out["beamlets"] = beamlet_batch(args.beamlets, args.threads, sync)
     if not args.no_stencil:
         out["stencil_scale"] = stencil_scale(sync)
+    if not args.no_cube:
+        out["proton_cube"] = proton_cube_bench(sync, 0 if args.no_cpu else 2)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(P, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 

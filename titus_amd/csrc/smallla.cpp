@@ -56,33 +56,21 @@ HMat matmul(const HMat& A, const HMat& B, bool ta, bool tb) {
   const int m = ta ? A.n : A.m, k = ta ? A.m : A.n;
   const int k2 = tb ? B.n : B.m, n = tb ? B.m : B.n;
   if (k != k2) throw std::runtime_error("matmul: inner dimensions differ");
+  // transposed operands are materialised once, so every product runs as
+  // column updates c += a_l * b_lj, vectorised over the rows; each entry is
+  // still summed over l in ascending order
+  HMat At, Bt;
+  const HMat& Ao = ta ? (At = transpose(A)) : A;
+  const HMat& Bo = tb ? (Bt = transpose(B)) : B;
   HMat C(m, n);
-  if (!ta && !tb) {
-    for (int j = 0; j < n; ++j) {
-      double* c = C.col(j);
-      for (int l = 0; l < k; ++l) {
-        const double b = B(l, j);
-        if (b == 0.0) continue;
-        const double* a = A.col(l);
-        for (int i = 0; i < m; ++i) c[i] += a[i] * b;
-      }
+  for (int j = 0; j < n; ++j) {
+    double* c = C.col(j);
+    for (int l = 0; l < k; ++l) {
+      const double b = Bo(l, j);
+      if (b == 0.0) continue;
+      const double* a = Ao.col(l);
+      for (int i = 0; i < m; ++i) c[i] += a[i] * b;
     }
-  } else if (ta && !tb) {
-    for (int j = 0; j < n; ++j)
-      for (int i = 0; i < m; ++i) {
-        const double* a = A.col(i);
-        const double* b = B.col(j);
-        double s = 0.0;
-        for (int l = 0; l < k; ++l) s += a[l] * b[l];
-        C(i, j) = s;
-      }
-  } else {
-    for (int j = 0; j < n; ++j)
-      for (int i = 0; i < m; ++i) {
-        double s = 0.0;
-        for (int l = 0; l < k; ++l) s += (ta ? A(l, i) : A(i, l)) * (tb ? B(j, l) : B(l, j));
-        C(i, j) = s;
-      }
   }
   return C;
 }
