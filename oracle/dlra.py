@@ -130,7 +130,8 @@ def upwind_flux(psi, rho, omega, nx, ny, dx, dy):
 # ---------------------------------------------------------------------------
 # bases
 # ---------------------------------------------------------------------------
-def range_basis(A, ncols, candidates, against=None, scale=None, tol=None, greedy=False):
+def range_basis(A, ncols, candidates, against=None, scale=None, tol=None, greedy=False,
+                info=None):
     """Orthonormal basis of range(A) (minus span(against)), completed canonically.
 
     Singular values <= tol * scale count as zero (scale defaults to sigma_1).
@@ -139,7 +140,9 @@ def range_basis(A, ncols, candidates, against=None, scale=None, tol=None, greedy
     candidates in order, skipping one whose residual norm^2 is <= 1/2.
     ``greedy=True`` (small bases): take the candidate of largest residual,
     lowest index among those within ``COMPLETION_TIE`` of it.  Returns an
-    m x ncols orthonormal matrix.
+    m x ncols orthonormal matrix.  A dict passed as ``info`` receives the
+    kept rank ``p``, the SVD (``U``, ``sv``, ``Vt``) of the projected A and the
+    chosen candidates in column order (``chosen``).
     """
     if tol is None:
         tol = RANK_TOL
@@ -147,7 +150,7 @@ def range_basis(A, ncols, candidates, against=None, scale=None, tol=None, greedy
     if against is not None:
         for _ in range(2):
             A = A - against @ (against.T @ A)
-    U, sv, _ = np.linalg.svd(A, full_matrices=False)
+    U, sv, Vt = np.linalg.svd(A, full_matrices=False)
     if scale is None:
         scale = sv[0] if sv.size else 0.0
     p = int(np.count_nonzero(sv > tol * scale)) if scale > 0 else 0
@@ -160,6 +163,7 @@ def range_basis(A, ncols, candidates, against=None, scale=None, tol=None, greedy
     cols = [B]
     need = ncols - p
     cand = list(candidates)
+    chosen = []
     if not greedy:
         for c in cand:
             if need == 0:
@@ -171,6 +175,7 @@ def range_basis(A, ncols, candidates, against=None, scale=None, tol=None, greedy
             nv = np.linalg.norm(v)
             if nv * nv > 0.5:
                 v = (v / nv)[:, None]
+                chosen.append(c)
                 cols.append(v)
                 base = np.hstack([base, v])
                 need -= 1
@@ -189,11 +194,14 @@ def range_basis(A, ncols, candidates, against=None, scale=None, tol=None, greedy
             # ties, won by the lowest index
             t = int(np.nonzero(nr >= top * (1.0 - COMPLETION_TIE))[0][0])
             v = (R[:, t] / np.sqrt(nr[t]))[:, None]
+            chosen.append(cand[t])
             cols.append(v)
             base = np.hstack([base, v])
             need -= 1
     if need:
         raise RuntimeError("canonical completion ran out of candidates")
+    if info is not None:
+        info.update(p=p, U=U, sv=sv, Vt=Vt, chosen=chosen)
     return np.hstack(cols)
 
 
