@@ -163,6 +163,30 @@ int titus_kernel_stats(const struct titus_ctx* ctx, const char* name, double* to
  * Writes at most n values. */
 int titus_get_tuning(const struct titus_ctx* ctx, const char* kernel, int32_t* values, int32_t n);
 
+/* ---- adjoint (no reference counterpart: the reference has no gradients) ----
+ * The differentiated map is this library's fixed-rank solve
+ * (SolveFirstCollisionSourceDLR, code/SolverCSD.jl:891-1120): the dose after
+ * the steps taken, as a function of the density (problem->density), with the
+ * energy grid, beam source and P_N operators held fixed.  FP64, fixed-rank
+ * integrator and 2D grids only (TITUS_E_ARG otherwise).
+ *
+ * titus_set_checkpoints: before the first step, keep the state (X, psi on the
+ * device; S, W on the host) before steps 1, 1 + every, 1 + 2 every, ... of the
+ * forward solve.  TITUS_E_STATE after a step has been taken. */
+int titus_set_checkpoints(struct titus_ctx* ctx, int32_t every);
+
+/* Gradient of <dose_bar, dose> with respect to the density (C values each):
+ * d/d rho_c.  Recomputes each checkpointed segment once with its intermediates
+ * kept, then runs it backwards; the forward state (dose, factors, psi) is
+ * unchanged afterwards.  TITUS_E_STATE without checkpoints or steps.  The
+ * rrule of a dose objective J(dose) calls it with dose_bar = dJ/d dose. */
+int titus_adjoint(struct titus_ctx* ctx, const double* dose_bar, double* grad_density);
+
+/* titus_adjoint over n contexts on up to nthreads host threads (as
+ * titus_solve_batch); dose_bars[i] and grads[i] belong to ctxs[i]. */
+int titus_adjoint_batch(struct titus_ctx** ctxs, int32_t n, int32_t nthreads,
+                        const double* const* dose_bars, double* const* grads, int32_t* status);
+
 void titus_destroy(struct titus_ctx* ctx);
 const char* titus_last_error(void);
 int32_t titus_abi_version(void);

@@ -27,6 +27,7 @@ ERRORS = {
 EXPORTED = ("titus_create", "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
             "titus_get_state",
             "titus_get_history", "titus_set_timing", "titus_kernel_stats", "titus_get_tuning",
+            "titus_set_checkpoints", "titus_adjoint", "titus_adjoint_batch",
             "titus_destroy", "titus_last_error", "titus_abi_version")
 
 
@@ -80,13 +81,18 @@ def load():
     lib.titus_kernel_stats.argtypes = [vp, ctypes.c_char_p, _P, ctypes.POINTER(ctypes.c_int64),
                                        _P, _P]
     lib.titus_get_tuning.argtypes = [vp, ctypes.c_char_p, _PI, ctypes.c_int32]
+    lib.titus_set_checkpoints.argtypes = [vp, ctypes.c_int32]
+    lib.titus_adjoint.argtypes = [vp, _P, _P]
+    lib.titus_adjoint_batch.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.POINTER(_P), ctypes.POINTER(_P), _PI]
     lib.titus_destroy.argtypes = [vp]
     lib.titus_destroy.restype = None
     lib.titus_last_error.restype = ctypes.c_char_p
     lib.titus_abi_version.restype = ctypes.c_int32
     for name in ("titus_create", "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
                  "titus_get_state", "titus_get_history", "titus_set_timing",
-                 "titus_kernel_stats", "titus_get_tuning"):
+                 "titus_kernel_stats", "titus_get_tuning", "titus_set_checkpoints",
+                 "titus_adjoint", "titus_adjoint_batch"):
         getattr(lib, name).restype = ctypes.c_int
     if lib.titus_abi_version() != ABI_VERSION:
         raise RuntimeError("libtitus_hip.so ABI version mismatch")
@@ -165,6 +171,21 @@ def solve_batch(contexts, nthreads=0):
     return status
 
 
+def adjoint_batch(contexts, dose_bars, nthreads=0):
+    """titus_adjoint over many contexts concurrently; dose_bars is C x n
+    (column i seeds context i).  Returns the density gradients, C x n."""
+    lib = load()
+    n = len(contexts)
+    bars = [np.ascontiguousarray(dose_bars[:, i], dtype=np.float64) for i in range(n)]
+    grads = [np.zeros(c.C) for c in contexts]
+    arr = (ctypes.c_void_p * n)(*[c.h for c in contexts])
+    pb = (_P * n)(*[_dp(b) for b in bars])
+    pg = (_P * n)(*[_dp(g) for g in grads])
+    status = np.zeros(n, dtype=np.int32)
+    check(lib.titus_adjoint_batch(arr, n, nthreads, pb, pg, _ip(status)))
+    return np.stack(grads, axis=1) if n else np.zeros((0, 0))
+
+
 class Context:
     """Owner of one ``titus_ctx`` (a device-resident solver state)."""
 
@@ -229,6 +250,19 @@ class Context:
         check(self._lib.titus_get_tuning(self.h, kernel.encode(), _ip(v), 4))
         return {"rows": int(v[0]), "groups": int(v[1]), "tuned": bool(v[2]),
                 "nontemporal": bool(v[3])}
+
+    def set_checkpoints(self, every):
+        """Keep the state every `every` steps of the coming solve (for adjoint)."""
+        check(self._lib.titus_set_checkpoints(self.h, int(every)))
+
+    def adjoint(self, dose_bar):
+        """d <dose_bar, dose> / d density (C values) for the steps solved."""
+        bar = np.ascontiguousarray(dose_bar, dtype=np.float64)
+        if bar.shape != (self.C,):
+            raise ValueError("dose_bar must have one value per cell")
+        grad = np.zeros(self.C)
+        check(self._lib.titus_adjoint(self.h, _dp(bar), _dp(grad)))
+        return grad
 
     def close(self):
         if getattr(self, "h", None):

@@ -18,13 +18,15 @@ from .solver import MIXED_RANK_TOL, RANK_TOL
 
 
 def beamlet_plan(n_beamlets=64, n=128, N=15, seed=0, rank_range=(5, 40), e_range=(6.0, 20.0),
-                 max_angle_deg=15.0, length=10.0, sigma_x=0.25, angular_inv_width=1000.0):
+                 max_angle_deg=15.0, length=10.0, sigma_x=0.25, angular_inv_width=1000.0,
+                 integrator=INTEGRATOR_BUG):
     """Seeded beamlet plan on an n x n ellipse phantom (as the 200x200 case).
 
     Entry points uniform along the y = 0 and y = length edges (alternating),
     in the inner 80% of the edge; directions within +-max_angle_deg of the
     inward normal; kinetic energies uniform in e_range (MeV); rank caps
-    uniform integers in rank_range (initial rank cap // 2).  The angular width
+    uniform integers in rank_range (initial rank cap // 2 for the rank-adaptive
+    integrator; the fixed-rank one keeps the cap throughout).  The angular width
     sigma_Omega^-1 = 1000 (about 2 degrees) keeps every tilted beam on several
     P_15 ordinates.  Returns a list of (Settings, problem dict).
     """
@@ -45,7 +47,7 @@ def beamlet_plan(n_beamlets=64, n=128, N=15, seed=0, rank_range=(5, 40), e_range
                      sigmaO1Inv=angular_inv_width, sigmaO3Inv=angular_inv_width,
                      theta=1e-2, thetaIndex=1.0, cfl=1.5)
         s.set_density(rho)
-        plan.append((s, build_problem(s, "subset", INTEGRATOR_BUG)))
+        plan.append((s, build_problem(s, "subset", integrator)))
     return plan
 
 

@@ -119,5 +119,25 @@ void launch_gather_rows(hipStream_t st, const double* A, int lda, int ncols, con
 // each, the indices stored as doubles).
 void launch_scatter_add(hipStream_t st, double* A, int lda, const double* packed, int n);
 
+// ---- adjoint (FP64; y = 1/rho) -------------------------------------------------
+// K = X Z0 - dE sum_d [d interior] X[d,:] Zd: the transposed stencils, blocks
+// laid out as for launch_stream_k, no 1/rho.
+void launch_stream_kt(hipStream_t st, const Grid& g, const double* X, int ldx, int r,
+                      const double* Z, int n, double dE, double* K, int ldk);
+// out += alpha y o T (out may be null); ybar += alpha rowsum(V o T) (V may be null)
+void launch_adj_combine(hipStream_t st, int C, int n, double* out, int ldo, const double* T, int ldt,
+                        const double* V, int ldv, const double* rho, double alpha, double* ybar);
+void launch_axpy(hipStream_t st, size_t n, double alpha, const double* x, double* y);
+void launch_zero_rows(hipStream_t st, double* A, int lda, int ncols, const int* rows, int nrows);
+// transpose of dose += a y (X v + psi m0) for the seed Db; writes g = a y Db
+void launch_dose_adj(hipStream_t st, const Grid& g, const double* Db, double a, const double* X,
+                     int ldx, int r, const double* v, const double* psi, int nq, const double* m0,
+                     double* Xb, double* psib, double* ybar, double* gout);
+// transpose of the 2D uncollided update (restore of ordinate 0 included):
+// out = adjoint of psi before the sweep, ybar accumulates the density part
+void launch_sweep_adj(hipStream_t st, const Grid& g, const double* G, const double* psi_a, int nq,
+                      const double* omega, double dE, double denom, const unsigned char* restore,
+                      double* out, double* ybar);
+
 }  // namespace dev
 }  // namespace titus

@@ -168,7 +168,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep(
 // Zs[(a*NZB + d)*NB + k] and are read as broadcasts.  Axis strides: x = ny*nz,
 // y = nz, z = 1 (ND = 2: nz = 1).  Ghost cells use themselves as neighbours
 // with 1/rho = 0, so their stencil sum is exactly zero and K = X Z0 there.
-template <class TC, int NB, int ND>
+//
+// TR (the transposed stencils of the adjoint): every cell c sums over the
+// neighbours d that are interior cells, K[c,b] = X[c,:] Z0[:,b] - dE * sum_d
+// [d interior] X[d,:] Zd[:,b], with no 1/rho (the adjoint applies it to the
+// result); the centre term counts when c itself is interior.
+template <class TC, int NB, int ND, bool TR>
 __global__ __launch_bounds__(kBlock) void k_stream_k(
     const double* __restrict__ X, int ldx, int r, int nx, int ny, int nz,
     const double* __restrict__ rho, const double* __restrict__ Z, int n, double dE,
@@ -189,19 +194,36 @@ __global__ __launch_bounds__(kBlock) void k_stream_k(
   const int syz = ny * nz;
   const int i = c / syz, rem = c - (c / syz) * syz;
   const int j = rem / nz, kz = rem - (rem / nz) * nz;
-  bool in = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
-  if (ND == 3) in = in && kz >= 1 && kz <= nz - 2;
+  const int coord[3] = {i, j, kz}, ext[3] = {nx, ny, nz};
+  bool inax[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) inax[d] = d >= ND || (coord[d] >= 1 && coord[d] <= ext[d] - 2);
+  const bool in = inax[0] && inax[1] && inax[2];
   const int stride[3] = {syz, nz, 1};
   int nb[2 * ND];
   TC inv[2 * ND];
 #pragma unroll
   for (int d = 0; d < ND; ++d) {
-    nb[2 * d] = in ? c + stride[d] : c;
-    nb[2 * d + 1] = in ? c - stride[d] : c;
-    inv[2 * d] = in ? (TC)(1.0 / rho[nb[2 * d]]) : TC(0);
-    inv[2 * d + 1] = in ? (TC)(1.0 / rho[nb[2 * d + 1]]) : TC(0);
+    if (TR) {
+      // neighbour along axis d is interior when the other axes are and its
+      // own coordinate lies in 1 .. n-2
+      bool others = true;
+#pragma unroll
+      for (int e = 0; e < 3; ++e) if (e != d) others = others && inax[e];
+      const bool up = others && coord[d] + 1 >= 1 && coord[d] + 1 <= ext[d] - 2;
+      const bool dn = others && coord[d] - 1 >= 1 && coord[d] - 1 <= ext[d] - 2;
+      nb[2 * d] = up ? c + stride[d] : c;
+      nb[2 * d + 1] = dn ? c - stride[d] : c;
+      inv[2 * d] = up ? TC(1) : TC(0);
+      inv[2 * d + 1] = dn ? TC(1) : TC(0);
+    } else {
+      nb[2 * d] = in ? c + stride[d] : c;
+      nb[2 * d + 1] = in ? c - stride[d] : c;
+      inv[2 * d] = in ? (TC)(1.0 / rho[nb[2 * d]]) : TC(0);
+      inv[2 * d + 1] = in ? (TC)(1.0 / rho[nb[2 * d + 1]]) : TC(0);
+    }
   }
-  const TC ic = in ? (TC)(1.0 / rho[c]) : TC(0);
+  const TC ic = in ? (TR ? TC(1) : (TC)(1.0 / rho[c])) : TC(0);
   TC a0[NB], as[NB];
 #pragma unroll
   for (int k = 0; k < NB; ++k) { a0[k] = TC(0); as[k] = TC(0); }
@@ -744,13 +766,13 @@ void launch_sweep(hipStream_t st, const Grid& g, const void* pin, void* pout, DT
 }
 
 namespace {
-template <class TC, int NB, int ND>
+template <class TC, int NB, int ND, bool TR = false>
 void stream_k_nd(hipStream_t st, const Grid& g, const double* X, int ldx, int r, const double* Z,
                  int n, double dE, double* K, int ldk) {
   const dim3 grid(blocks_for(g.C), (n + NB - 1) / NB);
   const size_t lds = (size_t)r * (2 * ND + 2) * NB * sizeof(TC);
-  hipLaunchKernelGGL((k_stream_k<TC, NB, ND>), grid, dim3(kBlock), lds, st, X, ldx, r, g.nx, g.ny,
-                     g.nz, g.rho, Z, n, dE, K, ldk);
+  hipLaunchKernelGGL((k_stream_k<TC, NB, ND, TR>), grid, dim3(kBlock), lds, st, X, ldx, r, g.nx,
+                     g.ny, g.nz, g.rho, Z, n, dE, K, ldk);
 }
 template <class TC, int NB>
 void stream_k_nb(hipStream_t st, const Grid& g, const double* X, int ldx, int r, const double* Z,
@@ -874,6 +896,166 @@ void launch_scatter_add(hipStream_t st, double* A, int lda, const double* packed
   if (n == 0) return;
   hipLaunchKernelGGL(k_scatter_add, dim3(blocks_for(n)), dim3(kBlock), 0, st, A, lda, packed, n);
   check_launch("scatter_add");
+}
+
+
+// ---------------------------------------------------------------------------
+// Adjoint kernels (FP64).  y = 1/rho throughout.
+
+// Transposed stencil blocks: K = X Z0 - dE sum_d [d interior] X[d,:] Zd.
+void launch_stream_kt(hipStream_t st, const Grid& g, const double* X, int ldx, int r,
+                      const double* Z, int n, double dE, double* K, int ldk) {
+  if (n == 0) return;
+  if (r > 48) {
+    if (g.ndim() == 3) stream_k_nd<double, 8, 3, true>(st, g, X, ldx, r, Z, n, dE, K, ldk);
+    else stream_k_nd<double, 8, 2, true>(st, g, X, ldx, r, Z, n, dE, K, ldk);
+  } else {
+    if (g.ndim() == 3) stream_k_nd<double, 16, 3, true>(st, g, X, ldx, r, Z, n, dE, K, ldk);
+    else stream_k_nd<double, 16, 2, true>(st, g, X, ldx, r, Z, n, dE, K, ldk);
+  }
+  check_launch("stream_kt");
+}
+
+namespace {
+
+// out[c,:] += alpha y_c T[c,:];  ybar[c] += alpha sum_k V[c,k] T[c,k] (V may be null)
+__global__ void k_adj_combine(int C, int n, double* __restrict__ out, int ldo,
+                              const double* __restrict__ T, int ldt, const double* __restrict__ V,
+                              int ldv, const double* __restrict__ rho, double alpha,
+                              double* __restrict__ ybar) {
+  const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  const double y = 1.0 / rho[c];
+  double acc = 0.0;
+  for (int k = 0; k < n; ++k) {
+    const double t = T[(size_t)k * ldt + c];
+    if (out) out[(size_t)k * ldo + c] += alpha * y * t;
+    if (V) acc += V[(size_t)k * ldv + c] * t;
+  }
+  if (V) ybar[c] += alpha * acc;
+}
+
+// y += alpha x over n contiguous values
+__global__ void k_axpy(size_t n, double alpha, const double* __restrict__ x, double* __restrict__ y) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] += alpha * x[t];
+}
+
+// A[rows[t], :] = 0
+__global__ void k_zero_rows(double* __restrict__ A, int lda, int ncols, const int* __restrict__ rows,
+                            int nrows) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nrows * ncols) return;
+  const int r = (int)(t % nrows), k = (int)(t / nrows);
+  A[(size_t)k * lda + rows[r]] = 0.0;
+}
+
+// Transpose of dose[c] += a y_c (X[c,:] v + psi[c,:] m0) for the seed Db:
+// g_c = a y_c Db_c; Xb += g v^T; psib += g m0^T; ybar += a Db (X v + psi m0).
+__global__ void k_dose_adj(int C, const double* __restrict__ Db, double a,
+                           const double* __restrict__ rho, const double* __restrict__ X, int ldx,
+                           int r, const double* __restrict__ v, const double* __restrict__ psi,
+                           int nq, const double* __restrict__ m0, double* __restrict__ Xb,
+                           double* __restrict__ psib, double* __restrict__ ybar,
+                           double* __restrict__ gout) {
+  const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  const double db = Db[c];
+  const double g = a * db / rho[c];
+  double u = 0.0;
+  for (int k = 0; k < r; ++k) {
+    u += X[(size_t)k * ldx + c] * v[k];
+    Xb[(size_t)k * ldx + c] += g * v[k];
+  }
+  for (int q = 0; q < nq; ++q) {
+    u += psi[(size_t)q * C + c] * m0[q];
+    psib[(size_t)q * C + c] += g * m0[q];
+  }
+  ybar[c] += a * db * u;
+  gout[c] = g;
+}
+
+// Transpose of the uncollided update psi_b = (psi_a - dE F(y psi_a)) / denom
+// with ordinate 0 kept at the restore cells.  G = psib (adjoint of psi_b),
+// G' = G with G'[restore, 0] = 0; t_q = F_q^T G'_q with
+//   F^T: (Omega_x >= 0) (ox/dx) ([c int] G'_c - [c+ex int] G'_{c+ex})
+//        (Omega_x <  0) (ox/dx) ([c-ex int] G'_{c-ex} - [c int] G'_c)
+// and the same along j with Omega_z / dy.  out = (G' - dE y t) / denom (+ G
+// at the kept entries); ybar -= dE / denom sum_q psi_a t.
+__global__ void k_sweep_adj(int nx, int ny, int nq, const double* __restrict__ G,
+                            const double* __restrict__ psi_a, const double* __restrict__ omega,
+                            double dx, double dy, const double* __restrict__ rho, double dE,
+                            double denom, const unsigned char* __restrict__ restore,
+                            double* __restrict__ out, double* __restrict__ ybar) {
+  const int C = nx * ny;
+  const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  const int i = c / ny, j = c - (c / ny) * ny;
+  auto inner = [&](int ii, int jj) { return ii >= 1 && ii <= nx - 2 && jj >= 1 && jj <= ny - 2; };
+  const bool ic = inner(i, j);
+  const bool ixp = inner(i + 1, j), ixm = inner(i - 1, j);
+  const bool ijp = inner(i, j + 1), ijm = inner(i, j - 1);
+  const double y = 1.0 / rho[c];
+  double yb = 0.0;
+  for (int q = 0; q < nq; ++q) {
+    const double* gq = G + (size_t)q * C;
+    auto gp = [&](int cc) { return (q == 0 && restore[cc]) ? 0.0 : gq[cc]; };
+    const double ox = omega[q], oz = omega[2 * nq + q];
+    const double gc = gp(c);
+    double t = 0.0;
+    if (ox >= 0.0) t += (ox / dx) * ((ic ? gc : 0.0) - (ixp ? gp(c + ny) : 0.0));
+    else t += (ox / dx) * ((ixm ? gp(c - ny) : 0.0) - (ic ? gc : 0.0));
+    if (oz >= 0.0) t += (oz / dy) * ((ic ? gc : 0.0) - (ijp ? gp(c + 1) : 0.0));
+    else t += (oz / dy) * ((ijm ? gp(c - 1) : 0.0) - (ic ? gc : 0.0));
+    double o = (gc - dE * y * t) / denom;
+    if (q == 0 && restore[c]) o += gq[c];
+    out[(size_t)q * C + c] = o;
+    yb += psi_a[(size_t)q * C + c] * t;
+  }
+  ybar[c] -= dE / denom * yb;
+}
+
+}  // namespace
+
+void launch_adj_combine(hipStream_t st, int C, int n, double* out, int ldo, const double* T, int ldt,
+                        const double* V, int ldv, const double* rho, double alpha, double* ybar) {
+  if (C == 0) return;
+  hipLaunchKernelGGL(k_adj_combine, dim3(blocks_for(C)), dim3(kBlock), 0, st, C, n, out, ldo, T,
+                     ldt, V, ldv, rho, alpha, ybar);
+  check_launch("adj_combine");
+}
+
+void launch_axpy(hipStream_t st, size_t n, double alpha, const double* x, double* y) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_axpy, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, alpha, x, y);
+  check_launch("axpy");
+}
+
+void launch_zero_rows(hipStream_t st, double* A, int lda, int ncols, const int* rows, int nrows) {
+  const size_t n = (size_t)nrows * ncols;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_zero_rows, dim3(blocks_for(n)), dim3(kBlock), 0, st, A, lda, ncols, rows,
+                     nrows);
+  check_launch("zero_rows");
+}
+
+void launch_dose_adj(hipStream_t st, const Grid& g, const double* Db, double a, const double* X,
+                     int ldx, int r, const double* v, const double* psi, int nq, const double* m0,
+                     double* Xb, double* psib, double* ybar, double* gout) {
+  if (g.C == 0) return;
+  hipLaunchKernelGGL(k_dose_adj, dim3(blocks_for(g.C)), dim3(kBlock), 0, st, g.C, Db, a, g.rho, X,
+                     ldx, r, v, psi, nq, m0, Xb, psib, ybar, gout);
+  check_launch("dose_adj");
+}
+
+void launch_sweep_adj(hipStream_t st, const Grid& g, const double* G, const double* psi_a, int nq,
+                      const double* omega, double dE, double denom, const unsigned char* restore,
+                      double* out, double* ybar) {
+  if (g.ndim() != 2) throw std::runtime_error("sweep adjoint: 2D grids only");
+  if (g.C == 0) return;
+  hipLaunchKernelGGL(k_sweep_adj, dim3(blocks_for(g.C)), dim3(kBlock), 0, st, g.nx, g.ny, nq, G,
+                     psi_a, omega, g.dx, g.dy, g.rho, dE, denom, restore, out, ybar);
+  check_launch("sweep_adj");
 }
 
 }  // namespace dev

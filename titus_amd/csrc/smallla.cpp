@@ -343,8 +343,10 @@ bool canonical_completion(const HMat& Bc, const std::vector<int>& cand_index, in
 }
 
 HMat range_basis(const HMat& A, int ncols, const HMat* against, double scale, double tol,
-                 bool greedy, const std::vector<int>& candidates, int* rank_out) {
+                 bool greedy, const std::vector<int>& candidates, int* rank_out, OrthRecord* rec) {
   const int m = A.m;
+  if (rec && against && against->n > 0)
+    throw std::runtime_error("range_basis: no adjoint record with a basis to project out");
   HMat Ap = A;
   if (against && against->n > 0) {
     for (int pass = 0; pass < 2; ++pass) {
@@ -366,6 +368,14 @@ HMat range_basis(const HMat& A, int ncols, const HMat* against, double scale, do
   HMat B = p > 0 ? cols(U, 0, p) : HMat(m, 0);
   if (rank_out) *rank_out = p;
   const int need = ncols - p;
+  if (rec) {
+    rec->p = p;
+    rec->U = U;
+    rec->V = V;
+    rec->sv = sv;
+    rec->Tp = HMat(p, ncols);
+    for (int i = 0; i < p; ++i) rec->Tp(i, i) = 1.0;
+  }
   if (need == 0) return B;
   HMat base = (against && against->n > 0) ? hcat(*against, B) : B;
   std::vector<int> cand = candidates;
@@ -380,6 +390,15 @@ HMat range_basis(const HMat& A, int ncols, const HMat* against, double scale, do
   HMat Rinv;
   if (!canonical_completion(Bc, cand, need, greedy, chosen, Rinv))
     throw std::runtime_error("canonical completion ran out of candidates");
+  if (rec && p > 0) {
+    // completion (E - U_p U_p[chosen]^T) Rinv: coefficients -U_p[chosen]^T Rinv
+    HMat Uch(need, p);
+    for (int s = 0; s < need; ++s)
+      for (int k = 0; k < p; ++k) Uch(s, k) = B(chosen[s], k);
+    HMat T = matmul(Uch, Rinv, true, false);
+    for (int s = 0; s < need; ++s)
+      for (int k = 0; k < p; ++k) rec->Tp(k, p + s) = -T(k, s);
+  }
   // Y = E_chosen - base * Bc_chosen^T, completion = Y * Rinv
   HMat Y(m, need);
   for (int s = 0; s < need; ++s) {
@@ -391,6 +410,43 @@ HMat range_basis(const HMat& A, int ncols, const HMat* against, double scale, do
     }
   }
   return hcat(B, matmul(Y, Rinv));
+}
+
+HMat orth_bar(const OrthRecord& rec, const HMat& X1, const HMat& X1bar) {
+  const int m = rec.U.m, k = (int)rec.sv.size(), p = rec.p;
+  HMat Ab(m, rec.V.m);
+  if (p == 0) return Ab;
+  // adjoint of U_p, projected off span(X1)
+  HMat Ub = matmul(X1bar, rec.Tp, false, true);                 // m x p
+  HMat h = matmul(X1, Ub, true, false);
+  HMat Xh = matmul(X1, h);
+  for (size_t i = 0; i < Ub.a.size(); ++i) Ub.a[i] -= Xh.a[i];
+  // (I - U U^T) Ub Sigma_p^-1 V_p^T
+  HMat Y = Ub;
+  for (int c = 0; c < p; ++c)
+    for (int i = 0; i < m; ++i) Y(i, c) /= rec.sv[c];
+  HMat UtY = matmul(rec.U, Y, true, false);                     // k x p
+  HMat UUtY = matmul(rec.U, UtY);
+  for (size_t i = 0; i < Y.a.size(); ++i) Y.a[i] -= UUtY.a[i];
+  HMat Vp = cols(rec.V, 0, p);
+  Ab = matmul(Y, Vp, false, true);
+  if (k > p) {
+    // beta = U_d^T Ub; coupling of each kept i with each dropped j
+    HMat UtUb = matmul(rec.U, Ub, true, false);                 // k x p
+    HMat B1(k - p, p), B2(p, k - p);
+    for (int i = 0; i < p; ++i)
+      for (int j = p; j < k; ++j) {
+        const double si = rec.sv[i], sj = rec.sv[j];
+        const double f = UtUb(j, i) / (si * si - sj * sj);
+        B1(j - p, i) = f * si;
+        B2(i, j - p) = f * sj;
+      }
+    HMat Ud = cols(rec.U, p, k), Up = cols(rec.U, 0, p), Vd = cols(rec.V, p, k);
+    HMat t1 = matmul(matmul(Ud, B1), Vp, false, true);
+    HMat t2 = matmul(matmul(Up, B2), Vd, false, true);
+    for (size_t i = 0; i < Ab.a.size(); ++i) Ab.a[i] += t1.a[i] + t2.a[i];
+  }
+  return Ab;
 }
 
 void svd_completed(const HMat& S, double tol, HMat& U, std::vector<double>& d, HMat& V) {

@@ -79,9 +79,25 @@ bool canonical_completion(const HMat& Bc, const std::vector<int>& cand_index,
 // singular values <= tol * scale treated as zero (scale < 0: sigma_1 of the
 // projected A), completed canonically from `candidates` (all rows when empty)
 // to `ncols` columns.  Returns the basis (m x ncols) and its numerical rank.
+// `rec` (without `against` only) receives what the adjoint of the basis
+// update needs: the thin SVD A = U diag(sv) V^T, the kept rank p and the
+// coefficients Tp (p x ncols) of the basis on the kept vectors U[:, :p].
+struct OrthRecord {
+  int p = 0;
+  HMat U, V;
+  std::vector<double> sv;
+  HMat Tp;
+};
 HMat range_basis(const HMat& A, int ncols, const HMat* against, double scale,
                  double tol, bool greedy, const std::vector<int>& candidates,
-                 int* rank_out = nullptr);
+                 int* rank_out = nullptr, OrthRecord* rec = nullptr);
+
+// Adjoint of X1 = range_basis(A): Abar from X1bar.  X1 moves only through
+// its kept singular subspace U_p, dX1 = (I - X1 X1^T) dU_p Tp (mod span X1);
+// dU_p has the part (I - U U^T) dA V_p Sigma_p^-1 outside range(A) and, for
+// each dropped pair j > p, u_j (s_i u_j^T dA v_i + s_j u_i^T dA v_j) /
+// (s_i^2 - s_j^2).
+HMat orth_bar(const OrthRecord& rec, const HMat& X1, const HMat& X1bar);
 
 // SVD of a square matrix whose singular values <= tol * s_1 are set to 0 and
 // whose corresponding singular vectors are completed canonically (greedy).
