@@ -163,6 +163,13 @@ int titus_kernel_stats(const struct titus_ctx* ctx, const char* name, double* to
  * Writes at most n values. */
 int titus_get_tuning(const struct titus_ctx* ctx, const char* kernel, int32_t* values, int32_t n);
 
+/* Running hash (FNV-1a) of the solve's discrete decisions -- kept rank and
+ * completion vectors of every basis update, truncation ranks -- and their
+ * count.  No reference counterpart.  Two solves with equal hashes took the
+ * same decisions, so a central difference between them differentiates one
+ * smooth branch of the forward map. */
+int titus_get_decisions(const struct titus_ctx* ctx, uint64_t* hash, int64_t* count);
+
 /* ---- adjoint (no reference counterpart: the reference has no gradients) ----
  * The differentiated map is this library's fixed-rank solve
  * (SolveFirstCollisionSourceDLR, code/SolverCSD.jl:891-1120): the dose after

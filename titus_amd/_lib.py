@@ -27,7 +27,7 @@ ERRORS = {
 EXPORTED = ("titus_create", "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
             "titus_get_state",
             "titus_get_history", "titus_set_timing", "titus_kernel_stats", "titus_get_tuning",
-            "titus_set_checkpoints", "titus_adjoint", "titus_adjoint_batch",
+            "titus_get_decisions", "titus_set_checkpoints", "titus_adjoint", "titus_adjoint_batch",
             "titus_destroy", "titus_last_error", "titus_abi_version")
 
 
@@ -81,6 +81,8 @@ def load():
     lib.titus_kernel_stats.argtypes = [vp, ctypes.c_char_p, _P, ctypes.POINTER(ctypes.c_int64),
                                        _P, _P]
     lib.titus_get_tuning.argtypes = [vp, ctypes.c_char_p, _PI, ctypes.c_int32]
+    lib.titus_get_decisions.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64),
+                                        ctypes.POINTER(ctypes.c_int64)]
     lib.titus_set_checkpoints.argtypes = [vp, ctypes.c_int32]
     lib.titus_adjoint.argtypes = [vp, _P, _P]
     lib.titus_adjoint_batch.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32,
@@ -91,7 +93,8 @@ def load():
     lib.titus_abi_version.restype = ctypes.c_int32
     for name in ("titus_create", "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
                  "titus_get_state", "titus_get_history", "titus_set_timing",
-                 "titus_kernel_stats", "titus_get_tuning", "titus_set_checkpoints",
+                 "titus_kernel_stats", "titus_get_tuning", "titus_get_decisions",
+                 "titus_set_checkpoints",
                  "titus_adjoint", "titus_adjoint_batch"):
         getattr(lib, name).restype = ctypes.c_int
     if lib.titus_abi_version() != ABI_VERSION:
@@ -250,6 +253,12 @@ class Context:
         check(self._lib.titus_get_tuning(self.h, kernel.encode(), _ip(v), 4))
         return {"rows": int(v[0]), "groups": int(v[1]), "tuned": bool(v[2]),
                 "nontemporal": bool(v[3])}
+
+    def decisions(self):
+        """(hash, count) of the discrete decisions taken so far."""
+        h, n = ctypes.c_uint64(), ctypes.c_int64()
+        check(self._lib.titus_get_decisions(self.h, ctypes.byref(h), ctypes.byref(n)))
+        return h.value, n.value
 
     def set_checkpoints(self, every):
         """Keep the state every `every` steps of the coming solve (for adjoint)."""

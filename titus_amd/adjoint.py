@@ -69,17 +69,23 @@ def plan_objective_rrule(problems, weights, scale, target, every=16, nthreads=0,
     resid = doses @ w - np.asarray(target, dtype=np.float64)
     J = float(resid @ resid)
 
-    def pullback(Jbar=1.0):
+    def pullback(Jbar=1.0, per_beamlet=False):
+        """(wbar, sbar); with per_beamlet also each beamlet's share of sbar
+        (C x B) and the forward's decision hashes."""
         try:
             wbar = Jbar * 2.0 * (doses.T @ resid)
             bars = Jbar * 2.0 * resid[:, None] * w[None, :]
             grads = _lib.adjoint_batch(ctxs, bars, nthreads)        # d/d rho, C x B
             sbar = rho0 * grads.sum(axis=1)
+            if per_beamlet:
+                return wbar, sbar, rho0[:, None] * grads, [c.decisions()[0] for c in ctxs]
             return wbar, sbar
         finally:
             for c in ctxs:
                 c.close()
 
+    pullback.doses = doses
+    pullback.residual = resid
     return J, pullback
 
 

@@ -383,6 +383,8 @@ void run_adjoint(titus_ctx& x, const double* dose_bar, double* grad) {
   HIPCHK(hipMemsetAsync(a.Xb, 0, sizeof(double) * cr, st));
   HIPCHK(hipMemsetAsync(a.psib, 0, sizeof(double) * cq, st));
   HIPCHK(hipMemsetAsync(a.ybar, 0, sizeof(double) * C, st));
+  const uint64_t hash_was = x.decision_hash;
+  const int64_t decisions_was = x.decisions;
   const bool timing_was = x.timing.enabled;
   x.timing.enabled = false;
   Reverse rev(x, a);
@@ -407,9 +409,13 @@ void run_adjoint(titus_ctx& x, const double* dose_bar, double* grad) {
   } catch (...) {
     x.tape = nullptr;
     x.timing.enabled = timing_was;
+    x.decision_hash = hash_was;
+    x.decisions = decisions_was;
     throw;
   }
   x.timing.enabled = timing_was;
+  x.decision_hash = hash_was;
+  x.decisions = decisions_was;
   // d/drho = -y^2 d/dy
   std::vector<double> yb(C), rho(C);
   HIPCHK(hipMemcpyAsync(yb.data(), a.ybar, sizeof(double) * C, hipMemcpyDeviceToHost, st));
@@ -466,6 +472,9 @@ int titus_adjoint_batch(titus_ctx** ctxs, int32_t n, int32_t nthreads, const dou
   std::vector<int> codes(n, TITUS_OK);
   std::vector<std::string> msgs(n);
   std::atomic<int> next{0};
+  if (nt > 1)
+    for (int i = 0; i < n; ++i)
+      if (ctxs[i]) ctxs[i]->host_parallel = false;
   auto worker = [&] {
     for (;;) {
       const int i = next.fetch_add(1);

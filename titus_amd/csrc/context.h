@@ -237,6 +237,15 @@ struct titus_ctx {
   std::vector<int> ranks;
   std::vector<double> etas;
   mutable Timing timing;
+  // running hash of the solve's discrete decisions (kept ranks, completion
+  // vectors, truncation ranks): equal hashes of two solves mean the same
+  // decisions, i.e. both lie on one smooth branch of the forward map
+  uint64_t decision_hash = 1469598103934665603ull;
+  int64_t decisions = 0;
+  void fold(int64_t v) {
+    decision_hash = (decision_hash ^ (uint64_t)v) * 1099511628211ull;
+    ++decisions;
+  }
   // adjoint: checkpoints and recomputation (adjoint.cpp)
   std::shared_ptr<AdjointStore> adj;
   StepTape* tape = nullptr;
@@ -340,16 +349,35 @@ struct titus_ctx {
   // by the Galerkin products, the L-step (A (W S^T) = (A W) S^T) and the
   // augmented basis (A [W Wt] = [A W, A Wt])
   using AWs = std::vector<HMat>;
+  // f(o) for every angular operator o: on one thread per operator when the
+  // products are large enough to pay for the threads (the 3D operators) and
+  // the context is not one of a batch (whose contexts already run side by
+  // side); each product is computed the same way either way
+  bool host_parallel = true;
+  template <class F>
+  void for_ops(size_t work, F&& f) const {
+    const int n = (int)sA.size();
+    if (!host_parallel || work < (size_t(1) << 18) || n < 2) {
+      for (int o = 0; o < n; ++o) f(o);
+      return;
+    }
+    std::vector<std::thread> th;
+    for (int o = 1; o < n; ++o) th.emplace_back([&f, o] { f(o); });
+    f(0);
+    for (auto& t : th) t.join();
+  }
   AWs apply_a(const HMat& Wm) const {
     HostScope hs(timing, "host:galerkin_w");
-    AWs out;
-    for (const titus::SpMat& A : sA) out.push_back(titus::spmm(A, Wm));
+    AWs out(sA.size());
+    for_ops(sA.empty() ? 0 : sA[0].val.size() * (size_t)Wm.n,
+            [&](int o) { out[o] = titus::spmm(sA[o], Wm); });
     return out;
   }
   std::vector<HMat> galerkin_from(const HMat& Wa, const AWs& awb) const {
     HostScope hs(timing, "host:galerkin_w");
-    std::vector<HMat> out;
-    for (const HMat& aw : awb) out.push_back(titus::matmul(Wa, aw, true));
+    std::vector<HMat> out(awb.size());
+    for_ops((size_t)Wa.m * Wa.n * (awb.empty() ? 0 : awb[0].n),
+            [&](int o) { out[o] = titus::matmul(Wa, awb[o], true); });
     return out;
   }
   std::vector<HMat> galerkin_w(const HMat& Wa, const HMat& Wb) const {
@@ -410,10 +438,15 @@ struct titus_ctx {
     return Z;
   }
   HMat range_w(const HMat& L, int ncols, const HMat* against, double scale,
-               titus::OrthRecord* rec = nullptr) const {
+               titus::OrthRecord* rec = nullptr) {
     HostScope hs(timing, "host:range_w");
     static const std::vector<int> all;
-    return titus::range_basis(L, ncols, against, scale, rank_tol, true, all, nullptr, rec);
+    int p = 0;
+    std::vector<int> chosen;
+    HMat B = titus::range_basis(L, ncols, against, scale, rank_tol, true, all, &p, rec, &chosen);
+    fold(p);
+    for (int c : chosen) fold(c);
+    return B;
   }
   void stream_k(int r, const std::vector<HMat>& WAW) {
     const double* Zd = upload(stream_Z(S, WAW));
@@ -517,6 +550,8 @@ struct titus_ctx {
     HMat Rinv;
     if (need > 0 && !titus::canonical_completion(Bc, cidx, need, false, chosen, Rinv))
       throw TitusError(TITUS_E_RANK, "canonical completion ran out of candidates");
+    fold(p);
+    for (int c : chosen) fold(c);
     std::vector<int> chpos;
     for (int c : chosen)
       chpos.push_back((int)(std::find(cidx.begin(), cidx.end(), c) - cidx.begin()));
@@ -649,6 +684,7 @@ struct titus_ctx {
     begin_step(s, Dvec, Dout);
     if (tape) { tape->Dvec = Dvec; tape->Dout = Dout; tape->sig0 = sig(s)[0]; }
     const int r = rank;
+    size_t xtp0 = 0;
     if (s > 1) {
       // K-step and old Galerkin products (SolverCSD.jl:1019-1038)
       areset();
@@ -660,13 +696,14 @@ struct titus_ctx {
       fetch();
       std::vector<HMat> XLX = galerkin_view(gx, r, r);
       range_finish(j, r, Xn, tape ? &tape->Rx1 : nullptr);
-      // L-step on the host while the device writes XNew (:1033-1046)
-      HMat L = stream_L(aw, XLX);
-      HMat WNew = range_w(L, r, nullptr, -1.0, tape ? &tape->Rw1 : nullptr);
-      HMat NUp = titus::matmul(WNew, W, true, false);
+      // the device writes XNew and its products while the host runs the
+      // L-step (:1033-1046)
       areset();
       const size_t mu = tn(Xn, r, X, r, dev::OP_NONE);
       const size_t gn = galerkin_x(Xn, r, Xn, r);
+      HMat L = stream_L(aw, XLX);
+      HMat WNew = range_w(L, r, nullptr, -1.0, tape ? &tape->Rw1 : nullptr);
+      HMat NUp = titus::matmul(WNew, W, true, false);
       fetch();
       HMat MUp = view(mu, r, r);
       if (tape) {
@@ -676,6 +713,10 @@ struct titus_ctx {
       XLX = galerkin_view(gn, r, r);
       W = WNew;
       std::swap(X, Xn);
+      // X^T psi of the in-scattering needs only the new X: it runs on the
+      // device during the host S-step and out-scattering
+      areset();
+      xtp0 = tn_psi(X, r);
       // S-step (:1051-1063)
       S = titus::matmul(titus::matmul(MUp, S), NUp, false, true);
       const std::vector<HMat> WAW1 = galerkin_w(W, W);
@@ -684,26 +725,28 @@ struct titus_ctx {
         HIPCHK(hipMemcpyAsync(tape->X1, X, cr, hipMemcpyDeviceToDevice, st));
       }
       S = stream_S(S, XLX, WAW1);
+    } else {
+      areset();
+      xtp0 = tn_psi(X, r);
     }
     out_scatter(Dout);
     // in-scattering (:1079-1102)
-    areset();
-    const size_t xtp0 = tn_psi(X, r);                       // X^T psi, old X
     const HMat MtDW = mtdw(Dvec, W);
     scatter_K(MtDW);
     RangeJob j = range_launch(K, r, nullptr, 0);
     fetch();
     HMat PtX = titus::transpose(view(xtp0, r, nq));
     range_finish(j, r, Xn, tape ? &tape->Rx3 : nullptr);
+    // products of XNew on the device while the host builds the new W
+    areset();
+    const size_t mu = tn(Xn, r, X, r, dev::OP_NONE);
+    const size_t xtp = tn_psi(Xn, r);
     HMat L = titus::matmul(W, S, false, true);
     HMat MP = titus::matmul(M, PtX);
     for (int c = 0; c < r; ++c)
       for (int i = 0; i < nm; ++i) L(i, c) += dE * Dvec[i] * MP(i, c);
     HMat WNew = range_w(L, r, nullptr, -1.0, tape ? &tape->Rw3 : nullptr);
     HMat NUp = titus::matmul(WNew, W, true, false);
-    areset();
-    const size_t mu = tn(Xn, r, X, r, dev::OP_NONE);
-    const size_t xtp = tn_psi(Xn, r);
     fetch();
     HMat MUp = view(mu, r, r);
     HMat XtP = view(xtp, r, nq);
@@ -740,13 +783,14 @@ struct titus_ctx {
       if (std::fabs(tail) < tol) { rmax = jj + 1; break; }
     }
     rmax = std::max(std::min(rmax, rank_cap), 2);
+    fold(rmax);
     gemm(X1, n, titus::cols(U, 0, rmax), 1.0, 0.0, X, nullptr);
     W = titus::matmul(W1, titus::cols(V, 0, rmax));
     S = HMat(rmax, rmax);
     for (int i = 0; i < rmax; ++i) S(i, i) = D[i];
     rank = rmax;
   }
-  HMat augment_w(const HMat& L) const {
+  HMat augment_w(const HMat& L) {
     return titus::hcat(W, range_w(L, rank, &W, titus::frob(L)));
   }
   void copy_x_to_aug(int r) {
@@ -768,11 +812,12 @@ struct titus_ctx {
       fetch();
       std::vector<HMat> XLX = galerkin_view(gx, r, r);
       range_finish(j, r, Xn + (size_t)r * C);               // Xn = X1 = [X | Xt]
-      HMat W1 = augment_w(stream_L(aw, XLX));
-      HMat Nu = titus::matmul(W1, W, true, false);
+      // products of X1 on the device while the host augments W
       areset();
       const size_t mu = tn(Xn, 2 * r, X, r, dev::OP_NONE);
       const size_t g1 = galerkin_x(Xn, 2 * r, Xn, 2 * r);
+      HMat W1 = augment_w(stream_L(aw, XLX));
+      HMat Nu = titus::matmul(W1, W, true, false);
       fetch();
       HMat Mu = view(mu, 2 * r, r);
       XLX = galerkin_view(g1, 2 * r, 2 * r);
@@ -795,26 +840,28 @@ struct titus_ctx {
       etas.push_back(titus::frob(acc));
       truncate_to(Sup, Xn, W1);
     }
-    out_scatter(Dout);
-    // in-scattering (:2252-2306)
+    // X^T psi (old X) on the device during the host out-scattering
     const int r = rank;
     areset();
-    const size_t xtp = tn_psi(X, r);                          // old X
+    const size_t xtp = tn_psi(X, r);
+    out_scatter(Dout);
+    // in-scattering (:2252-2306)
     scatter_K(mtdw(Dvec, W));
     copy_x_to_aug(r);
     RangeJob j = range_launch(K, r, X, r);
     fetch();
     HMat Xtpsi = view(xtp, r, nq);
     range_finish(j, r, Xn + (size_t)r * C);
+    // products of X1 on the device while the host augments W
+    areset();
+    const size_t mu = tn(Xn, 2 * r, X, r, dev::OP_NONE);
+    const size_t x1p = tn_psi(Xn, 2 * r);
     HMat L = titus::matmul(W, S, false, true);
     HMat MX = titus::matmul(M, Xtpsi, false, true);
     for (int c = 0; c < r; ++c)
       for (int i = 0; i < nm; ++i) L(i, c) += dE * Dvec[i] * MX(i, c);
     HMat W1 = augment_w(L);
     HMat Nu = titus::matmul(W1, W, true, false);
-    areset();
-    const size_t mu = tn(Xn, 2 * r, X, r, dev::OP_NONE);
-    const size_t x1p = tn_psi(Xn, 2 * r);
     fetch();
     HMat Mu = view(mu, 2 * r, r);
     HMat S1 = titus::matmul(titus::matmul(Mu, S), Nu, false, true);

@@ -343,7 +343,8 @@ bool canonical_completion(const HMat& Bc, const std::vector<int>& cand_index, in
 }
 
 HMat range_basis(const HMat& A, int ncols, const HMat* against, double scale, double tol,
-                 bool greedy, const std::vector<int>& candidates, int* rank_out, OrthRecord* rec) {
+                 bool greedy, const std::vector<int>& candidates, int* rank_out, OrthRecord* rec,
+                 std::vector<int>* chosen_out) {
   const int m = A.m;
   if (rec && against && against->n > 0)
     throw std::runtime_error("range_basis: no adjoint record with a basis to project out");
@@ -367,6 +368,7 @@ HMat range_basis(const HMat& A, int ncols, const HMat* against, double scale, do
   p = std::min(p, ncols);
   HMat B = p > 0 ? cols(U, 0, p) : HMat(m, 0);
   if (rank_out) *rank_out = p;
+  if (chosen_out) chosen_out->clear();
   const int need = ncols - p;
   if (rec) {
     rec->p = p;
@@ -390,6 +392,7 @@ HMat range_basis(const HMat& A, int ncols, const HMat* against, double scale, do
   HMat Rinv;
   if (!canonical_completion(Bc, cand, need, greedy, chosen, Rinv))
     throw std::runtime_error("canonical completion ran out of candidates");
+  if (chosen_out) *chosen_out = chosen;
   if (rec && p > 0) {
     // completion (E - U_p U_p[chosen]^T) Rinv: coefficients -U_p[chosen]^T Rinv
     HMat Uch(need, p);

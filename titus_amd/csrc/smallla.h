@@ -88,9 +88,11 @@ struct OrthRecord {
   std::vector<double> sv;
   HMat Tp;
 };
+// `chosen_out` receives the completion candidates taken, in column order.
 HMat range_basis(const HMat& A, int ncols, const HMat* against, double scale,
                  double tol, bool greedy, const std::vector<int>& candidates,
-                 int* rank_out = nullptr, OrthRecord* rec = nullptr);
+                 int* rank_out = nullptr, OrthRecord* rec = nullptr,
+                 std::vector<int>* chosen_out = nullptr);
 
 // Adjoint of X1 = range_basis(A): Abar from X1bar.  X1 moves only through
 // its kept singular subspace U_p, dX1 = (I - X1 X1^T) dU_p Tp (mod span X1);

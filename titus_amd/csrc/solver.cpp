@@ -257,6 +257,9 @@ int titus_solve_batch(titus_ctx** ctxs, int32_t n, int32_t nthreads, int32_t* st
   std::vector<int> codes(n, TITUS_OK);
   std::vector<std::string> msgs(n);
   std::atomic<int> next{0};
+  if (nt > 1)
+    for (int i = 0; i < n; ++i)
+      if (ctxs[i]) ctxs[i]->host_parallel = false;
   auto worker = [&] {
     for (;;) {
       const int i = next.fetch_add(1);
@@ -358,6 +361,14 @@ int titus_get_tuning(const titus_ctx* ctx, const char* kernel, int32_t* values, 
     const int32_t v[4] = {ctx->sweep_plan.rows, ctx->sweep_plan.groups, ctx->sweep_tuned ? 1 : 0,
                           ctx->sweep_plan.nontemporal ? 1 : 0};
     for (int i = 0; i < n && i < 4; ++i) values[i] = v[i];
+  });
+}
+
+int titus_get_decisions(const titus_ctx* ctx, uint64_t* hash, int64_t* count) {
+  return guarded([&] {
+    require(ctx != nullptr, "null context");
+    if (hash) *hash = ctx->decision_hash;
+    if (count) *count = ctx->decisions;
   });
 }
 
