@@ -421,7 +421,7 @@ def main():
                     help="skip the plan-gradient (adjoint) measurement")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the plan-gradient (adjoint) measurement")
-    ap.add_argument("--fd-dirs", type=int, default=2,
+    ap.add_argument("--fd-dirs", type=int, default=1,
                     help="random directions of the plan-gradient difference check")
     ap.add_argument("--stencil-only", action="store_true",
                     help="only the large-grid stencil measurement (+ calibration copy), for PMC runs")

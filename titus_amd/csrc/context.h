@@ -524,7 +524,7 @@ struct titus_ctx {
       HMat R = titus::matmul(R3, titus::matmul(R2, R1));
       HMat P, Zr;
       std::vector<double> sv;
-      titus::svd(R, P, sv, Zr);
+      titus::svd(R, P, sv, Zr, ot != nullptr);
       const double sc = scale < 0.0 ? (sv.empty() ? 0.0 : sv[0]) : scale;
       if (sc > 0.0)
         for (double x : sv) if (x > rank_tol * sc) ++p;

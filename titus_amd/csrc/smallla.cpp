@@ -104,6 +104,21 @@ double frob(const HMat& A) {
   return std::sqrt(s);
 }
 
+// Dot product with four independent partial sums (vectorisable; the sum
+// order is fixed, so results are reproducible).
+static inline double dot(const double* a, const double* b, int n) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int i = 0;
+  for (; i + 4 <= n; i += 4) {
+    s0 += a[i] * b[i];
+    s1 += a[i + 1] * b[i + 1];
+    s2 += a[i + 2] * b[i + 2];
+    s3 += a[i + 3] * b[i + 3];
+  }
+  for (; i < n; ++i) s0 += a[i] * b[i];
+  return (s0 + s1) + (s2 + s3);
+}
+
 // ---------------------------------------------------------------------------
 // Householder QR of an m x n matrix, m >= n.  On return A holds R in its upper
 // triangle and the reflector vectors (unit leading entry implied) below.
@@ -112,8 +127,7 @@ static void householder_qr(HMat& A, std::vector<double>& tau) {
   tau.assign(n, 0.0);
   for (int j = 0; j < n; ++j) {
     double* a = A.col(j);
-    double xnorm2 = 0.0;
-    for (int i = j + 1; i < m; ++i) xnorm2 += a[i] * a[i];
+    const double xnorm2 = dot(a + j + 1, a + j + 1, m - j - 1);
     const double alpha = a[j];
     if (xnorm2 == 0.0) { tau[j] = 0.0; continue; }
     const double nrm = std::sqrt(alpha * alpha + xnorm2);
@@ -125,8 +139,7 @@ static void householder_qr(HMat& A, std::vector<double>& tau) {
     // apply H = I - tau v v^T to the trailing columns
     for (int k = j + 1; k < n; ++k) {
       double* b = A.col(k);
-      double w = b[j];
-      for (int i = j + 1; i < m; ++i) w += a[i] * b[i];
+      double w = b[j] + dot(a + j + 1, b + j + 1, m - j - 1);
       w *= tau[j];
       b[j] -= w;
       for (int i = j + 1; i < m; ++i) b[i] -= w * a[i];
@@ -142,8 +155,7 @@ static void apply_q(const HMat& QR, const std::vector<double>& tau, HMat& B) {
     const double* v = QR.col(j);
     for (int k = 0; k < B.n; ++k) {
       double* b = B.col(k);
-      double w = b[j];
-      for (int i = j + 1; i < m; ++i) w += v[i] * b[i];
+      double w = b[j] + dot(v + j + 1, b + j + 1, m - j - 1);
       w *= tau[j];
       b[j] -= w;
       for (int i = j + 1; i < m; ++i) b[i] -= w * v[i];
@@ -151,28 +163,23 @@ static void apply_q(const HMat& QR, const std::vector<double>& tau, HMat& B) {
   }
 }
 
-// One-sided Jacobi on a square n x n matrix B (overwritten by U diag(s)).
-static void jacobi_square(HMat& B, HMat& V) {
+// One-sided Jacobi on a square n x n matrix B (overwritten by U diag(s));
+// the rotations are accumulated in V only when want_v.
+static void jacobi_square(HMat& B, HMat& V, bool want_v) {
   const int n = B.n;
-  V = HMat::eye(n, n);
+  if (want_v) V = HMat::eye(n, n);
   const double tol = 1e-15;
   std::vector<double> nrm(n);
   for (int sweep = 0; sweep < 60; ++sweep) {
     bool rotated = false;
-    for (int j = 0; j < n; ++j) {
-      const double* b = B.col(j);
-      double s = 0.0;
-      for (int i = 0; i < B.m; ++i) s += b[i] * b[i];
-      nrm[j] = s;
-    }
+    for (int j = 0; j < n; ++j) nrm[j] = dot(B.col(j), B.col(j), B.m);
     for (int p = 0; p < n - 1; ++p) {
       for (int q = p + 1; q < n; ++q) {
         double* bp = B.col(p);
         double* bq = B.col(q);
         const double alpha = nrm[p], beta = nrm[q];
         if (alpha == 0.0 || beta == 0.0) continue;
-        double gamma = 0.0;
-        for (int i = 0; i < B.m; ++i) gamma += bp[i] * bq[i];
+        const double gamma = dot(bp, bq, B.m);
         if (std::fabs(gamma) <= tol * std::sqrt(alpha * beta)) continue;
         rotated = true;
         const double zeta = (beta - alpha) / (2.0 * gamma);
@@ -184,12 +191,14 @@ static void jacobi_square(HMat& B, HMat& V) {
           bp[i] = c * x - s * y;
           bq[i] = s * x + c * y;
         }
-        double* vp = V.col(p);
-        double* vq = V.col(q);
-        for (int i = 0; i < n; ++i) {
-          const double x = vp[i], y = vq[i];
-          vp[i] = c * x - s * y;
-          vq[i] = s * x + c * y;
+        if (want_v) {
+          double* vp = V.col(p);
+          double* vq = V.col(q);
+          for (int i = 0; i < n; ++i) {
+            const double x = vp[i], y = vq[i];
+            vp[i] = c * x - s * y;
+            vq[i] = s * x + c * y;
+          }
         }
         // squared norms after the rotation (exact in exact arithmetic);
         // recomputed from the columns at the start of every sweep
@@ -201,10 +210,10 @@ static void jacobi_square(HMat& B, HMat& V) {
   }
 }
 
-void svd(const HMat& A0, HMat& U, std::vector<double>& s, HMat& V) {
+void svd(const HMat& A0, HMat& U, std::vector<double>& s, HMat& V, bool want_v) {
   if (A0.m < A0.n) {
     HMat Ut, Vt;
-    svd(transpose(A0), Ut, s, Vt);
+    svd(transpose(A0), Ut, s, Vt, true);
     U = Vt;
     V = Ut;
     return;
@@ -217,26 +226,22 @@ void svd(const HMat& A0, HMat& U, std::vector<double>& s, HMat& V) {
   for (int j = 0; j < n; ++j)
     for (int i = 0; i <= j; ++i) R(i, j) = QR(i, j);
   HMat Vr;
-  jacobi_square(R, Vr);
+  jacobi_square(R, Vr, want_v);
   std::vector<double> sv(n);
-  for (int j = 0; j < n; ++j) {
-    const double* b = R.col(j);
-    double t = 0.0;
-    for (int i = 0; i < n; ++i) t += b[i] * b[i];
-    sv[j] = std::sqrt(t);
-  }
+  for (int j = 0; j < n; ++j) sv[j] = std::sqrt(dot(R.col(j), R.col(j), n));
   std::vector<int> ord(n);
   std::iota(ord.begin(), ord.end(), 0);
   std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return sv[a] > sv[b]; });
   HMat Ur(m, n);   // padded to m rows for apply_q
-  V = HMat(n, n);
+  V = want_v ? HMat(n, n) : HMat();
   s.assign(n, 0.0);
   for (int k = 0; k < n; ++k) {
     const int j = ord[k];
     s[k] = sv[j];
     if (sv[j] > 0.0)
       for (int i = 0; i < n; ++i) Ur(i, k) = R(i, j) / sv[j];
-    for (int i = 0; i < n; ++i) V(i, k) = Vr(i, j);
+    if (want_v)
+      for (int i = 0; i < n; ++i) V(i, k) = Vr(i, j);
   }
   apply_q(QR, tau, Ur);
   U = Ur;
@@ -360,7 +365,7 @@ HMat range_basis(const HMat& A, int ncols, const HMat* against, double scale, do
   std::vector<double> sv;
   int p = 0;
   if (Ap.n > 0) {
-    svd(Ap, U, sv, V);
+    svd(Ap, U, sv, V, rec != nullptr);
     if (scale < 0.0) scale = sv.empty() ? 0.0 : sv[0];
     if (scale > 0.0)
       for (double x : sv) if (x > tol * scale) ++p;

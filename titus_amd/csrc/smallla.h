@@ -49,7 +49,8 @@ double frob(const HMat& A);
 // followed by one-sided (Hestenes) Jacobi on the triangular factor: singular
 // values carry errors of order eps * s_1, singular vectors of order
 // eps * s_1 / gap, as LAPACK's gesvd.
-void svd(const HMat& A, HMat& U, std::vector<double>& s, HMat& V);
+// V is left empty when want_v is false (m >= n only).
+void svd(const HMat& A, HMat& U, std::vector<double>& s, HMat& V, bool want_v = true);
 
 // Cholesky A = R^T R of a symmetric positive definite matrix; returns false
 // (and leaves R undefined) when a pivot is not positive.
