@@ -1,4 +1,5 @@
-"""The oracle's discrete adjoint against central finite differences.
+"""The oracle's discrete adjoint against central finite differences and
+against the forward-mode (dual-number) derivative of the same map.
 
 ``oracle.adjoint`` differentiates ``oracle.dlra.solve_fixed`` (the fixed-rank
 loop of ``SolveFirstCollisionSourceDLR``, ``code/SolverCSD.jl:891-1120``) with
@@ -16,7 +17,7 @@ import math
 import numpy as np
 import pytest
 
-from oracle import adjoint, dlra
+from oracle import adjoint, dlra, tangent
 from titus_amd.problem import INTEGRATOR_FIXED, build_problem
 from titus_amd.settings import Settings, ellipse_phantom
 
@@ -65,6 +66,39 @@ def test_adjoint_fd_40x40_p7():
     singular values sit within a factor 3 of each other at the rank cut."""
     P = _beam(39, 7, 20)
     _fd_check(P, None, 3, 2, 1e-5)
+
+
+def test_adjoint_matches_forward_mode_40x40_p7():
+    """Dot-product test against the forward-mode (dual-number) derivative of
+    the same map, 40 x 40 cells, P_7, rank 20, the whole energy grid:
+    <dose_bar, d dose[v]> = <grad, v> with no difference step involved."""
+    P = _beam(39, 7, 20)
+    _, tape = adjoint.forward(P)
+    Lh, Fh = adjoint.stencil_matrices(P), adjoint.flux_matrices(P)
+    rng = np.random.default_rng(6)
+    Db = rng.standard_normal(P["C"])
+    g = adjoint.adjoint(P, tape, Db, Lh, Fh)
+    rho0 = np.asarray(P["density"])
+    for _ in range(3):
+        v = rng.standard_normal(P["C"]) * rho0
+        dd = tangent.tangent(P, tape, v, Lh, Fh)
+        assert abs(Db @ dd - g @ v) <= 1e-9 * abs(g @ v)
+
+
+def test_forward_mode_matches_differences():
+    """The forward-mode derivative of the whole dose vector against central
+    differences (24 x 24, P_5, rank 6, 30 steps)."""
+    P = _beam(23, 5, 6)
+    _, tape = adjoint.forward(P, 30)
+    rng = np.random.default_rng(7)
+    rho0 = np.asarray(P["density"])
+    for _ in range(3):
+        v = rng.standard_normal(P["C"]) * rho0
+        dd = tangent.tangent(P, tape, v)
+        h = 1e-7 * np.linalg.norm(rho0) / np.linalg.norm(v)
+        fd = (dlra.solve_fixed(dict(P, density=rho0 + h * v), 30)["dose"]
+              - dlra.solve_fixed(dict(P, density=rho0 - h * v), 30)["dose"]) / (2 * h)
+        assert np.linalg.norm(dd - fd) <= 1e-5 * np.linalg.norm(fd)
 
 
 def test_adjoint_is_linear_in_the_seed():

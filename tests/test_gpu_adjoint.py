@@ -2,9 +2,10 @@
 
 The differentiated map is the fixed-rank solve (``SolveFirstCollisionSourceDLR``,
 ``code/SolverCSD.jl:891-1120``) as a function of the density.  No reference
-adjoint exists; the pins are the oracle's reverse pass (itself pinned by
-central differences in ``tests/test_adjoint_oracle.py``) and central
-differences of the device forward map on 32 random directions.
+adjoint exists; the pins are the oracle's reverse pass and its forward-mode
+(dual-number) derivative (both pinned by central differences in
+``tests/test_adjoint_oracle.py``) and central differences of the device
+forward map on 32 random directions.
 """
 import math
 
@@ -12,6 +13,7 @@ import numpy as np
 import pytest
 
 from oracle import adjoint as oadj
+from oracle import tangent as otan
 from titus_amd import _lib
 from titus_amd.adjoint import piecewise_target, plan_objective_grad
 from titus_amd.beamlets import beamlet_plan
@@ -60,6 +62,22 @@ def test_adjoint_matches_oracle_40x40():
     assert _rel(d0, dose_o) < 1e-10
     assert np.array_equal(d0, d1)                 # forward state restored
     assert _rel(g, g_o) < 1e-6
+
+
+def test_adjoint_matches_forward_mode_40x40_p7():
+    """The device gradient against the oracle's forward-mode (dual-number)
+    derivative: <grad, v> = <dose_bar, d dose[v]> on three directions."""
+    P = _beam(39, 7, 20)
+    rng = np.random.default_rng(3)
+    Db = rng.standard_normal(P["C"])
+    _, tape = oadj.forward(P)
+    _, _, g = _device_grad(P, 0, Db, 16)
+    rho0 = np.asarray(P["density"])
+    Lh, Fh = oadj.stencil_matrices(P), oadj.flux_matrices(P)
+    for _ in range(3):
+        v = rng.standard_normal(P["C"]) * rho0
+        dd = otan.tangent(P, tape, v, Lh, Fh)
+        assert abs(Db @ dd - g @ v) <= 1e-7 * abs(Db @ dd)
 
 
 def test_checkpoint_interval_does_not_change_the_gradient():
