@@ -33,6 +33,7 @@ namespace {
 
 struct Checkpoint {
   int s = 0;
+  bool saved = false;
   HMat S, W;
   double* X = nullptr;
   double* psi = nullptr;
@@ -64,20 +65,31 @@ struct AdjointStore {
   }
 };
 
+// Device slots of every checkpoint, allocated when checkpointing is enabled
+// (so the solve itself allocates nothing); slot i holds the state before
+// step 1 + i * every.
+void adjoint_reserve(titus_ctx& x) {
+  AdjointStore& a = *x.adj;
+  const int n = (x.ne - 2) / a.every + 1;                 // steps 1 .. ne-1
+  a.cps.resize(n);
+  for (int i = 0; i < n; ++i) {
+    a.cps[i].s = 1 + i * a.every;
+    a.cps[i].X = a.alloc((size_t)x.C * x.rank);
+    a.cps[i].psi = a.alloc((size_t)x.C * x.nq);
+  }
+}
+
 void adjoint_checkpoint(titus_ctx& x, int s) {
   AdjointStore& a = *x.adj;
   if (x.tape || (s - 1) % a.every != 0) return;
-  for (const Checkpoint& c : a.cps)
-    if (c.s == s) return;
-  Checkpoint c;
-  c.s = s;
+  const size_t i = (size_t)((s - 1) / a.every);
+  if (i >= a.cps.size()) throw TitusError(TITUS_E_STATE, "adjoint: checkpoint beyond the grid");
+  Checkpoint& c = a.cps[i];
   c.S = x.S;
   c.W = x.W;
-  c.X = a.alloc((size_t)x.C * x.rank);
-  c.psi = a.alloc((size_t)x.C * x.nq);
+  c.saved = true;
   HIPCHK(hipMemcpyAsync(c.X, x.X, sizeof(double) * (size_t)x.C * x.rank, hipMemcpyDeviceToDevice, x.st));
   HIPCHK(hipMemcpyAsync(c.psi, x.psi, sizeof(double) * (size_t)x.C * x.nq, hipMemcpyDeviceToDevice, x.st));
-  a.cps.push_back(std::move(c));
 }
 
 namespace {
@@ -345,7 +357,7 @@ void run_adjoint(titus_ctx& x, const double* dose_bar, double* grad) {
   if (!x.adj) throw TitusError(TITUS_E_STATE, "adjoint: checkpoints were not enabled before the solve");
   AdjointStore& a = *x.adj;
   if (x.steps_done < 1) throw TitusError(TITUS_E_STATE, "adjoint: nothing solved yet");
-  if (a.cps.empty() || a.cps.front().s != 1)
+  if (a.cps.empty() || !a.cps.front().saved)
     throw TitusError(TITUS_E_STATE, "adjoint: no checkpoint before the first step");
   const int C = x.C, r = x.rank, nq = x.nq;
   const size_t cr = (size_t)C * r, cq = (size_t)C * nq;
@@ -392,6 +404,7 @@ void run_adjoint(titus_ctx& x, const double* dose_bar, double* grad) {
     for (int ci = (int)a.cps.size() - 1; ci >= 0; --ci) {
       const Checkpoint& cp = a.cps[ci];
       if (cp.s > done) continue;
+      if (!cp.saved) throw TitusError(TITUS_E_STATE, "adjoint: missing checkpoint");
       const int s1 = (ci + 1 < (int)a.cps.size()) ? std::min(done, a.cps[ci + 1].s - 1) : done;
       if (s1 - cp.s + 1 > (int)a.tapes.size()) throw TitusError(TITUS_E_STATE, "adjoint: checkpoint gap");
       // restore and recompute the segment with the tape on
@@ -450,6 +463,7 @@ int titus_set_checkpoints(titus_ctx* ctx, int32_t every) {
     if (ctx->steps_done != 0) throw TitusError(TITUS_E_STATE, "checkpoints must be set before the first step");
     ctx->adj = std::make_shared<AdjointStore>();
     ctx->adj->every = every;
+    adjoint_reserve(*ctx);
   });
 }
 

@@ -17,6 +17,7 @@ constexpr int kTnBTiles = 4;         // B tiles of one A^T B wavefront (shared A
 constexpr int kXcds = 8;             // accelerator complex dies of an MI355X
 constexpr int kSweepCols = 62;       // output columns of one sweep wavefront
 constexpr int kRowsKT = 256;         // rows of T staged in LDS at a time (k_gemm)
+constexpr int kMaxChol = 96;         // widest block of k_chol_inv
 
 inline void check_launch(const char* what) {
   hipError_t e = hipGetLastError();
@@ -376,6 +377,23 @@ template <> struct Mfma<float> {
 // XCD-contiguous, so neighbouring chunks (stencil halo rows) and the tile
 // groups of one chunk (shared A) run on the same XCD.  The four wavefronts'
 // sums are added in LDS in a fixed order.
+// Four consecutive values at p (32-byte aligned for double, 16 for float) in
+// one vector load, converted to TC; zeros when !ok.
+template <class TS, class TC>
+__device__ inline void load4(const TS* p, bool ok, TC* v) {
+  if (!ok) {
+    v[0] = v[1] = v[2] = v[3] = TC(0);
+    return;
+  }
+  if constexpr (sizeof(TS) == 8) {
+    const double4 q = *reinterpret_cast<const double4*>(p);
+    v[0] = (TC)q.x; v[1] = (TC)q.y; v[2] = (TC)q.z; v[3] = (TC)q.w;
+  } else {
+    const float4 q = *reinterpret_cast<const float4*>(p);
+    v[0] = (TC)q.x; v[1] = (TC)q.y; v[2] = (TC)q.z; v[3] = (TC)q.w;
+  }
+}
+
 template <int NOPS, int NBT, class TB, class TC>
 __global__ __launch_bounds__(kBlock) void k_tn_mfma(
     const double* __restrict__ A, int lda, int m, const TB* __restrict__ B, int ldb, int n,
@@ -412,9 +430,24 @@ __global__ __launch_bounds__(kBlock) void k_tn_mfma(
   acc_t acc[NACC];
 #pragma unroll
   for (int o = 0; o < NACC; ++o) acc[o] = acc_t{};
+  // plain products over whole 16-cell blocks: each lane's four cells are one
+  // aligned vector load when both leading dimensions keep columns aligned
+  const bool vec = NOPS == 1 && lda % 4 == 0 && ldb % 4 == 0 &&
+                   reinterpret_cast<uintptr_t>(A) % 32 == 0 &&
+                   reinterpret_cast<uintptr_t>(B) % (4 * sizeof(TB)) == 0;
   for (int c0 = cbeg; c0 < cend; c0 += 16) {
     const int cs = c0 + 4 * kg;
     TC av[4], bv[NACC][4];
+    if (NOPS == 1 && vec && c0 + 16 <= cend) {
+      load4(Acol + cs, va, av);
+#pragma unroll
+      for (int t = 0; t < NBT; ++t) load4(Bcol[t] + cs, vb[t], bv[t]);
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int o = 0; o < NACC; ++o) acc[o] = MF::step(av[s], bv[o][s], acc[o]);
+      continue;
+    }
     // cell coordinates of the lane's first cell, advanced with carries
     int ii = cs / syz, jj = (cs - (cs / syz) * syz) / nz, kk = cs - (cs / nz) * nz;
 #pragma unroll
@@ -501,8 +534,11 @@ __global__ void k_tn_reduce(const double* __restrict__ partial, int nchunks, int
 __global__ void k_chol_inv(const double* __restrict__ G, int k, int allow_plain,
                            double* __restrict__ R, double* __restrict__ Rinv,
                            double* __restrict__ status) {
-  extern __shared__ double A[];              // k x k, column-major
-  __shared__ int ok;
+  // A (k x k, column-major, LDS): the factor in the upper triangle; its
+  // diagonal is kept apart in Dg, and the strict lower triangle receives
+  // R^-1 transposed during the back substitution
+  extern __shared__ double A[];
+  __shared__ double Dg[kMaxChol];
   __shared__ double tr;
   const int tid = threadIdx.x, nt = blockDim.x;
   if (tid == 0) {
@@ -513,22 +549,20 @@ __global__ void k_chol_inv(const double* __restrict__ G, int k, int allow_plain,
   __syncthreads();
   double s = allow_plain ? 0.0 : (tr > 0.0 ? 1e-14 * tr : 1e-300);
   int attempt = 0;
+  bool good = false;
   for (; attempt < 13; ++attempt) {
     for (int e = tid; e < k * k; e += nt) {
       const int i = e % k, j = e / k;
       A[e] = G[e] + (i == j ? s : 0.0);
     }
-    if (tid == 0) ok = 1;
     __syncthreads();
+    good = true;
     for (int j = 0; j < k; ++j) {
-      if (tid == 0) {
-        const double d = A[(size_t)j * k + j];
-        if (!(d > 0.0)) ok = 0;
-        else A[(size_t)j * k + j] = sqrt(d);
-      }
-      __syncthreads();
-      if (!ok) break;
-      const double piv = A[(size_t)j * k + j];
+      // every thread reads the same pivot, so the exit below is uniform
+      const double d = A[(size_t)j * k + j];
+      if (!(d > 0.0)) { good = false; break; }
+      const double piv = sqrt(d);
+      if (tid == 0) Dg[j] = piv;
       for (int c = j + 1 + tid; c < k; c += nt) A[(size_t)c * k + j] /= piv;
       __syncthreads();
       const int w = k - j - 1;                // trailing block (j, k) x (j, k)
@@ -538,25 +572,30 @@ __global__ void k_chol_inv(const double* __restrict__ G, int k, int allow_plain,
       }
       __syncthreads();
     }
-    if (ok) break;
+    if (good) break;
     __syncthreads();
     s = (s == 0.0) ? (tr > 0.0 ? 1e-14 * tr : 1e-300) : 10.0 * s;
   }
-  const int good = ok;
   __syncthreads();
   for (int e = tid; e < k * k; e += nt) {
     const int i = e % k, j = e / k;
-    R[e] = (i <= j) ? A[e] : 0.0;
+    R[e] = (i < j) ? A[e] : (i == j ? Dg[j] : 0.0);
   }
-  // column c of R^-1 by back substitution (one thread per column)
+  __syncthreads();
+  // column c of R^-1 by back substitution, one thread per column; entry
+  // (i, c), i < c, is kept in the lower triangle at A[i*k + c]
   for (int c = tid; c < k; c += nt) {
-    for (int i = 0; i < k; ++i) Rinv[(size_t)c * k + i] = 0.0;
-    Rinv[(size_t)c * k + c] = 1.0 / A[(size_t)c * k + c];
+    const double dc = 1.0 / Dg[c];
     for (int i = c - 1; i >= 0; --i) {
-      double acc = 0.0;
-      for (int m = i + 1; m <= c; ++m) acc += A[(size_t)m * k + i] * Rinv[(size_t)c * k + m];
-      Rinv[(size_t)c * k + i] = -acc / A[(size_t)i * k + i];
+      double acc = A[(size_t)c * k + i] * dc;
+      for (int m = i + 1; m < c; ++m) acc += A[(size_t)m * k + i] * A[(size_t)m * k + c];
+      A[(size_t)i * k + c] = -acc / Dg[i];
     }
+  }
+  __syncthreads();
+  for (int e = tid; e < k * k; e += nt) {
+    const int i = e % k, c = e / k;
+    Rinv[e] = (i < c) ? A[(size_t)i * k + c] : (i == c ? 1.0 / Dg[c] : 0.0);
   }
   if (tid == 0) status[0] = good ? (double)(attempt + 1) : -1.0;
 }
