@@ -41,9 +41,13 @@ METRIC = "energy steps/s & beamlets/s, mixed vs FP64 GPU at dose rel-L2 error â‰
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 
-# HBM bytes per launch of the dominant kernel measured with rocprofv3 --pmc
-# (FETCH_SIZE + WRITE_SIZE, scaled by a calibration copy); None until measured.
-MEASURED_TRAFFIC = {}
+# HBM bytes per launch of the headline's dominant kernel, measured with
+# rocprofv3 --pmc on this workload (FETCH_SIZE x 2 + WRITE_SIZE x 1, the
+# factors from a 256 MiB calibration copy; commands and per-kernel figures in
+# profiles/dlra_step_phantom200_p21_pmc.md).  "gram" is one launch_tn call: a
+# k_tn_mfma<1, ...> dispatch plus its k_tn_reduce, averaged over all such
+# calls.  Re-measure when the kernels change.
+MEASURED_TRAFFIC = {"gram": 3.19e6}
 
 
 def build_workload(n=200, N=21, cap=40, seed=0):
