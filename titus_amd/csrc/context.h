@@ -27,6 +27,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -361,10 +362,16 @@ struct titus_ctx {
       for (int o = 0; o < n; ++o) f(o);
       return;
     }
+    std::vector<std::exception_ptr> err(n);
     std::vector<std::thread> th;
-    for (int o = 1; o < n; ++o) th.emplace_back([&f, o] { f(o); });
-    f(0);
+    for (int o = 1; o < n; ++o)
+      th.emplace_back([&f, &err, o] {
+        try { f(o); } catch (...) { err[o] = std::current_exception(); }
+      });
+    try { f(0); } catch (...) { err[0] = std::current_exception(); }
     for (auto& t : th) t.join();
+    for (auto& e : err)
+      if (e) std::rethrow_exception(e);
   }
   AWs apply_a(const HMat& Wm) const {
     HostScope hs(timing, "host:galerkin_w");
