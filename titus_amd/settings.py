@@ -15,6 +15,10 @@ caller of ``Settings(Nx, Ny, r, problem, particle)`` finds the same attributes
                     ``sigmaO1Inv``/``sigmaO3Inv``/``sigmaEInv``;
 * ``rMax``       -- rank cap of the adaptive integrator (reference: ``r``,
                     ``SolverCSD.jl:3373``);
+* ``phantomImage`` -- path of the image the "lung" / "liver" presets read
+                    (reference: ``Lung.png`` / ``liver_cut.jpg`` in the working
+                    directory, ``settings.jl:266-310``); ``None`` keeps a unit
+                    density;
 * ``csPhase``    -- sign convention of the real spherical harmonics.  With
                     ``False`` the ordinate transform satisfies
                     ``M diag(Omega_x) O = +Ax`` (physically consistent x-streaming);
@@ -67,6 +71,7 @@ class Settings:
         self.rMax = None
         self.csPhase = False
         self.floorPsi = 1e-17          # SolverCSD.jl:906
+        self.phantomImage = overrides.pop("phantomImage", None)
 
         if problem in ("waterBeam", "waterBeamElectrons"):
             # settings.jl:175-242 (density inserts of those presets omitted: the
@@ -88,7 +93,24 @@ class Settings:
             self.sigmaS = 1.0
             self.cfl = 0.99 / math.sqrt(2)
             self.thetaIndex, self.theta = 0.0, 0.3
-        elif problem in ("lung", "slab", "phantom"):
+        elif problem in ("lung", "liver"):
+            # settings.jl:266-310: density from an image, 1.85 g/cc at white,
+            # floored at densityMin; then the preset's domain and beam
+            self.densityMin = 0.05
+            if self.phantomImage is not None:
+                self.density = image_density(self.phantomImage, self.NCellsX, self.NCellsY,
+                                             self.densityMin)
+            self.cfl = 1.5
+            self.Omega1, self.Omega3 = -1.0, -1.0
+            if problem == "lung":
+                self.b = self.d = 14.5
+                self.eMax = math.sqrt(21.0 ** 2 + eRest ** 2)
+                self.x0, self.y0 = 0.5 * self.b, 1.0 * self.d
+            else:
+                self.b = self.d = 35.0
+                self.eMax = 60.0
+                self.x0, self.y0 = 1.0 * self.b, 0.35 * self.d
+        elif problem in ("slab", "phantom"):
             # "lung"-style beam (SolverCSD.jl:242-247): exp profiles, energy factor
             pass
         else:
@@ -126,6 +148,40 @@ class Settings:
     @property
     def rank_cap(self):
         return self.r if self.rMax is None else int(self.rMax)
+
+
+# ---------------------------------------------------------------------------
+# image phantoms (settings.jl:266-310)
+# ---------------------------------------------------------------------------
+def image_gray(path):
+    """``Float64.(Gray.(load(path)))``: an (rows x columns) array in [0, 1].
+
+    8-bit images; gray values are k/255.  Colour images use the ITU-R BT.601
+    luma weights in single precision rounded back to 8 bits, as Colors.jl's
+    RGB-to-Gray conversion; where R = G = B (the shipped Lung.png) every
+    weighting gives k/255 exactly, so only colour inputs depend on the rule.
+    """
+    from PIL import Image
+    im = Image.open(path)
+    if im.mode == "L":
+        return np.asarray(im, dtype=np.float64) / 255.0
+    a = np.asarray(im.convert("RGB"), dtype=np.float32) / np.float32(255.0)
+    g = np.float32(0.299) * a[..., 0] + np.float32(0.587) * a[..., 1] + np.float32(0.114) * a[..., 2]
+    return np.rint(g.astype(np.float64) * 255.0) / 255.0
+
+
+def image_density(image, NCellsX, NCellsY, density_min, scale=1.85):
+    """density[i, j] = max(scale * img[floor(i/NCellsX*nx), floor(j/NCellsY*ny)],
+    density_min) with 1-based i, j and image indices (settings.jl:276-281);
+    `image` is a path or an (nx x ny) gray array."""
+    img = image_gray(image) if isinstance(image, (str, bytes)) or hasattr(image, "__fspath__") \
+        else np.asarray(image, dtype=np.float64)
+    nx, ny = img.shape[:2]
+    ii = [int(math.floor(i / NCellsX * nx)) - 1 for i in range(1, NCellsX + 1)]
+    jj = [int(math.floor(j / NCellsY * ny)) - 1 for j in range(1, NCellsY + 1)]
+    if min(ii) < 0 or min(jj) < 0:
+        raise ValueError("image smaller than the grid: the reference's index map starts at 0")
+    return np.maximum(scale * img[np.ix_(ii, jj)], density_min)
 
 
 # ---------------------------------------------------------------------------
