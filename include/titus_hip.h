@@ -44,7 +44,10 @@ enum {
   TITUS_E_NODEVICE = -6    /* no usable GPU */
 };
 
-enum { TITUS_INTEGRATOR_FIXED = 0, TITUS_INTEGRATOR_BUG = 1 };
+/* FIXED: SolveFirstCollisionSourceDLR (SolverCSD.jl:891-1120); BUG:
+ * ...DLRBUGRejection with the rejection branches off (:2011-2374); PARALLEL:
+ * SolveFirstCollisionSourceDLRParallel (:1726-2008). */
+enum { TITUS_INTEGRATOR_FIXED = 0, TITUS_INTEGRATOR_BUG = 1, TITUS_INTEGRATOR_PARALLEL = 2 };
 /* FP64: every operation in FP64.
  * MIXED: the uncollided flux psi stored and swept in FP32 (half the HBM
  *   traffic of the dominant kernel); everything else in FP64.
@@ -137,8 +140,10 @@ int titus_progress(const struct titus_ctx* ctx, int32_t* steps_done, int32_t* ra
 int titus_get_state(struct titus_ctx* ctx, double* dose, double* X, double* S, double* W,
                     double* psi);
 
-/* Rank before each step taken (`rankInTime`, SolverCSD.jl:2109) and the
- * streaming eta indicator (SolverCSD.jl:2220); n entries each at most. */
+/* Rank history and the streaming eta indicator, n entries each at most.
+ * BUG: the rank before each step (`rankInTime`, SolverCSD.jl:2109) and eta
+ * (SolverCSD.jl:2220).  PARALLEL: the rank after each step (`rVec[2,:]`,
+ * SolverCSD.jl:1946); no eta. */
 int titus_get_history(const struct titus_ctx* ctx, int32_t* ranks, double* etas, int32_t n);
 
 /* Device time of the library's kernels since timing was (re)enabled, measured

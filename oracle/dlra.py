@@ -9,6 +9,8 @@ It restates, in numpy with T = Float64 throughout:
 * ``solve_fixed``  -- ``SolveFirstCollisionSourceDLR``   (``code/SolverCSD.jl:891-1120``)
 * ``solve_bug``    -- ``SolveFirstCollisionSourceDLRBUGRejection`` with the
                       rejection branches off (``code/SolverCSD.jl:2011-2374``)
+* ``solve_parallel`` -- ``SolveFirstCollisionSourceDLRParallel``
+                      (``code/SolverCSD.jl:1726-2008``)
 * ``truncate``     -- ``truncate!``                       (``code/SolverCSD.jl:3369-3399``)
 * ``upwind_flux``  -- ``solveFluxUpwind!``                (``code/SolverCSD.jl:388-463``)
 * ``stencils``     -- ``Stencils`` applied matrix-free    (``code/stencils.jl:12-130``)
@@ -503,6 +505,92 @@ def solve_bug(P, steps=None):
     U, sv, Vt = np.linalg.svd(S)
     return dict(dose=dose, X=X, S=S, W=W, psi=psi, Xout=X @ U, sigma=sv / np.sqrt(2.0),
                 WV=W @ Vt.T, ranks=np.array(ranks), etas=np.array(etas))
+
+
+def solve_parallel(P, steps=None):
+    """Parallel BUG (``SolveFirstCollisionSourceDLRParallel``,
+    ``SolverCSD.jl:1726-2008``).
+
+    The K- and L-steps both start from the old factors; the S-step uses the
+    old Galerkin matrices; the augmented coefficient matrix is
+    [[S, L^T Wt], [Xt^T K, 0]] (``:1867-1888``, ``:1916-1940``), truncated by
+    ``truncate!``.  The second dose half uses the r x r S before the
+    truncation (``:1944-1950``).  The loop runs over the whole grid.  Returns
+    the solve_bug fields plus ``rvec`` (2 x nE: time and rank after each step,
+    the first column holding the initial rank in both rows, as ``:1805``).
+    """
+    ops = _Ops(P)
+    r = P["r"]
+    X = np.array(P["X0"], dtype=np.float64)
+    W = np.array(P["W0"], dtype=np.float64)
+    S = np.array(P["S0"], dtype=np.float64)
+    C, Q = P["C"], P["Q"]
+    psi = np.zeros((C, Q))
+    dose = np.zeros(C)
+    rho = ops.rho
+    dE = P["dE"]
+    Sst = P["Sst"]
+    omega = P["omega"]
+    M = ops.M
+    theta, thidx, cap = P["theta"], P["thetaIndex"], P["rank_cap"]
+    nE = P["nE"] if steps is None else min(P["nE"], steps + 1)
+    icand = ops.interior
+    mcand = range(ops.Nm)
+    rvec = np.full((2, P["nE"]), float(r))
+    t = 0.0
+    X[ops.bcells, :] = 0.0
+
+    def augmented(S_, Xt, K, L, Wt):
+        n = S_.shape[0]
+        Sn = np.zeros((2 * n, 2 * n))
+        Sn[:n, :n] = S_
+        Sn[n:, :n] = Xt.T @ K
+        Sn[:n, n:] = L.T @ Wt
+        return Sn
+
+    for s in range(1, nE):
+        sig = P["sigma"][s]
+        psi = _begin_step(ops, P, psi, s)
+        dose += 0.5 * dE * (X @ (S @ W[0, :]) + psi @ M[0, :]) * Sst[s - 1] / rho
+        psi = _sweep(ops, psi, sig[0], dE, omega)
+        Dvec = sig[ops.deg]
+        Dout = sig[0] - Dvec
+        if s > 1:
+            XLX = ops.galerkin_x(X, X)
+            WAW = ops.galerkin_w(W, W)
+            X[ops.bcells, :] = 0.0
+            K = ops.stream_K(X @ S, WAW, dE)
+            Xt = range_basis(K, r, icand, against=X, scale=np.linalg.norm(K))
+            L = ops.stream_L(W @ S.T, XLX, dE)
+            Wt = range_basis(L, r, mcand, against=W, scale=np.linalg.norm(L), greedy=True)
+            S = ops.stream_S(S, XLX, WAW, dE)
+            Sn = augmented(S, Xt, K, L, Wt)
+            X, S, W = truncate(np.hstack([X, Xt]), Sn, np.hstack([W, Wt]), theta, thidx, cap)
+            r = S.shape[0]
+        # out-scattering (:1891-1898)
+        L = (W @ S.T) / (1.0 + dE * Dout)[:, None]
+        W = range_basis(L, r, mcand, greedy=True)
+        S = L.T @ W
+        # in-scattering (:1900-1940)
+        MtDW = M.T @ (Dvec[:, None] * W)
+        Xtpsi = X.T @ psi
+        X[ops.bcells, :] = 0.0
+        K = X @ S + dE * psi @ MtDW
+        K[ops.bcells, :] = 0.0
+        Xt = range_basis(K, r, icand, against=X, scale=np.linalg.norm(K))
+        L = W @ S.T + dE * Dvec[:, None] * (M @ Xtpsi.T)
+        Wt = range_basis(L, r, mcand, against=W, scale=np.linalg.norm(L), greedy=True)
+        S = S + dE * Xtpsi @ MtDW
+        Sn = augmented(S, Xt, K, L, Wt)
+        dose += 0.5 * dE * (X @ (S @ W[0, :]) + psi @ M[0, :]) * Sst[s] / rho
+        X, S, W = truncate(np.hstack([X, Xt]), Sn, np.hstack([W, Wt]), theta, thidx, cap)
+        r = S.shape[0]
+        rvec[0, s] = t
+        rvec[1, s] = r
+        t += dE
+    U, sv, Vt = np.linalg.svd(S)
+    return dict(dose=dose, X=X, S=S, W=W, psi=psi, Xout=X @ U, sigma=sv / np.sqrt(2.0),
+                WV=W @ Vt.T, rvec=rvec)
 
 
 def solve_dense(P, steps=None, stream=True):

@@ -1,5 +1,5 @@
 """Problem instances shared by the parity tests (seeded, small enough for the oracle)."""
-from titus_amd.problem import INTEGRATOR_BUG, INTEGRATOR_FIXED, build_problem
+from titus_amd.problem import INTEGRATOR_BUG, INTEGRATOR_FIXED, INTEGRATOR_PARALLEL, build_problem
 from titus_amd.settings import Settings, ellipse_phantom, heterogeneous_phantom, water_slab_pencil_beam
 
 
@@ -22,6 +22,15 @@ def small_bug(n=30, N=7, cap=12, seed=3, steps=None, angular=2.0):
     if steps is not None:
         s.nEnergies = steps + 1
     return build_problem(s, "subset", INTEGRATOR_BUG)
+
+
+def small_parallel(n=30, N=7, cap=12, seed=3, steps=None, angular=2.0):
+    """The small_bug phantom for the parallel integrator."""
+    s = heterogeneous_phantom(n=n, N=N, seed=seed, r=cap, eKin=8.0)
+    s.sigmaO1Inv = s.sigmaO3Inv = angular
+    if steps is not None:
+        s.nEnergies = steps + 1
+    return build_problem(s, "subset", INTEGRATOR_PARALLEL)
 
 
 def slab_beam():

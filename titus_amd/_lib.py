@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtitus_hip.so")
 
 ABI_VERSION = 2
-INTEGRATOR_FIXED, INTEGRATOR_BUG = 0, 1
+INTEGRATOR_FIXED, INTEGRATOR_BUG, INTEGRATOR_PARALLEL = 0, 1, 2
 PRECISION_FP64, PRECISION_MIXED, PRECISION_MIXED_FULL_FP32 = 0, 1, 2
 
 _P = ctypes.POINTER(ctypes.c_double)

@@ -224,6 +224,7 @@ struct titus_ctx {
   dev::SweepPlan sweep_plan{};
   bool sweep_tuned = false;
   double *X = nullptr, *Xn = nullptr, *K = nullptr, *Q1 = nullptr, *Q2 = nullptr;
+  double* Kc = nullptr;                     // copy of K before its projection (parallel BUG)
   double *partial = nullptr, *arena = nullptr, *upl = nullptr;
   int *bcells = nullptr, *cand_d = nullptr;
   unsigned char *bmask = nullptr, *restore = nullptr;
@@ -252,7 +253,7 @@ struct titus_ctx {
   StepTape* tape = nullptr;
 
   ~titus_ctx() {
-    void* ptrs[] = {rho, irho, omega, bc_base, m0, psi, psi2, dose, u0p, X, Xn, K, Q1, Q2, partial, arena,
+    void* ptrs[] = {rho, irho, omega, bc_base, m0, psi, psi2, dose, u0p, X, Xn, K, Kc, Q1, Q2, partial, arena,
                     upl, bcells, cand_d, bmask, restore};
     for (void* p : ptrs) if (p) hipFree(p);
     void* hptrs[] = {h_arena, h_upl};
@@ -878,6 +879,79 @@ struct titus_ctx {
     dose_half(stopping[s]);
   }
 
+  // [[S, L^T Wt], [Xt^T K, 0]] (SolverCSD.jl:1882-1888, 1935-1940)
+  static HMat augmented_s(const HMat& S0, const HMat& XtK, const HMat& L, const HMat& W1) {
+    const int r = S0.m;
+    HMat Sn(2 * r, 2 * r);
+    const HMat LW = titus::matmul(L, titus::cols(W1, r, W1.n), true, false);   // L^T Wt
+    for (int j = 0; j < r; ++j)
+      for (int i = 0; i < r; ++i) {
+        Sn(i, j) = S0(i, j);
+        Sn(r + i, j) = XtK(i, j);
+        Sn(i, r + j) = LW(i, j);
+      }
+    return Sn;
+  }
+
+  // Parallel BUG (SolveFirstCollisionSourceDLRParallel, SolverCSD.jl:1726-2008):
+  // K-, L- and S-steps all start from the old factors and the old Galerkin
+  // products; the new directions Xt, Wt augment the bases and the coupling
+  // blocks Xt^T K, L^T Wt fill the augmented coefficient matrix, which
+  // truncate! reduces.
+  void step_parallel(int s) {
+    std::vector<double> Dvec, Dout;
+    begin_step(s, Dvec, Dout);
+    if (s > 1) {
+      const int r = rank;
+      areset();
+      const size_t gx = galerkin_x(X, r, X, r);
+      const AWs aw = apply_a(W);
+      const std::vector<HMat> WAW = galerkin_from(W, aw);
+      stream_k(r, WAW);                                   // K (:1843-1846)
+      copy_x_to_aug(r);
+      HIPCHK(hipMemcpyAsync(Kc, K, sizeof(double) * (size_t)C * r, hipMemcpyDeviceToDevice, st));
+      RangeJob j = range_launch(K, r, X, r);              // K is left projected off X
+      fetch();
+      std::vector<HMat> XLX = galerkin_view(gx, r, r);
+      range_finish(j, r, Xn + (size_t)r * C);             // Xn = [X | Xt]
+      areset();
+      const size_t xk = tn(Xn + (size_t)r * C, r, Kc, r, dev::OP_NONE);  // Xt^T K
+      HMat L = stream_L(aw, XLX);                          // (:1850-1854)
+      HMat W1 = augment_w(L);
+      HMat Sn = stream_S(S, XLX, WAW);                     // old Galerkin (:1859)
+      fetch();
+      truncate_to(augmented_s(Sn, view(xk, r, r), L, W1), Xn, W1);
+    }
+    // X^T psi (old X) on the device during the host out-scattering
+    const int r = rank;
+    areset();
+    const size_t xtp = tn_psi(X, r);
+    out_scatter(Dout);
+    // in-scattering (:1900-1940)
+    const HMat MtDW = mtdw(Dvec, W);
+    scatter_K(MtDW);
+    copy_x_to_aug(r);
+    HIPCHK(hipMemcpyAsync(Kc, K, sizeof(double) * (size_t)C * r, hipMemcpyDeviceToDevice, st));
+    RangeJob j = range_launch(K, r, X, r);
+    fetch();
+    const HMat Xtpsi = view(xtp, r, nq);
+    range_finish(j, r, Xn + (size_t)r * C);
+    areset();
+    const size_t xk = tn(Xn + (size_t)r * C, r, Kc, r, dev::OP_NONE);
+    HMat L = titus::matmul(W, S, false, true);
+    const HMat MX = titus::matmul(M, Xtpsi, false, true);
+    for (int c = 0; c < r; ++c)
+      for (int i = 0; i < nm; ++i) L(i, c) += dE * Dvec[i] * MX(i, c);
+    HMat W1 = augment_w(L);
+    const HMat add = titus::matmul(Xtpsi, MtDW);
+    for (size_t i = 0; i < S.a.size(); ++i) S.a[i] += dE * add.a[i];
+    fetch();
+    const HMat Sn = augmented_s(S, view(xk, r, r), L, W1);
+    dose_half(stopping[s]);                                // before the truncation (:1944-1950)
+    truncate_to(Sn, Xn, W1);
+    ranks.push_back(rank);                                 // rVec[2, n] (:1946)
+  }
+
   void step(int s) {
     if (s != steps_done + 1) throw TitusError(TITUS_E_STATE, "steps must run in order");
     if (s >= ne) throw TitusError(TITUS_E_STATE, "step beyond the energy grid");
@@ -885,6 +959,7 @@ struct titus_ctx {
     HostScope hs(timing, "host:step");
     if (adj) adjoint_checkpoint(*this, s);
     if (integrator == TITUS_INTEGRATOR_FIXED) step_fixed(s);
+    else if (integrator == TITUS_INTEGRATOR_PARALLEL) step_parallel(s);
     else step_bug(s);
     steps_done = s;
     for (double v : S.a)

@@ -76,7 +76,8 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     require(p->density && p->degree && p->Ax && p->Az && p->absAx && p->absAz && p->omega &&
                 p->M && p->stopping && p->sigma && p->bcells && p->bc_base && p->bc_scale &&
                 p->cand_cells, "missing operand");
-    require(p->integrator == TITUS_INTEGRATOR_FIXED || p->integrator == TITUS_INTEGRATOR_BUG,
+    require(p->integrator == TITUS_INTEGRATOR_FIXED || p->integrator == TITUS_INTEGRATOR_BUG ||
+                p->integrator == TITUS_INTEGRATOR_PARALLEL,
             "unknown integrator");
     require(p->precision == TITUS_PRECISION_FP64 || p->precision == TITUS_PRECISION_MIXED ||
                 p->precision == TITUS_PRECISION_MIXED_FULL_FP32,
@@ -110,9 +111,10 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     x.dx = p->dx; x.dy = p->dy; x.dE = p->dE; x.theta = p->theta; x.theta_index = p->theta_index;
     x.rank_tol = p->rank_tol > 0.0 ? p->rank_tol : 1e-7;
     x.rank = p->r;
-    x.rank_cap = p->integrator == TITUS_INTEGRATOR_BUG ? p->rank_cap : p->r;
+    const bool adaptive = p->integrator != TITUS_INTEGRATOR_FIXED;
+    x.rank_cap = adaptive ? p->rank_cap : p->r;
     require(x.rank_cap >= x.rank, "rank cap below the initial rank");
-    x.capx = p->integrator == TITUS_INTEGRATOR_BUG ? 2 * x.rank_cap : x.rank;
+    x.capx = adaptive ? 2 * x.rank_cap : x.rank;
     require(x.capx <= p->nm || p->integrator == TITUS_INTEGRATOR_FIXED, "2 * rank cap exceeds the moment count");
     require(x.rank <= p->nm, "rank exceeds the moment count");
     require(x.capx <= 96, "rank cap too large for the device Cholesky (2 r <= 96)");
@@ -155,6 +157,7 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     x.X = dalloc<double>(cx);
     x.Xn = dalloc<double>(cx);
     x.K = dalloc<double>(cx);
+    if (p->integrator == TITUS_INTEGRATOR_PARALLEL) x.Kc = dalloc<double>(cx);
     x.Q1 = dalloc<double>(cx);
     x.Q2 = dalloc<double>(cx);
     const int wmax = std::max(x.capx, p->nq);

@@ -22,6 +22,7 @@ from . import physics as ph
 
 INTEGRATOR_FIXED = 0      # SolveFirstCollisionSourceDLR
 INTEGRATOR_BUG = 1        # SolveFirstCollisionSourceDLRBUGRejection (rejection off)
+INTEGRATOR_PARALLEL = 2   # SolveFirstCollisionSourceDLRParallel
 
 
 @functools.lru_cache(maxsize=8)
@@ -65,8 +66,8 @@ def build_problem(s, moments="subset", integrator=INTEGRATOR_FIXED):
     interior = np.ones(C, dtype=bool)
     interior[bcells] = False
 
-    if integrator == INTEGRATOR_BUG:
-        r0 = s.r // 2                         # SolverCSD.jl:2013
+    if integrator in (INTEGRATOR_BUG, INTEGRATOR_PARALLEL):
+        r0 = s.r // 2                         # SolverCSD.jl:2013, 1728
     else:
         r0 = s.r
     if r0 < 1:

@@ -7,6 +7,8 @@ Reference interface (Julia, ``code/SolverCSD.jl``):
   returns ``(X*U, 0.5*sqrt(gamma[1])*Sigma, O*W*V, W*V, dose, psi)``
 * ``SolveFirstCollisionSourceDLRBUGRejection(obj, recordGif=false)`` -- ``:2011-2374``
   returns the above plus ``[E; rank]``, ``[E eta]``, ``[E bound]`` before ``psi``
+* ``SolveFirstCollisionSourceDLRParallel(obj, recordGif=false)`` -- ``:1726-2008``
+  returns the fixed-rank tuple with ``rVec`` (2 x nEnergies) before ``psi``
 
 Like the reference, ``Solve*`` mutates ``obj``: ``obj.dose`` accumulates and
 ``obj.settings.dE`` is overwritten with the grid step.  The energy loop runs
@@ -18,7 +20,7 @@ import math
 import numpy as np
 
 from . import _lib
-from .problem import INTEGRATOR_BUG, INTEGRATOR_FIXED, build_problem
+from .problem import INTEGRATOR_BUG, INTEGRATOR_FIXED, INTEGRATOR_PARALLEL, build_problem
 
 
 # Rank cut of the range rule.  FP64: 1e-7 of the scale.  Mixed form: FP32
@@ -123,3 +125,27 @@ def SolveFirstCollisionSourceDLRBUGRejection(obj, recordGif=False):
     eta_E = P["eGrid"][2:done + 1]                    # eta exists from step 2 on
     eta_hist = np.column_stack([eta_E, etas[:len(eta_E)]]) if done > 1 else np.zeros((0, 2))
     return (*out[:5], rank_hist, eta_hist, np.zeros((0, 2)), out[5])
+
+
+def SolveFirstCollisionSourceDLRParallel(obj, recordGif=False):
+    """Parallel rank-adaptive BUG solve on the GPU (``SolverCSD.jl:1726-2008``).
+
+    Returns ``(X U, Sigma/sqrt2, O W V, W V, dose, rVec, psi)``; ``rVec`` is
+    2 x nEnergies with the energy-step time t (the sum of the steps before
+    it) and the rank after each step, and the initial rank in both rows of
+    the first column, as the reference fills it (``:1805, 1945-1948``).
+    """
+    P = obj.problem(INTEGRATOR_PARALLEL)
+    ctx = obj.context(INTEGRATOR_PARALLEL)
+    try:
+        ctx.solve(0)
+        st = ctx.state()
+        done, _ = ctx.progress()
+        ranks, _ = ctx.history(done)
+    finally:
+        ctx.close()
+    out = _finish(obj, P, st)
+    rvec = np.full((2, P["nE"]), float(P["r"]))
+    rvec[0, 1:done + 1] = np.arange(done) * P["dE"]
+    rvec[1, 1:done + 1] = ranks[:done]
+    return (*out[:5], rvec, out[5])
