@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define TITUS_ABI_VERSION 2
+#define TITUS_ABI_VERSION 3
 
 typedef struct titus_ctx titus_ctx;
 
@@ -61,7 +61,7 @@ enum { TITUS_PRECISION_FP64 = 0, TITUS_PRECISION_MIXED = 1, TITUS_PRECISION_MIXE
  * ordinate reduction and initial factors of the Solve* functions
  * (code/SolverCSD.jl:63-146, 904-961, 2025-2082). */
 typedef struct titus_problem_v1 {
-  int32_t version;            /* TITUS_ABI_VERSION (1 accepted for 2D grids) */
+  int32_t version;            /* TITUS_ABI_VERSION (1 and 2 accepted: fields they lack default) */
   int32_t nx, ny;             /* NCellsX, NCellsY */
   double dx, dy;              /* settings.dx, settings.dy */
   const double* density;      /* C = nx*ny*nz, densityVec (SolverCSD.jl:143) */
@@ -109,6 +109,10 @@ typedef struct titus_problem_v1 {
   double dz;
   const double* Ay;           /* nm x nm  pn.Ay (PNSystem.jl:142-194), nz > 1 only */
   const double* absAy;        /* nm x nm  |Ay| (Roe, as SolverCSD.jl:83-97), nz > 1 only */
+
+  /* version 3: the initial uncollided flux psi = SetupIC(obj, ...)
+   * (SolverCSD.jl:155-185, 916), nonzero for the "waterBeam" problem. */
+  const double* psi0;         /* C x nq (NULL: zeros) */
 } titus_problem_v1;
 
 /* SolverCSD(settings): validate, copy operands to device `device`. */
@@ -200,6 +204,70 @@ int titus_adjoint_batch(struct titus_ctx** ctxs, int32_t n, int32_t nthreads,
                         const double* const* dose_bars, double* const* grads, int32_t* status);
 
 void titus_destroy(struct titus_ctx* ctx);
+
+/* ---- full-rank P_N solver ---------------------------------------------------
+ * SolveFirstCollisionSource(obj), code/SolverCSD.jl:465-586: the dense C x nm
+ * moment matrix u advanced by explicit-Euler streaming and an implicit
+ * per-cell scattering solve, beside the same uncollided sweep.  It differs
+ * from the DLR path where the reference does: no 1/rho in the uncollided flux
+ * (:551), the beam source at energy[n] (:530-537), the in-scattering source
+ * inside the scattering solve (:564), and the dose
+ * dE (u[:,1] + uOUnc) SMid[n-1] / rho / (1 + [n == 2 or n == nEnergies]) (:575).
+ * FP64 and 2D.  Square grids only: the scattering loop numbers cells
+ * (i-1)*nx + j (:563), the cell numbering only when nx == ny.
+ * Conventions (status codes, titus_last_error, one stream per context) as
+ * above. */
+typedef struct titus_fcs titus_fcs;
+
+typedef struct titus_fcs_problem_v1 {
+  int32_t version;            /* TITUS_ABI_VERSION */
+  int32_t nx, ny;             /* NCellsX == NCellsY */
+  double dx, dy;
+  const double* density;      /* C = nx*ny: stencil 1/rho and the dose */
+  int32_t nm;
+  const int32_t* degree;      /* nm */
+  const double* Ax;           /* nm x nm  pn.Ax */
+  const double* Az;           /* nm x nm  pn.Az */
+  const double* absAx;        /* nm x nm  AbsAx */
+  const double* absAz;        /* nm x nm  AbsAz */
+  int32_t nq;
+  const double* omega;        /* nq x 3   qReduced (:489) */
+  const double* M;            /* nm x nq  MReduced (:490) */
+  int32_t ne;                 /* energy grid points; steps 1..ne-1 */
+  const double* dE;           /* ne-1: eTrafo[n] - eTrafo[n-1] of step n-1 (:527) */
+  const double* stopping_mid; /* ne-1: csd.SMid[n-1] of step n-1 (:575) */
+  int32_t nsigma;
+  const double* sigma;        /* ne x nsigma, row-major: sigma_l(eGrid[n]) (:524) */
+  int32_t nb;
+  const int32_t* bcells;      /* the four edges of the grid (:528-537) */
+  const double* bc_base;      /* nb x nq: PsiBeam without its energy factor */
+  const double* bc_scale;     /* ne-1: energy factor of PsiBeam(energy[n], n-1), step n-1 */
+  const double* psi0;         /* C x nq initial flux, SetupIC (:484); NULL: zeros */
+} titus_fcs_problem_v1;
+
+/* SolverCSD(settings) for the full solver: validate, build the block-sparse
+ * operator plan, copy operands to device `device`. */
+int titus_fcs_create(titus_fcs** out, const titus_fcs_problem_v1* problem, int32_t device);
+/* One iteration of `for n=2:nEnergies` (:522-582); `step` (= n-1) must be the next. */
+int titus_fcs_step(titus_fcs* ctx, int32_t step);
+/* Remaining steps up to and including last_step (<= 0: to the end of the grid). */
+int titus_fcs_solve(titus_fcs* ctx, int32_t last_step);
+/* State (NULL pointers skipped): u C x nm (unscaled; the reference returns
+ * 0.5*sqrt(gamma[1])*u), dose C, psi C x nq. */
+int titus_fcs_get_state(titus_fcs* ctx, double* u, double* dose, double* psi);
+/* Kernel timing with HIP events (on != 0) and per-kernel totals, as
+ * titus_set_timing / titus_kernel_stats; names "bc", "sweep", "collide",
+ * "dose". */
+int titus_fcs_set_timing(titus_fcs* ctx, int32_t on);
+int titus_fcs_kernel_stats(const titus_fcs* ctx, const char* name, double* total_ms,
+                           int64_t* launches, double* bytes, double* flops);
+/* Operator plan of the collided kernel (no reference counterpart): values[0]
+ * column blocks per wavefront, [1] block groups, [2] nonzero 4 x 16 operator
+ * blocks, [3] dense block count, [4] 1 when chosen by timing at creation,
+ * [5] 16-cell tiles per wavefront.  Writes at most n values. */
+int titus_fcs_get_plan(const titus_fcs* ctx, int32_t* values, int32_t n);
+void titus_fcs_destroy(titus_fcs* ctx);
+
 const char* titus_last_error(void);
 int32_t titus_abi_version(void);
 

@@ -133,7 +133,7 @@ def forward(P, steps=None):
     W = np.array(P["W0"], dtype=np.float64)
     S = np.array(P["S0"], dtype=np.float64)
     C, Q = P["C"], P["Q"]
-    psi = np.zeros((C, Q))
+    psi = dlra.initial_psi(P)
     dose = np.zeros(C)
     rho = ops.rho
     dE = P["dE"]

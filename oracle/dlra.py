@@ -316,6 +316,15 @@ class _Ops:
         return S - dE * acc
 
 
+def initial_psi(P):
+    """psi = SetupIC(obj, ...) (``SolverCSD.jl:155-185, 916``): the problem's
+    ``psi0`` (C x Q) when it has one, else zeros."""
+    psi0 = P.get("psi0")
+    if psi0 is None:
+        return np.zeros((P["C"], P["Q"]))
+    return np.array(psi0, dtype=np.float64)
+
+
 def _sweep(ops, psi, sig0, dE, omega):
     """Uncollided update with the reference's boundary restore (Appendix quirk Q2)."""
     if ops.nd == 3:
@@ -345,7 +354,7 @@ def solve_fixed(P, steps=None):
     W = np.array(P["W0"], dtype=np.float64)
     S = np.array(P["S0"], dtype=np.float64)
     C, Q = P["C"], P["Q"]
-    psi = np.zeros((C, Q))
+    psi = initial_psi(P)
     dose = np.zeros(C)
     rho = ops.rho
     dE = P["dE"]
@@ -420,7 +429,7 @@ def solve_bug(P, steps=None):
     W = np.array(P["W0"], dtype=np.float64)
     S = np.array(P["S0"], dtype=np.float64)
     C, Q = P["C"], P["Q"]
-    psi = np.zeros((C, Q))
+    psi = initial_psi(P)
     dose = np.zeros(C)
     rho = ops.rho
     dE = P["dE"]
@@ -525,7 +534,7 @@ def solve_parallel(P, steps=None):
     W = np.array(P["W0"], dtype=np.float64)
     S = np.array(P["S0"], dtype=np.float64)
     C, Q = P["C"], P["Q"]
-    psi = np.zeros((C, Q))
+    psi = initial_psi(P)
     dose = np.zeros(C)
     rho = ops.rho
     dE = P["dE"]
@@ -598,7 +607,7 @@ def solve_dense(P, steps=None, stream=True):
     ops = _Ops(P)
     C, Q, Nm = P["C"], P["Q"], P["Nm"]
     u = np.zeros((C, Nm))
-    psi = np.zeros((C, Q))
+    psi = initial_psi(P)
     dose = np.zeros(C)
     rho, dE, Sst, omega, M = ops.rho, P["dE"], P["Sst"], P["omega"], ops.M
     nE = P["nE"] if steps is None else min(P["nE"], steps + 1)

@@ -41,3 +41,25 @@ def slab_beam():
 def phantom_bug(n=200, N=21, cap=40, seed=0):
     """The 200x200, P_21 heterogeneous phantom (rank-adaptive)."""
     return build_problem(heterogeneous_phantom(n=n, N=N, seed=seed, r=cap), "subset", INTEGRATOR_BUG)
+
+
+def small_fcs(n=24, N=5, steps=None, angular=2.0, density=None, moments="subset", problem="waterBeamElectrons"):
+    """The small_fixed beam for the full-rank P_N solver (square grid)."""
+    from titus_amd.problem import build_fcs_problem
+    s = Settings(n + 1, n + 1, 2, problem, "Electrons", a=0.0, b=3.0, c=0.0, d=3.0,
+                 x0=1.5, y0=0.0, nPN=N, eMin=0.5, sigmaX=0.4, sigmaY=0.4, eMax=12.5,
+                 sigmaO1Inv=angular, sigmaO3Inv=angular,
+                 nEnergies=None if steps is None else steps + 1)
+    if density is not None:
+        s.set_density(density)
+    return build_fcs_problem(s, moments)
+
+
+def phantom_fcs(n=48, N=9, seed=3, steps=None, angular=2.0):
+    """Heterogeneous ellipse phantom for the full-rank P_N solver."""
+    from titus_amd.problem import build_fcs_problem
+    s = heterogeneous_phantom(n=n, N=N, seed=seed, r=2, eKin=8.0)
+    s.sigmaO1Inv = s.sigmaO3Inv = angular
+    if steps is not None:
+        s.nEnergies = steps + 1
+    return build_fcs_problem(s, "subset")

@@ -11,7 +11,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtitus_hip.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 INTEGRATOR_FIXED, INTEGRATOR_BUG, INTEGRATOR_PARALLEL = 0, 1, 2
 PRECISION_FP64, PRECISION_MIXED, PRECISION_MIXED_FULL_FP32 = 0, 1, 2
 
@@ -28,7 +28,9 @@ EXPORTED = ("titus_create", "titus_step", "titus_solve", "titus_solve_batch", "t
             "titus_get_state",
             "titus_get_history", "titus_set_timing", "titus_kernel_stats", "titus_get_tuning",
             "titus_get_decisions", "titus_set_checkpoints", "titus_adjoint", "titus_adjoint_batch",
-            "titus_destroy", "titus_last_error", "titus_abi_version")
+            "titus_destroy", "titus_fcs_create", "titus_fcs_step", "titus_fcs_solve",
+            "titus_fcs_get_state", "titus_fcs_set_timing", "titus_fcs_kernel_stats",
+            "titus_fcs_get_plan", "titus_fcs_destroy", "titus_last_error", "titus_abi_version")
 
 
 class TitusError(RuntimeError):
@@ -55,6 +57,22 @@ class ProblemV1(ctypes.Structure):
         ("theta_index", ctypes.c_double), ("rank_tol", ctypes.c_double),
         ("X0", _P), ("S0", _P), ("W0", _P),
         ("nz", ctypes.c_int32), ("dz", ctypes.c_double), ("Ay", _P), ("absAy", _P),
+        ("psi0", _P),
+    ]
+
+
+class FcsProblemV1(ctypes.Structure):
+    """``titus_fcs_problem_v1``."""
+    _fields_ = [
+        ("version", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("density", _P),
+        ("nm", ctypes.c_int32), ("degree", _PI), ("Ax", _P), ("Az", _P), ("absAx", _P),
+        ("absAz", _P),
+        ("nq", ctypes.c_int32), ("omega", _P), ("M", _P),
+        ("ne", ctypes.c_int32), ("dE", _P), ("stopping_mid", _P),
+        ("nsigma", ctypes.c_int32), ("sigma", _P),
+        ("nb", ctypes.c_int32), ("bcells", _PI), ("bc_base", _P), ("bc_scale", _P),
+        ("psi0", _P),
     ]
 
 
@@ -89,13 +107,25 @@ def load():
                                         ctypes.POINTER(_P), ctypes.POINTER(_P), _PI]
     lib.titus_destroy.argtypes = [vp]
     lib.titus_destroy.restype = None
+    lib.titus_fcs_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(FcsProblemV1), ctypes.c_int32]
+    lib.titus_fcs_step.argtypes = [vp, ctypes.c_int32]
+    lib.titus_fcs_solve.argtypes = [vp, ctypes.c_int32]
+    lib.titus_fcs_get_state.argtypes = [vp, _P, _P, _P]
+    lib.titus_fcs_set_timing.argtypes = [vp, ctypes.c_int32]
+    lib.titus_fcs_kernel_stats.argtypes = [vp, ctypes.c_char_p, _P, ctypes.POINTER(ctypes.c_int64),
+                                           _P, _P]
+    lib.titus_fcs_get_plan.argtypes = [vp, _PI, ctypes.c_int32]
+    lib.titus_fcs_destroy.argtypes = [vp]
+    lib.titus_fcs_destroy.restype = None
     lib.titus_last_error.restype = ctypes.c_char_p
     lib.titus_abi_version.restype = ctypes.c_int32
     for name in ("titus_create", "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
                  "titus_get_state", "titus_get_history", "titus_set_timing",
                  "titus_kernel_stats", "titus_get_tuning", "titus_get_decisions",
                  "titus_set_checkpoints",
-                 "titus_adjoint", "titus_adjoint_batch"):
+                 "titus_adjoint", "titus_adjoint_batch", "titus_fcs_create", "titus_fcs_step",
+                 "titus_fcs_solve", "titus_fcs_get_state", "titus_fcs_set_timing",
+                 "titus_fcs_kernel_stats", "titus_fcs_get_plan"):
         getattr(lib, name).restype = ctypes.c_int
     if lib.titus_abi_version() != ABI_VERSION:
         raise RuntimeError("libtitus_hip.so ABI version mismatch")
@@ -157,6 +187,40 @@ def make_problem(P, integrator, precision=PRECISION_FP64, rank_tol=1e-7, initial
         for name in ("X0", "S0", "W0"):
             keep[name] = f64(initial[name])
             setattr(s, name, _dp(keep[name]))
+    if P.get("psi0") is not None:
+        keep["psi0"] = f64(P["psi0"])
+        s.psi0 = _dp(keep["psi0"])
+    return s, keep
+
+
+def make_fcs_problem(P):
+    """Fill a ``FcsProblemV1`` from ``titus_amd.problem.build_fcs_problem``
+    output.  Returns (struct, keepalive)."""
+    f64 = lambda a: np.asfortranarray(np.asarray(a, dtype=np.float64))
+    i32 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+    keep = {
+        "density": f64(P["density"]), "degree": i32(P["degree"]),
+        "Ax": f64(P["Ax"]), "Az": f64(P["Az"]), "absAx": f64(P["absAx"]), "absAz": f64(P["absAz"]),
+        "omega": f64(P["omega"]), "M": f64(P["M"]), "dE": f64(P["dE_steps"]),
+        "smid": f64(P["SMid"]), "sigma": np.ascontiguousarray(P["sigma"], dtype=np.float64),
+        "bcells": i32(P["bcells"]), "bc_base": f64(P["bc_base"]),
+        "bc_scale": f64(P["fcs_bc_scale"]),
+    }
+    s = FcsProblemV1()
+    s.version = ABI_VERSION
+    s.nx, s.ny, s.dx, s.dy = P["nx"], P["ny"], P["dx"], P["dy"]
+    s.density = _dp(keep["density"])
+    s.nm, s.degree = P["Nm"], _ip(keep["degree"])
+    s.Ax, s.Az = _dp(keep["Ax"]), _dp(keep["Az"])
+    s.absAx, s.absAz = _dp(keep["absAx"]), _dp(keep["absAz"])
+    s.nq, s.omega, s.M = P["Q"], _dp(keep["omega"]), _dp(keep["M"])
+    s.ne, s.dE, s.stopping_mid = P["nE"], _dp(keep["dE"]), _dp(keep["smid"])
+    s.nsigma, s.sigma = keep["sigma"].shape[1], _dp(keep["sigma"])
+    s.nb, s.bcells = len(keep["bcells"]), _ip(keep["bcells"])
+    s.bc_base, s.bc_scale = _dp(keep["bc_base"]), _dp(keep["bc_scale"])
+    if P.get("psi0") is not None:
+        keep["psi0"] = f64(P["psi0"])
+        s.psi0 = _dp(keep["psi0"])
     return s, keep
 
 
@@ -276,6 +340,65 @@ class Context:
     def close(self):
         if getattr(self, "h", None):
             self._lib.titus_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FcsContext:
+    """Owner of one ``titus_fcs`` (the full-rank P_N solver on the device)."""
+
+    KERNELS = ("bc", "sweep", "collide", "dose")
+
+    def __init__(self, P, device=0):
+        lib = load()
+        self._lib = lib
+        self.C, self.nm, self.nq, self.ne = P["C"], P["Nm"], P["Q"], P["nE"]
+        st, keep = make_fcs_problem(P)
+        h = ctypes.c_void_p()
+        check(lib.titus_fcs_create(ctypes.byref(h), ctypes.byref(st), device))
+        del keep
+        self.h = h
+
+    def step(self, s):
+        check(self._lib.titus_fcs_step(self.h, s))
+
+    def solve(self, last_step=0):
+        check(self._lib.titus_fcs_solve(self.h, last_step))
+
+    def state(self):
+        """dict(u C x nm (unscaled), dose C, psi C x nq)."""
+        u = np.zeros((self.C, self.nm), order="F")
+        dose = np.zeros(self.C)
+        psi = np.zeros((self.C, self.nq), order="F")
+        check(self._lib.titus_fcs_get_state(self.h, _dp(u), _dp(dose), _dp(psi)))
+        return dict(u=u, dose=dose, psi=psi)
+
+    def set_timing(self, on=True):
+        check(self._lib.titus_fcs_set_timing(self.h, int(bool(on))))
+
+    def kernel_stats(self, name):
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        by, fl = ctypes.c_double(), ctypes.c_double()
+        check(self._lib.titus_fcs_kernel_stats(self.h, name.encode(), ctypes.byref(ms),
+                                               ctypes.byref(n), ctypes.byref(by), ctypes.byref(fl)))
+        return ms.value, n.value, by.value, fl.value
+
+    def plan(self):
+        """Operator plan of the collided kernel."""
+        v = np.zeros(6, dtype=np.int32)
+        check(self._lib.titus_fcs_get_plan(self.h, _ip(v), 6))
+        return {"column_blocks_per_wave": int(v[0]), "groups": int(v[1]),
+                "nonzero_blocks": int(v[2]), "dense_blocks": int(v[3]), "tuned": bool(v[4]),
+                "cell_tiles_per_wave": int(v[5])}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.titus_fcs_destroy(self.h)
             self.h = None
 
     def __del__(self):

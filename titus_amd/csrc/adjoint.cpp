@@ -53,7 +53,7 @@ struct AdjointStore {
   double *psib = nullptr, *psib2 = nullptr;                 // C x nq
   double *ybar = nullptr, *Db = nullptr, *gbuf = nullptr;   // C
   // forward state saved across the recomputation
-  double *Xsave = nullptr, *psisave = nullptr, *dosesave = nullptr;
+  double *Xsave = nullptr, *psisave = nullptr, *dosesave = nullptr, *u0save = nullptr;
 
   double* alloc(size_t n) {
     double* p = dalloc<double>(n);
@@ -368,6 +368,7 @@ void run_adjoint(titus_ctx& x, const double* dose_bar, double* grad) {
     a.psib = a.alloc(cq); a.psib2 = a.alloc(cq);
     a.ybar = a.alloc(C); a.Db = a.alloc(C); a.gbuf = a.alloc(C);
     a.Xsave = a.alloc(cr); a.psisave = a.alloc(cq); a.dosesave = a.alloc(C);
+    a.u0save = a.alloc((size_t)dev::kMaxSweepGroups * C);
     for (const titus::SpMat& A : x.sA) {
       HMat D(A.m, A.n);
       for (int i = 0; i < A.m; ++i)
@@ -391,6 +392,8 @@ void run_adjoint(titus_ctx& x, const double* dose_bar, double* grad) {
   HIPCHK(hipMemcpyAsync(a.Xsave, x.X, sizeof(double) * cr, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(a.psisave, x.psi, sizeof(double) * cq, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(a.dosesave, x.dose, sizeof(double) * C, hipMemcpyDeviceToDevice, st));
+  const size_t cu = (size_t)x.u0_groups * C;
+  HIPCHK(hipMemcpyAsync(a.u0save, x.u0p, sizeof(double) * cu, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(a.Db, dose_bar, sizeof(double) * C, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(a.Xb, 0, sizeof(double) * cr, st));
   HIPCHK(hipMemsetAsync(a.psib, 0, sizeof(double) * cq, st));
@@ -437,6 +440,7 @@ void run_adjoint(titus_ctx& x, const double* dose_bar, double* grad) {
   HIPCHK(hipMemcpyAsync(x.X, a.Xsave, sizeof(double) * cr, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(x.psi, a.psisave, sizeof(double) * cq, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(x.dose, a.dosesave, sizeof(double) * C, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(x.u0p, a.u0save, sizeof(double) * cu, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipStreamSynchronize(st));
   x.S = S_end;
   x.W = W_end;

@@ -139,5 +139,32 @@ void launch_sweep_adj(hipStream_t st, const Grid& g, const double* G, const doub
                       const double* omega, double dE, double denom, const unsigned char* restore,
                       double* out, double* ybar);
 
+// ---- full-rank P_N solve (SolveFirstCollisionSource, SolverCSD.jl:465-586) -----
+// Block-sparse moment operators of the collided streaming sum.  The four
+// operators |Ax|, |Az|, Ax, Az (o = 0..3, paired with the stencils L1x, L1y,
+// L2x, L2y) are cut into 4 x 16 blocks: rows k = 4 kb .. 4 kb + 3, columns
+// m = 16 mb .. 16 mb + 15.  Column blocks are dealt to `ngroups` groups of
+// `nbw` (mb = g nbw + t).  mask[(g nkb + kb) 4 + o] bit t marks block
+// (o, kb, g nbw + t) nonzero; the nonzero blocks of (g, kb) are stored from
+// block index start[g (nkb + 1) + kb] on, ordered by o, then t, 64 values each
+// (value l = operator[4 kb + l / 16, 16 mb + l % 16]).
+struct PnBlocks {
+  int nm = 0, nkb = 0, nnb = 0, nbw = 0, ngroups = 0;
+  int ct = 1;                 // 16-cell tiles per wavefront (1, or 2 when nbw <= 8)
+  const unsigned* mask = nullptr;
+  const int* start = nullptr;
+  const double* pk = nullptr;
+};
+constexpr int kPnMaxNbw = 16;
+
+// un[c, m] = (u[c, m] - dE (sum_o (L_o u)[c, :] A_o[:, m] - Dvec[m] (psi[c, :] M^T)[m]))
+//            / (1 + dE (sigma_0 - Dvec[m]))                     on interior cells,
+// 0 on ghost cells (SolverCSD.jl:556-571), Dvec[m] = sig[degree[m]]; the
+// stencils act on u/rho (g.irho) as the reference's stencil matrices do
+// (stencils.jl:26-128).  u, un: C x nm; psi: C x nq; M: nm x nq.
+void launch_pn_collide(hipStream_t st, const Grid& g, const PnBlocks& b, const double* u, double* un,
+                       const double* psi, int nq, const double* M, const int* degree,
+                       const double* sig, double dE);
+
 }  // namespace dev
 }  // namespace titus

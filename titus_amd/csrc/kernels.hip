@@ -1097,5 +1097,212 @@ void launch_sweep_adj(hipStream_t st, const Grid& g, const double* G, const doub
   check_launch("sweep_adj");
 }
 
+// ---------------------------------------------------------------------------
+// Full-rank P_N collided update (SolverCSD.jl:556-571).  A wavefront owns CT
+// tiles of 16 cells (the MFMA columns) and NBW column blocks of 16 moments
+// (the MFMA rows); a workgroup is four wavefronts on 64 CT consecutive cells
+// of one block group.  For each K-block of 4 moments the stencils of u/rho
+// are formed in registers as the B operands -- lane l: cell l % 16 of each
+// tile, moment 4 kb + l / 16, from five loads (c, c -+ ny, c -+ 1) -- and the
+// nonzero operator blocks of that K-block, staged in LDS by the whole
+// workgroup, are the A operands; each A operand read serves the CT tiles.
+// Software pipeline over the K-blocks: while K-block kb runs on the matrix
+// cores, the operator blocks of kb+1 (loaded one K-block earlier) go to the
+// other LDS buffer and the loads of kb+2 (operator blocks, u values, masks)
+// are issued, so no wait falls on a load younger than one K-block; the u
+// values alternate between two register sets (even / odd K-blocks).  Loads
+// and LDS stores carry no conditions: the packed blocks are followed by
+// 4 NBW zero blocks (a K-block's full capacity can always be read), a ghost
+// or padding cell reads a valid cell whose values its zero 1/rho cancels, and
+// rows k >= nm meet zero operator rows.  The in-scattering source
+// -Dvec[m] M[m, q] psi[c, q] follows as a dense K-loop over the ordinates.
+// Epilogue: the implicit per-cell scattering solve, zero on ghost cells.
+template <int NBW, int CT>
+__global__ __launch_bounds__(kBlock) void k_pn_collide(
+    const double* __restrict__ u, double* __restrict__ un, int C, int nx, int ny, double idx_,
+    double idy_, const double* __restrict__ irho, int nm, int nkb, int ngroups,
+    const unsigned* __restrict__ mask,
+    const int* __restrict__ start, const double* __restrict__ pk, const double* __restrict__ psi,
+    int nq, const double* __restrict__ M, const int* __restrict__ degree,
+    const double* __restrict__ sig, double dE) {
+  using MF = Mfma<double>;
+  constexpr int kCap = 4 * NBW * 64;           // operator values of one K-block
+  __shared__ double sa[2][kCap];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int L = xcd_block(blockIdx.x, gridDim.x);
+  const int grp = L % ngroups, tile = L / ngroups;
+  const int col = lane & 15, kg = lane >> 4;
+  int c[CT], cu[CT];
+  bool in[CT];
+  double ir[CT][5];            // 1/rho of each cell and its four neighbours
+#pragma unroll
+  for (int s = 0; s < CT; ++s) {
+    c[s] = ((tile * 4 + wave) * CT + s) * 16 + col;
+    const int i = c[s] / ny, j = c[s] - (c[s] / ny) * ny;
+    in[s] = c[s] < C && i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
+    cu[s] = in[s] ? c[s] : ny + 1;
+#pragma unroll
+    for (int d = 0; d < 5; ++d) ir[s][d] = 0.0;
+    if (in[s]) {
+      ir[s][0] = irho[c[s]]; ir[s][1] = irho[c[s] - ny]; ir[s][2] = irho[c[s] + ny];
+      ir[s][3] = irho[c[s] - 1]; ir[s][4] = irho[c[s] + 1];
+    }
+  }
+  const unsigned* mk = mask + (size_t)grp * nkb * 4;
+  const int* st = start + (size_t)grp * (nkb + 1);
+  double pf[NBW];
+  double vA[CT][5], vB[CT][5];
+  uint4 mA, mB;
+  auto load_blocks = [&](int kb) {
+    if (kb >= nkb) return;
+    const double* src = pk + (size_t)st[kb] * 64 + tid;
+#pragma unroll
+    for (int e = 0; e < NBW; ++e) pf[e] = src[e * kBlock];
+  };
+  auto load_u = [&](int kb, double (*v)[5], uint4& m) {
+    if (kb >= nkb) return;
+    const double* uk = u + (size_t)min(kb * 4 + kg, nm - 1) * C;
+#pragma unroll
+    for (int s = 0; s < CT; ++s) {
+      const double* p = uk + cu[s];
+      v[s][0] = p[0];
+      v[s][1] = p[-ny];
+      v[s][2] = p[ny];
+      v[s][3] = p[-1];
+      v[s][4] = p[1];
+    }
+    m = *reinterpret_cast<const uint4*>(mk + kb * 4);
+  };
+  auto stage = [&](int kb, int buf) {
+    if (kb >= nkb) return;
+#pragma unroll
+    for (int e = 0; e < NBW; ++e) sa[buf][tid + e * kBlock] = pf[e];
+  };
+  f64x4 acc[CT][NBW];
+#pragma unroll
+  for (int s = 0; s < CT; ++s)
+#pragma unroll
+    for (int t = 0; t < NBW; ++t) acc[s][t] = f64x4{0.0, 0.0, 0.0, 0.0};
+  auto iter = [&](int kb, double (*v)[5], uint4& m) {
+    const int buf = kb & 1;
+    double b[CT][4];
+#pragma unroll
+    for (int s = 0; s < CT; ++s) {
+      double y[5];
+#pragma unroll
+      for (int d = 0; d < 5; ++d) y[d] = v[s][d] * ir[s][d];
+      b[s][0] = (y[0] - 0.5 * y[1] - 0.5 * y[2]) * idx_;     // L1x
+      b[s][1] = (y[0] - 0.5 * y[3] - 0.5 * y[4]) * idy_;     // L1y
+      b[s][2] = (y[2] - y[1]) * (0.5 * idx_);                // L2x
+      b[s][3] = (y[4] - y[3]) * (0.5 * idy_);                // L2y
+    }
+    const uint4 mcur = m;
+    stage(kb + 1, buf ^ 1);
+    load_blocks(kb + 2);
+    load_u(kb + 2, v, m);
+    // the K-block's operator blocks sit in LDS in the order of their mask
+    // bits; each LDS read runs two blocks ahead of the matrix-core steps that
+    // consume it (indices clamped to the buffer, surplus reads unused)
+    const double* a = sa[buf] + lane;
+    int j = 0;
+    double a0 = a[0], a1 = a[64];
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+      const unsigned bits = o == 0 ? mcur.x : o == 1 ? mcur.y : o == 2 ? mcur.z : mcur.w;
+#pragma unroll
+      for (int t = 0; t < NBW; ++t)
+        if (bits & (1u << t)) {
+          const double cur = a0;
+          a0 = a1;
+          ++j;
+          a1 = a[64 * min(j + 1, 4 * NBW - 1)];
+#pragma unroll
+          for (int s = 0; s < CT; ++s) acc[s][t] = MF::step(cur, b[s][o], acc[s][t]);
+        }
+    }
+    __syncthreads();
+  };
+  load_blocks(0);
+  load_u(0, vA, mA);
+  stage(0, 0);
+  load_blocks(1);
+  load_u(1, vB, mB);
+  __syncthreads();
+  for (int kb = 0; kb < nkb; kb += 2) {
+    iter(kb, vA, mA);
+    if (kb + 1 < nkb) iter(kb + 1, vB, mB);
+  }
+  // in-scattering source
+  for (int q0 = 0; q0 < nq; q0 += 4) {
+    const int q = q0 + kg;
+    double bq[CT];
+#pragma unroll
+    for (int s = 0; s < CT; ++s) bq[s] = (in[s] && q < nq) ? psi[(size_t)q * C + c[s]] : 0.0;
+#pragma unroll
+    for (int t = 0; t < NBW; ++t) {
+      if ((grp * NBW + t) * 16 >= nm) continue;
+      const int m = (grp * NBW + t) * 16 + col;
+      const double aq = (m < nm && q < nq) ? -sig[degree[m]] * M[(size_t)q * nm + m] : 0.0;
+#pragma unroll
+      for (int s = 0; s < CT; ++s) acc[s][t] = MF::step(aq, bq[s], acc[s][t]);
+    }
+  }
+  const double s0 = sig[0];
+#pragma unroll
+  for (int s = 0; s < CT; ++s) {
+    if (c[s] >= C) continue;
+#pragma unroll
+    for (int t = 0; t < NBW; ++t)
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        const int m = (grp * NBW + t) * 16 + MF::row(kg, w);
+        if (m >= nm) continue;
+        double out = 0.0;
+        if (in[s]) {
+          const double d = sig[degree[m]];
+          out = (u[(size_t)m * C + c[s]] - dE * acc[s][t][w]) / (1.0 + dE * (s0 - d));
+        }
+        un[(size_t)m * C + c[s]] = out;
+      }
+  }
+}
+
+template <int NBW, int CT>
+void pn_collide_launch(hipStream_t st, const Grid& g, const PnBlocks& b, const double* u, double* un,
+                       const double* psi, int nq, const double* M, const int* degree,
+                       const double* sig, double dE) {
+  const int cells = 64 * CT;
+  const unsigned blocks = (unsigned)(((g.C + cells - 1) / cells) * b.ngroups);
+  hipLaunchKernelGGL((k_pn_collide<NBW, CT>), dim3(blocks), dim3(kBlock), 0, st, u, un, g.C, g.nx,
+                     g.ny, 1.0 / g.dx, 1.0 / g.dy, g.irho, b.nm, b.nkb, b.ngroups, b.mask, b.start,
+                     b.pk, psi, nq, M, degree, sig, dE);
+}
+
+template <int NBW>
+void pn_collide_nbw(hipStream_t st, const Grid& g, const PnBlocks& b, const double* u, double* un,
+                    const double* psi, int nq, const double* M, const int* degree,
+                    const double* sig, double dE) {
+  if (b.ct == 2 && NBW <= 8) pn_collide_launch<NBW, (NBW <= 8 ? 2 : 1)>(st, g, b, u, un, psi, nq, M, degree, sig, dE);
+  else if (b.ct == 1) pn_collide_launch<NBW, 1>(st, g, b, u, un, psi, nq, M, degree, sig, dE);
+  else throw std::runtime_error("pn_collide: cell tiles per wavefront must be 1, or 2 with at most 8 column blocks");
+}
+
+void launch_pn_collide(hipStream_t st, const Grid& g, const PnBlocks& b, const double* u, double* un,
+                       const double* psi, int nq, const double* M, const int* degree,
+                       const double* sig, double dE) {
+  if (g.ndim() != 2) throw std::runtime_error("pn_collide: 2D grids only");
+  if (b.ngroups * b.nbw < b.nnb || b.nkb * 4 < b.nm)
+    throw std::runtime_error("pn_collide: block plan does not cover the moments");
+  switch (b.nbw) {
+    case 1: pn_collide_nbw<1>(st, g, b, u, un, psi, nq, M, degree, sig, dE); break;
+    case 2: pn_collide_nbw<2>(st, g, b, u, un, psi, nq, M, degree, sig, dE); break;
+    case 4: pn_collide_nbw<4>(st, g, b, u, un, psi, nq, M, degree, sig, dE); break;
+    case 8: pn_collide_nbw<8>(st, g, b, u, un, psi, nq, M, degree, sig, dE); break;
+    case 16: pn_collide_nbw<16>(st, g, b, u, un, psi, nq, M, degree, sig, dE); break;
+    default: throw std::runtime_error("pn_collide: column blocks per wavefront must be 1, 2, 4, 8 or 16");
+  }
+  check_launch("pn_collide");
+}
+
 }  // namespace dev
 }  // namespace titus

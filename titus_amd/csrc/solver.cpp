@@ -67,7 +67,7 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
   return guarded([&] {
     require(out != nullptr && p != nullptr, "null argument");
     *out = nullptr;
-    require(p->version == 1 || p->version == TITUS_ABI_VERSION, "problem struct version mismatch");
+    require(p->version >= 1 && p->version <= TITUS_ABI_VERSION, "problem struct version mismatch");
     const int nz = p->version >= 2 ? p->nz : 1;
     require(p->nx >= 3 && p->ny >= 3 && (nz == 1 || nz >= 3), "need at least 3 cells per direction");
     if (nz > 1) require(p->dz > 0 && p->Ay && p->absAy, "3D grid needs dz, Ay and |Ay|");
@@ -205,6 +205,25 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     HIPCHK(hipMemset(x.u0p, 0, sizeof(double) * dev::kMaxSweepGroups * C));
     HIPCHK(hipMemset(x.psi2, 0, psz * (size_t)C * p->nq));
     HIPCHK(hipMemset(x.X, 0, sizeof(double) * cx));
+    if (p->version >= 3 && p->psi0) {
+      // initial flux (SetupIC) in the storage type, and its l = 0 moment as
+      // partial 0 of the moment sums the first dose half reads
+      std::vector<double> u0(C, 0.0);
+      if (x.mixed) {
+        std::vector<float> f((size_t)C * p->nq);
+        for (size_t e = 0; e < f.size(); ++e) f[e] = (float)p->psi0[e];
+        for (int q = 0; q < p->nq; ++q)
+          for (int c = 0; c < C; ++c) u0[c] += (double)f[(size_t)q * C + c] * m0[q];
+        HIPCHK(hipMemcpy(x.psi, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
+      } else {
+        for (int q = 0; q < p->nq; ++q)
+          for (int c = 0; c < C; ++c) u0[c] += p->psi0[(size_t)q * C + c] * m0[q];
+        HIPCHK(hipMemcpy(x.psi, p->psi0, sizeof(double) * (size_t)C * p->nq, hipMemcpyHostToDevice));
+      }
+      for (size_t e = 0; e < (size_t)C * p->nq; ++e)
+        require(std::isfinite(p->psi0[e]), "initial flux must be finite");
+      HIPCHK(hipMemcpy(x.u0p, u0.data(), sizeof(double) * C, hipMemcpyHostToDevice));
+    }
 
     // initial factors (SolverCSD.jl:932-957): svd!(zeros) -> identity columns,
     // ghost rows of X zeroed

@@ -414,3 +414,28 @@ def beam_energy_factor(s, E, S_n):
     if s.beamModel == "normpdf":
         return S_n * 1.0
     return np.exp(-s.sigmaEInv * (s.eMax - E) ** 2) * S_n * s.densityMin
+
+
+def setup_ic(s, omega):
+    """Initial uncollided flux of ``SetupIC`` (``SolverCSD.jl:155-185``) on the
+    reduced ordinates ``omega`` (Q x 3): C x Q (Fortran order), or None when it
+    is zero.
+
+    "waterBeam" starts with the beam already inside the slab: cells i, j in
+    3..n-2 (1-based) hold 1e5 exp(-1e4 (Omega1 - w_x)^2) exp(-1e4 (Omega3 -
+    w_z)^2) normpdf(x, x0, 0.1) normpdf(y, y0, 0.1) (the angular widths and
+    the 0.1 cm spot are fixed there, not taken from the settings).  The other
+    beam problems start from zero (``IC``, ``settings.jl:341-365``, is zero for
+    them).
+    """
+    if s.problem != "waterBeam":
+        return None
+    nx, ny = s.NCellsX, s.NCellsY
+    ang = (1e5 * np.exp(-10000.0 * (s.Omega1 - omega[:, 0]) ** 2)
+           * np.exp(-10000.0 * (s.Omega3 - omega[:, 2]) ** 2))
+    psi = np.zeros((nx, ny, len(omega)))
+    i = np.arange(2, nx - 2)
+    j = np.arange(2, ny - 2)
+    space = normpdf(s.xMid[i], s.x0, 0.1)[:, None] * normpdf(s.yMid[j], s.y0, 0.1)[None, :]
+    psi[2:nx - 2, 2:ny - 2, :] = space[:, :, None] * ang[None, None, :]
+    return np.asfortranarray(psi.reshape(nx * ny, len(omega)))

@@ -40,7 +40,15 @@ def angular_operands(N, moments, cs_phase):
     return lk, degree, Ax, Az, absAx, absAz, O, M, pts, w
 
 
+# problems whose beam enters through the boundary source (PsiBeam); the
+# line-source family is an initial-value problem with PsiBeam = 0 and its own
+# ordinate selection (SolverCSD.jl:907-911), not provided here
+BEAM_PROBLEMS = ("waterBeam", "waterBeamElectrons", "lung", "liver", "slab", "phantom")
+
+
 def build_problem(s, moments="subset", integrator=INTEGRATOR_FIXED):
+    if s.problem not in BEAM_PROBLEMS:
+        raise ValueError(f"problem {s.problem!r}: only the beam problems {BEAM_PROBLEMS} are provided")
     nx, ny = s.NCellsX, s.NCellsY
     C = nx * ny
     N = s.nPN
@@ -82,6 +90,7 @@ def build_problem(s, moments="subset", integrator=INTEGRATOR_FIXED):
     W0 = np.zeros((len(lk), r0), order="F")
     W0[np.arange(r0), np.arange(r0)] = 1.0
     S0 = np.zeros((r0, r0), order="F")
+    psi0 = ph.setup_ic(s, omega)
 
     return dict(
         nx=nx, ny=ny, C=C, dx=float(s.dx), dy=float(s.dy),
@@ -92,11 +101,29 @@ def build_problem(s, moments="subset", integrator=INTEGRATOR_FIXED):
         Q=len(idx), Qfull=len(w), omega=np.asfortranarray(omega),
         M=np.asfortranarray(Mred), O=np.asfortranarray(O), ordinate_index=idx,
         nE=csd.nEnergies, dE=csd.dE, eGrid=csd.eGrid, Sst=csd.S,
+        eTrafo=csd.eTrafo, SMid=csd.SMid,
         sigma=np.ascontiguousarray(csd.sigma_table()),
         bcells=bcells.astype(np.int32), bc_base=bc_base, bc_scale=bc_scale,
         restore_cells=ph.boundary_restore_cells(nx, ny).astype(np.int32),
         interior=np.nonzero(interior)[0].astype(np.int32),
         integrator=integrator, r=r0, rank_cap=int(s.rank_cap),
         theta=float(s.theta), thetaIndex=float(s.thetaIndex), cEta=float(s.cEta),
-        X0=X0, W0=W0, S0=S0,
+        X0=X0, W0=W0, S0=S0, psi0=psi0,
     )
+
+
+def build_fcs_problem(s, moments="subset"):
+    """Operands of ``SolveFirstCollisionSource(obj)`` (``SolverCSD.jl:465-586``):
+    those of ``build_problem`` plus the per-step energy width
+    ``eTrafo[n] - eTrafo[n-1]`` (``:527``), ``SMid[n-1]`` (``:575``) and the
+    beam's energy factor at ``energy[n]`` with ``S[n-1]`` (``PsiBeam(...,
+    energy[n], ..., n-1)``, ``:530-537``), each indexed by step - 1."""
+    if s.NCellsX != s.NCellsY:
+        raise ValueError("SolveFirstCollisionSource numbers cells (i-1)*nx + j (SolverCSD.jl:563): "
+                         "square grids only")
+    P = build_problem(s, moments, INTEGRATOR_FIXED)
+    nE = P["nE"]
+    P["dE_steps"] = np.diff(P["eTrafo"])
+    P["fcs_bc_scale"] = np.array([ph.beam_energy_factor(s, P["eGrid"][n], P["Sst"][n - 1])
+                                  for n in range(1, nE)])
+    return P

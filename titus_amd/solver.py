@@ -9,6 +9,8 @@ Reference interface (Julia, ``code/SolverCSD.jl``):
   returns the above plus ``[E; rank]``, ``[E eta]``, ``[E bound]`` before ``psi``
 * ``SolveFirstCollisionSourceDLRParallel(obj, recordGif=false)`` -- ``:1726-2008``
   returns the fixed-rank tuple with ``rVec`` (2 x nEnergies) before ``psi``
+* ``SolveFirstCollisionSource(obj)``                   -- ``:465-586``
+  returns ``(0.5*sqrt(gamma[1])*u, dose, psi)`` (the full-rank P_N solve)
 
 Like the reference, ``Solve*`` mutates ``obj``: ``obj.dose`` accumulates and
 ``obj.settings.dE`` is overwritten with the grid step.  The energy loop runs
@@ -20,7 +22,8 @@ import math
 import numpy as np
 
 from . import _lib
-from .problem import INTEGRATOR_BUG, INTEGRATOR_FIXED, INTEGRATOR_PARALLEL, build_problem
+from .problem import (INTEGRATOR_BUG, INTEGRATOR_FIXED, INTEGRATOR_PARALLEL, build_fcs_problem,
+                      build_problem)
 
 
 # Rank cut of the range rule.  FP64: 1e-7 of the scale.  Mixed form: FP32
@@ -149,3 +152,29 @@ def SolveFirstCollisionSourceDLRParallel(obj, recordGif=False):
     rvec[0, 1:done + 1] = np.arange(done) * P["dE"]
     rvec[1, 1:done + 1] = ranks[:done]
     return (*out[:5], rvec, out[5])
+
+
+def SolveFirstCollisionSource(obj):
+    """Full-rank P_N first-collision-source solve on the GPU
+    (``SolverCSD.jl:465-586``): the dense C x Nm moments u with explicit-Euler
+    streaming and the implicit per-cell scattering solve, FP64.
+
+    Returns ``(0.5*sqrt(gamma[1])*u, dose, psi)`` as the reference does, with
+    u C x Nm and psi nx x ny x Q; ``obj.dose`` accumulates the dose.  Square
+    grids only (the reference's scattering loop numbers cells (i-1)*nx + j).
+    """
+    if obj.precision != "fp64":
+        raise ValueError("SolveFirstCollisionSource runs in FP64 only")
+    if "fcs" not in obj._problem:
+        obj._problem["fcs"] = build_fcs_problem(obj.settings, obj.moments)
+    P = obj._problem["fcs"]
+    ctx = _lib.FcsContext(P, device=obj.device)
+    try:
+        ctx.solve(0)
+        st = ctx.state()
+    finally:
+        ctx.close()
+    obj.dose += st["dose"]
+    obj.settings.dE = P["dE"]
+    nx, ny = obj.settings.NCellsX, obj.settings.NCellsY
+    return 0.5 * math.sqrt(obj.gamma[0]) * st["u"], obj.dose, st["psi"].reshape(nx, ny, -1)
