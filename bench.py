@@ -48,6 +48,9 @@ PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 # k_tn_mfma<1, ...> dispatch plus its k_tn_reduce, averaged over all such
 # calls.  Re-measure when the kernels change.
 MEASURED_TRAFFIC = {"gram": 3.19e6}
+# HBM bytes per launch of the full-rank solver's collided kernel on the
+# phantom (FETCH_SIZE x 2 + WRITE_SIZE x 1, profiles/fcs_collide_phantom200_p21.md)
+MEASURED_TRAFFIC_FULL_PN = {"collide": 2.625e8}
 
 
 def build_workload(n=200, N=21, cap=40, seed=0):
@@ -269,6 +272,74 @@ def proton_cube_bench(sync, cpu_steps=2):
     return out
 
 
+def full_pn_bench(sync, cpu_steps=2, warmup=10, steps=200):
+    """The full-rank P_N solve (SolveFirstCollisionSource) on the headline
+    phantom: 200x200 cells, P_21 (253 moments, 24 ordinates), FP64.  The
+    context is created (and its block-sparse plan chosen) before the timed
+    region; `steps` energy steps after `warmup` are timed between stream
+    synchronisations, then the same steps of a second context with HIP events
+    around every kernel.  The collided kernel's roofline: algorithmic flops =
+    2 C (nnz(|Ax|) + nnz(|Az|) + nnz(Ax) + nnz(Az) + nm nq) per launch,
+    algorithmic bytes = 8 (2 C nm + C nq); the matrix-core work executed is
+    larger (whole 4 x 16 operator blocks) and reported beside it."""
+    from titus_amd import _lib
+    from titus_amd.problem import build_fcs_problem
+    from titus_amd.settings import heterogeneous_phantom
+    P = build_fcs_problem(heterogeneous_phantom(n=200, N=21, seed=0, r=2))
+    last = warmup + steps
+    out = {"workload": f"full-rank P_N, 2D phantom 200x200 (seed 0), P_21 ({P['Nm']} moments, "
+                       f"{P['Q']} ordinates), FP64, steps {warmup + 1}..{last} of {P['nE'] - 1}"}
+    ctx = _lib.FcsContext(P)
+    try:
+        ctx.solve(warmup)
+        sync()
+        t0 = time.perf_counter()
+        ctx.solve(last)
+        sync()
+        dt = time.perf_counter() - t0
+        out["energy_steps_per_s"] = steps / dt
+        out["ms_per_step"] = dt / steps * 1e3
+        out["plan"] = ctx.plan()
+    finally:
+        ctx.close()
+    ctx = _lib.FcsContext(P)
+    try:
+        ctx.solve(warmup)
+        ctx.set_timing(True)
+        ctx.solve(last)
+        sync()
+        k = {}
+        for name in ctx.KERNELS:
+            ms, n, by, fl = ctx.kernel_stats(name)
+            if n:
+                k[name] = {"us_per_launch": round(ms / n * 1e3, 2),
+                           "GB/s": round(by / (ms * 1e-3) / 1e9, 1),
+                           "TFLOP/s": round(fl / (ms * 1e-3) / 1e12, 2)}
+        out["kernels"] = k
+        ms, n, by, fl = ctx.kernel_stats("collide")
+        t_avg = ms / n * 1e-3
+        plan = out["plan"]
+        tiles = (P["C"] + 15) // 16
+        executed = 2048.0 * tiles * (plan["nonzero_blocks"]
+                                              + plan["groups"] * plan["column_blocks_per_wave"]
+                                              * ((P["Q"] + 3) // 4))
+        out["roofline_collide"] = {
+            "bound": "mfma", "achieved": fl / n / t_avg / 1e12, "peak": PEAK_TFLOPS["f64"],
+            "unit": "TFLOP/s", "frac": fl / n / t_avg / 1e12 / PEAK_TFLOPS["f64"],
+            "traffic": MEASURED_TRAFFIC_FULL_PN["collide"], "kernel": "collide",
+            "avg_launch_us": t_avg * 1e6,
+            "hbm_GB/s": by / n / t_avg / 1e9,
+            "executed_matrix_core_TFLOP/s": executed / t_avg / 1e12}
+    finally:
+        ctx.close()
+    if cpu_steps:
+        from oracle import fullpn
+        t0 = time.perf_counter()
+        fullpn.solve_fcs(P, steps=cpu_steps)
+        out["cpu_oracle_s_per_step"] = (time.perf_counter() - t0) / cpu_steps
+    return out
+
+
 def adjoint_bench(sync, nthreads, n_beamlets=64, every=16, fd_dirs=2, seed=0):
     """Gradient of the plan objective J = ||sum_b w_b D_b(rho0 o s) - D_t||^2
     over 64 fixed-rank beamlets (128x128 phantom, P_15, ranks 5-30): forward
@@ -427,6 +498,10 @@ def main():
                     help="only the plan-gradient (adjoint) measurement")
     ap.add_argument("--fd-dirs", type=int, default=1,
                     help="random directions of the plan-gradient difference check")
+    ap.add_argument("--no-full-pn", action="store_true",
+                    help="skip the full-rank P_N solver measurement")
+    ap.add_argument("--full-pn-only", action="store_true",
+                    help="only the full-rank P_N solver measurement")
     ap.add_argument("--stencil-only", action="store_true",
                     help="only the large-grid stencil measurement (+ calibration copy), for PMC runs")
     args = ap.parse_args()
@@ -447,6 +522,9 @@ def main():
         return
     if args.stencil_only:
         print(json.dumps({"stencil_scale": stencil_scale(sync, calibrate=True)}), flush=True)
+        return
+    if args.full_pn_only:
+        print(json.dumps({"full_pn": full_pn_bench(sync, 0 if args.no_cpu else 2)}), flush=True)
         return
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_bench(sync, args.threads, fd_dirs=args.fd_dirs)}), flush=True)
@@ -516,6 +594,8 @@ def main():
         out["proton_cube"] = proton_cube_bench(sync, 0 if args.no_cpu else 2)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_bench(sync, args.threads, fd_dirs=args.fd_dirs)
+    if not args.no_full_pn:
+        out["full_pn"] = full_pn_bench(sync, 0 if args.no_cpu else 2)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(P, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
