@@ -113,6 +113,11 @@ typedef struct titus_problem_v1 {
   /* version 3: the initial uncollided flux psi = SetupIC(obj, ...)
    * (SolverCSD.jl:155-185, 916), nonzero for the "waterBeam" problem. */
   const double* psi0;         /* C x nq (NULL: zeros) */
+  /* version 3: uncollided flux scheme.  1 (or 0): first-order upwind,
+   * solveFluxUpwind! (SolverCSD.jl:388-463); 2: the second-order limited
+   * flux solveFlux! (:295-386) of CSolveFirstCollisionSourceDLR
+   * (:1122-1360).  2D grids; the adjoint needs 1. */
+  int32_t flux_order;
 } titus_problem_v1;
 
 /* SolverCSD(settings): validate, copy operands to device `device`. */
@@ -184,7 +189,7 @@ int titus_get_decisions(const struct titus_ctx* ctx, uint64_t* hash, int64_t* co
  * (SolveFirstCollisionSourceDLR, code/SolverCSD.jl:891-1120): the dose after
  * the steps taken, as a function of the density (problem->density), with the
  * energy grid, beam source and P_N operators held fixed.  FP64, fixed-rank
- * integrator and 2D grids only (TITUS_E_ARG otherwise).
+ * integrator, upwind flux and 2D grids only (TITUS_E_ARG otherwise).
  *
  * titus_set_checkpoints: before the first step, keep the state (X, psi on the
  * device; S, W on the host) before steps 1, 1 + every, 1 + 2 every, ... of the

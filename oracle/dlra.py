@@ -13,6 +13,9 @@ It restates, in numpy with T = Float64 throughout:
                       (``code/SolverCSD.jl:1726-2008``)
 * ``truncate``     -- ``truncate!``                       (``code/SolverCSD.jl:3369-3399``)
 * ``upwind_flux``  -- ``solveFluxUpwind!``                (``code/SolverCSD.jl:388-463``)
+* ``limited_flux`` -- ``solveFlux!`` with ``slopefit``    (``code/SolverCSD.jl:278-386``),
+                      the uncollided flux of ``CSolveFirstCollisionSourceDLR``
+                      (``:1122-1360``), selected by ``P["flux_order"] = 2``
 * ``stencils``     -- ``Stencils`` applied matrix-free    (``code/stencils.jl:12-130``)
 * ``stencils3``, ``upwind_flux3`` -- their 3D extension (no reference: the
                       snapshot is 2D; pinned by dense assembly, axis-permutation
@@ -126,6 +129,54 @@ def upwind_flux(psi, rho, omega, nx, ny, dx, dy):
     fx = np.where(ox >= 0, dxp, dxm) * (ox / dx)
     fy = np.where(oz >= 0, dyp, dym) * (oz / dy)
     F[1:-1, 1:-1] = fx + fy
+    return F.reshape(nx * ny, Q)
+
+
+def _minmod(x, y):
+    """``minmod(x, y)`` of ``SolverCSD.jl:289``: y clamped to [x, 0] when
+    x < 0, to [0, x] otherwise."""
+    return np.where(x < 0, np.minimum(np.maximum(y, x), 0.0), np.minimum(np.maximum(y, 0.0), x))
+
+
+def slopefit(left, center, right):
+    """``slopefit`` (``SolverCSD.jl:291-294``)."""
+    tmp = _minmod(0.5 * (right - left), 2.0 * (center - left))
+    return _minmod(2.0 * (right - center), tmp)
+
+
+def limited_flux(psi, rho, omega, nx, ny, dx, dy):
+    """Second-order flux of psi/rho with the slope limiter, per ordinate
+    (``solveFlux!``, ``SolverCSD.jl:295-386``).
+
+    Cells i, j in 2..n-3 (0-based) only; the others stay 0.  For Omega >= 0
+    the faces are s + slopefit/2 of the upwind-side cells (s1..s4 = i-2..i+1),
+    for Omega < 0 s - slopefit/2 (s1..s4 = i-1..i+2); same along j.
+    """
+    Q = psi.shape[1]
+    V = (psi / rho[:, None]).reshape(nx, ny, Q)
+    F = np.zeros((nx, ny, Q))
+    ox, oz = omega[:, 0], omega[:, 2]
+    I, J = slice(2, nx - 2), slice(2, ny - 2)
+
+    def sx(d):
+        return V[2 + d:nx - 2 + d, J]
+
+    def sy(d):
+        return V[I, 2 + d:ny - 2 + d]
+
+    # x faces
+    e_p = sx(0) + 0.5 * slopefit(sx(-1), sx(0), sx(1))
+    w_p = sx(-1) + 0.5 * slopefit(sx(-2), sx(-1), sx(0))
+    e_m = sx(1) - 0.5 * slopefit(sx(0), sx(1), sx(2))
+    w_m = sx(0) - 0.5 * slopefit(sx(-1), sx(0), sx(1))
+    fx = np.where(ox >= 0, e_p - w_p, e_m - w_m) * (ox / dx)
+    # y faces
+    n_p = sy(0) + 0.5 * slopefit(sy(-1), sy(0), sy(1))
+    s_p = sy(-1) + 0.5 * slopefit(sy(-2), sy(-1), sy(0))
+    n_m = sy(1) - 0.5 * slopefit(sy(0), sy(1), sy(2))
+    s_m = sy(0) - 0.5 * slopefit(sy(-1), sy(0), sy(1))
+    fy = np.where(oz >= 0, n_p - s_p, n_m - s_m) * (oz / dy)
+    F[I, J] = fx + fy
     return F.reshape(nx * ny, Q)
 
 
@@ -273,6 +324,9 @@ class _Ops:
         self.M = np.asarray(P["M"])
         self.deg = np.asarray(P["degree"])
         self.Nm = P["Nm"]
+        # 1: solveFluxUpwind!; 2: the limited flux solveFlux! of
+        # CSolveFirstCollisionSourceDLR (SolverCSD.jl:1122-1360, :1240)
+        self.flux_order = int(P.get("flux_order", 1))
 
     def L(self, U):
         if self.nd == 3:
@@ -329,6 +383,8 @@ def _sweep(ops, psi, sig0, dE, omega):
     """Uncollided update with the reference's boundary restore (Appendix quirk Q2)."""
     if ops.nd == 3:
         flux = upwind_flux3(psi, ops.rho, omega, ops.nx, ops.ny, ops.nz, ops.dx, ops.dy, ops.dz)
+    elif ops.flux_order == 2:
+        flux = limited_flux(psi, ops.rho, omega, ops.nx, ops.ny, ops.dx, ops.dy)
     else:
         flux = upwind_flux(psi, ops.rho, omega, ops.nx, ops.ny, ops.dx, ops.dy)
     keep = psi[ops.restore, 0].copy()

@@ -57,7 +57,7 @@ class ProblemV1(ctypes.Structure):
         ("theta_index", ctypes.c_double), ("rank_tol", ctypes.c_double),
         ("X0", _P), ("S0", _P), ("W0", _P),
         ("nz", ctypes.c_int32), ("dz", ctypes.c_double), ("Ay", _P), ("absAy", _P),
-        ("psi0", _P),
+        ("psi0", _P), ("flux_order", ctypes.c_int32),
     ]
 
 
@@ -190,6 +190,7 @@ def make_problem(P, integrator, precision=PRECISION_FP64, rank_tol=1e-7, initial
     if P.get("psi0") is not None:
         keep["psi0"] = f64(P["psi0"])
         s.psi0 = _dp(keep["psi0"])
+    s.flux_order = int(P.get("flux_order", 1))
     return s, keep
 
 

@@ -464,6 +464,7 @@ int titus_set_checkpoints(titus_ctx* ctx, int32_t every) {
     if (ctx->precision != TITUS_PRECISION_FP64)
       throw TitusError(TITUS_E_ARG, "adjoint: FP64 precision only");
     if (ctx->nd != 2) throw TitusError(TITUS_E_ARG, "adjoint: 2D grids only");
+    if (ctx->flux_order != 1) throw TitusError(TITUS_E_ARG, "adjoint: upwind uncollided flux only");
     if (ctx->steps_done != 0) throw TitusError(TITUS_E_STATE, "checkpoints must be set before the first step");
     ctx->adj = std::make_shared<AdjointStore>();
     ctx->adj->every = every;

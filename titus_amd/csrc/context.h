@@ -204,6 +204,7 @@ struct titus_ctx {
   int nd = 2;                 // spatial dimensions (3: k axis paired with Ay)
   int nz = 1;
   double dz = 1.0;
+  int flux_order = 1;         // uncollided flux: 1 upwind, 2 limited (solveFlux!)
   int nops() const { return 2 * nd; }
   // term order of the streaming sums: central operators, then dissipative
   std::vector<int> term_order() const {

@@ -49,6 +49,7 @@ struct SweepPlan {
   int rows = 8;
   int groups = 1;
   bool nontemporal = false;   // psi streamed with non-temporal loads and stores
+  bool limited = false;       // second-order limited flux (2D; no non-temporal form)
 };
 SweepPlan sweep_default(const Grid& g, int nq, DT tpsi);
 std::vector<SweepPlan> sweep_candidates(const Grid& g, int nq, DT tpsi);
@@ -66,6 +67,8 @@ void launch_dose(hipStream_t st, const Grid& g, double* dose, const double* X, i
 // Uncollided upwind update with the boundary restore (SolverCSD.jl:388-463,
 // 1000-1005): pout = (pin - dE * flux(pin/rho)) / denom on interior cells,
 // pin / denom on ghost cells, pin itself for ordinate 0 of `restore` cells.
+// plan.limited: the second-order limited flux of solveFlux! (SolverCSD.jl:
+// 278-386) on cells 2..n-3 instead, pin / denom on the other cells.
 // Computes in the storage type of psi; writes the moment partials of pout.
 void launch_sweep(hipStream_t st, const Grid& g, const void* pin, void* pout, DT tpsi, int nq,
                   const double* omega, double dE, double denom, const unsigned char* restore,

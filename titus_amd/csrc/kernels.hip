@@ -162,6 +162,105 @@ __global__ __launch_bounds__(kBlock) void k_sweep(
   }
 }
 
+// slopefit / minmod of SolverCSD.jl:289-294
+template <class T>
+__device__ inline T minmod_ref(T x, T y) {
+  return x < T(0) ? fmin(fmax(y, x), T(0)) : fmin(fmax(y, T(0)), x);
+}
+template <class T>
+__device__ inline T slopefit_ref(T l, T c, T r) {
+  return minmod_ref(T(2) * (r - c), minmod_ref(T(0.5) * (r - l), T(2) * (c - l)));
+}
+
+// Uncollided update with the second-order limited flux (solveFlux!,
+// SolverCSD.jl:295-386).  A wavefront owns a strip of H rows (i) by
+// kSweepColsLim columns (j, lanes 2..61; lanes 0, 1, 62, 63 carry the
+// neighbouring columns) and walks the ordinates of its group; per ordinate it
+// loads the strip plus two halo rows on each side (the slope windows reach
+// two cells upwind and one downwind), the j-neighbours come from lanes +-1,
+// +-2.  The flux exists on cells i, j in 2..n-3; elsewhere the update is the
+// damping alone, with the ordinate-0 restore of the boundary cells.
+constexpr int kSweepColsLim = 60;
+template <class T, int H>
+__global__ __launch_bounds__(kBlock) void k_sweep_lim(
+    const T* __restrict__ pin, T* __restrict__ pout, int nx, int ny, int nq, int qper,
+    const double* __restrict__ omega, T dx, T dy, const double* __restrict__ rho, T dE, T denom,
+    const unsigned char* __restrict__ restore, const double* __restrict__ m0,
+    double* __restrict__ u0p, int ntj, int nsb) {
+  const int C = nx * ny;
+  const int L = xcd_block(blockIdx.x, gridDim.x);
+  const int jt = L % ntj, sb = (L / ntj) % nsb, gq = L / (ntj * nsb);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i0 = (sb * (kBlock / 64) + wave) * H;
+  const int j = jt * kSweepColsLim - 2 + lane;
+  const bool jv = j >= 0 && j < ny;
+  const bool owner = lane >= 2 && lane <= kSweepColsLim + 1 && jv;
+  const bool jin = j >= 2 && j <= ny - 3;
+  constexpr int R = H + 4;                    // rows i0-2 .. i0+H+1
+  T ir[R];
+#pragma unroll
+  for (int h = 0; h < R; ++h) {
+    const int i = i0 - 2 + h;
+    ir[h] = (jv && i >= 0 && i < nx) ? (T)(1.0 / rho[(size_t)i * ny + j]) : T(0);
+  }
+  double acc[H];
+#pragma unroll
+  for (int h = 0; h < H; ++h) acc[h] = 0.0;
+  const int qa = gq * qper, qb = min(nq, qa + qper);
+  for (int q = qa; q < qb; ++q) {
+    const T* p = pin + (size_t)q * C;
+    T P[R], v[R];
+#pragma unroll
+    for (int h = 0; h < R; ++h) {
+      const int i = i0 - 2 + h;
+      P[h] = (jv && i >= 0 && i < nx) ? p[(size_t)i * ny + j] : T(0);
+      v[h] = P[h] * ir[h];
+    }
+    const T ox = (T)omega[q], oz = (T)omega[2 * nq + q];
+    const T cx = ox / dx, cz = oz / dy;
+    const double mq = m0[q];
+    T* po = pout + (size_t)q * C;
+#pragma unroll
+    for (int h = 2; h < H + 2; ++h) {
+      const int i = i0 - 2 + h;
+      const T vm1 = __shfl_up(v[h], 1, 64), vm2 = __shfl_up(v[h], 2, 64);
+      const T vp1 = __shfl_down(v[h], 1, 64), vp2 = __shfl_down(v[h], 2, 64);
+      if (owner && i < nx) {
+        const size_t c = (size_t)i * ny + j;
+        T out;
+        if (jin && i >= 2 && i <= nx - 3) {
+          T e, w, n, s;
+          if (ox >= T(0)) {
+            e = v[h] + T(0.5) * slopefit_ref(v[h - 1], v[h], v[h + 1]);
+            w = v[h - 1] + T(0.5) * slopefit_ref(v[h - 2], v[h - 1], v[h]);
+          } else {
+            e = v[h + 1] - T(0.5) * slopefit_ref(v[h], v[h + 1], v[h + 2]);
+            w = v[h] - T(0.5) * slopefit_ref(v[h - 1], v[h], v[h + 1]);
+          }
+          if (oz >= T(0)) {
+            n = v[h] + T(0.5) * slopefit_ref(vm1, v[h], vp1);
+            s = vm1 + T(0.5) * slopefit_ref(vm2, vm1, v[h]);
+          } else {
+            n = vp1 - T(0.5) * slopefit_ref(v[h], vp1, vp2);
+            s = v[h] - T(0.5) * slopefit_ref(vm1, v[h], vp1);
+          }
+          const T flux = cx * (e - w) + cz * (n - s);
+          out = (P[h] - dE * flux) / denom;
+        } else {
+          out = (q == 0 && restore[c]) ? P[h] : P[h] / denom;
+        }
+        po[c] = out;
+        acc[h - 2] += (double)out * mq;
+      }
+    }
+  }
+#pragma unroll
+  for (int h = 2; h < H + 2; ++h) {
+    const int i = i0 - 2 + h;
+    if (owner && i < nx) u0p[(size_t)gq * C + (size_t)i * ny + j] = acc[h - 2];
+  }
+}
+
 // K[c,b] = X[c,:] Z0[:,b] - dE * sum_d (X[d,:]/rho[d]) Zd[:,b] (stencil part on
 // interior c only).  One thread per cell computes NB output columns b0.., so
 // every X value is loaded once per column block; the 2*ND + 2 coefficient
@@ -722,7 +821,8 @@ SweepShape sweep_shape(const Grid& g, int nq, const SweepPlan& p) {
     s.njb = (g.ny + (kBlock / 64) - 1) / (kBlock / 64);
     s.nsb = (g.nx + p.rows - 1) / p.rows;
   } else {
-    s.ntj = (g.ny + kSweepCols - 1) / kSweepCols;
+    const int cols = p.limited ? kSweepColsLim : kSweepCols;
+    s.ntj = (g.ny + cols - 1) / cols;
     s.njb = 1;
     s.nsb = (g.nx + (kBlock / 64) * p.rows - 1) / ((kBlock / 64) * p.rows);
   }
@@ -734,12 +834,19 @@ SweepShape sweep_shape(const Grid& g, int nq, const SweepPlan& p) {
 template <class T, int H>
 void sweep_launch(hipStream_t st, const Grid& g, const void* pin, void* pout, int nq,
                   const double* omega, double dE, double denom, const unsigned char* restore,
-                  const double* m0, double* u0p, const SweepShape& s, bool nt) {
+                  const double* m0, double* u0p, const SweepShape& s, bool nt, bool lim) {
   if (g.ndim() == 3) {
     hipLaunchKernelGGL((k_sweep3<T, H>), dim3((unsigned)(s.ntj * s.njb * s.nsb * s.G)),
                        dim3(kBlock), 0, st, static_cast<const T*>(pin), static_cast<T*>(pout),
                        g.nx, g.ny, g.nz, nq, s.qper, omega, (T)g.dx, (T)g.dy, (T)g.dz, g.rho,
                        (T)dE, (T)denom, restore, m0, u0p, s.ntj, s.njb, s.nsb);
+    return;
+  }
+  if (lim) {
+    hipLaunchKernelGGL((k_sweep_lim<T, H>), dim3((unsigned)(s.ntj * s.nsb * s.G)), dim3(kBlock), 0,
+                       st, static_cast<const T*>(pin), static_cast<T*>(pout), g.nx, g.ny, nq,
+                       s.qper, omega, (T)g.dx, (T)g.dy, g.rho, (T)dE, (T)denom, restore, m0, u0p,
+                       s.ntj, s.nsb);
     return;
   }
   auto kern = nt ? k_sweep<T, H, true> : k_sweep<T, H, false>;
@@ -779,24 +886,26 @@ void launch_sweep(hipStream_t st, const Grid& g, const void* pin, void* pout, DT
   if (nq == 0) return;
   if (plan.groups < 1 || plan.groups > kMaxSweepGroups)
     throw std::runtime_error("sweep: ordinate groups out of range");
+  if (plan.limited && (g.ndim() != 2 || plan.nontemporal))
+    throw std::runtime_error("sweep: the limited flux is 2D and has no non-temporal form");
   const SweepShape s = sweep_shape(g, nq, plan);
   const bool f64 = tpsi == DT::F64;
   switch (plan.rows) {
     case 2:
-      if (f64) sweep_launch<double, 2>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal);
-      else sweep_launch<float, 2>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal);
+      if (f64) sweep_launch<double, 2>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal, plan.limited);
+      else sweep_launch<float, 2>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal, plan.limited);
       break;
     case 4:
-      if (f64) sweep_launch<double, 4>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal);
-      else sweep_launch<float, 4>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal);
+      if (f64) sweep_launch<double, 4>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal, plan.limited);
+      else sweep_launch<float, 4>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal, plan.limited);
       break;
     case 8:
-      if (f64) sweep_launch<double, 8>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal);
-      else sweep_launch<float, 8>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal);
+      if (f64) sweep_launch<double, 8>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal, plan.limited);
+      else sweep_launch<float, 8>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal, plan.limited);
       break;
     case 16:
-      if (f64) sweep_launch<double, 16>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal);
-      else sweep_launch<float, 16>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal);
+      if (f64) sweep_launch<double, 16>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal, plan.limited);
+      else sweep_launch<float, 16>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal, plan.limited);
       break;
     default:
       throw std::runtime_error("sweep: rows per strip must be 2, 4, 8 or 16");

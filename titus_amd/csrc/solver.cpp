@@ -17,10 +17,13 @@ std::map<std::tuple<int, int, int, int>, dev::SweepPlan> g_tuned;
 }  // namespace
 
 static void tune_sweep(titus_ctx& x) {
+  const bool lim = x.flux_order == 2;
   x.sweep_plan = dev::sweep_default(x.g, x.nq, x.tpsi);
+  x.sweep_plan.limited = lim;
   x.sweep_tuned = false;
   if ((size_t)x.C * x.nq < kTuneMinWork) return;
-  const auto key = std::make_tuple(x.nx, x.ny * x.nz + (x.nz > 1 ? 1 << 24 : 0), x.nq, x.mixed ? 1 : 0);
+  const auto key = std::make_tuple(x.nx, x.ny * x.nz + (x.nz > 1 ? 1 << 24 : 0), x.nq,
+                                   (x.mixed ? 1 : 0) + (lim ? 2 : 0));
   std::lock_guard<std::mutex> lock(g_tune_mu);
   auto it = g_tuned.find(key);
   if (it != g_tuned.end()) {
@@ -33,7 +36,9 @@ static void tune_sweep(titus_ctx& x) {
   HIPCHK(hipEventCreate(&b));
   const double denom = 1.0 + x.dE * x.sigma[0];
   float best = 0.f;
-  for (const dev::SweepPlan& c : dev::sweep_candidates(x.g, x.nq, x.tpsi)) {
+  for (dev::SweepPlan c : dev::sweep_candidates(x.g, x.nq, x.tpsi)) {
+    if (lim && c.nontemporal) continue;
+    c.limited = lim;
     auto run = [&] {
       dev::launch_sweep(x.st, x.g, x.psi, x.psi2, x.tpsi, x.nq, x.omega, x.dE, denom, x.restore,
                         x.m0, x.u0p, c);
@@ -83,6 +88,9 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
                 p->precision == TITUS_PRECISION_MIXED_FULL_FP32,
             "unknown precision form");
     require(p->r >= 1, "rank must be >= 1");
+    const int flux_order = p->version >= 3 && p->flux_order != 0 ? p->flux_order : 1;
+    require(flux_order == 1 || flux_order == 2, "flux order must be 1 (upwind) or 2 (limited)");
+    require(flux_order == 1 || nz == 1, "the limited flux is 2D only");
     require((long long)p->nx * p->ny * nz < (1LL << 31), "grid too large");
     const int C = p->nx * p->ny * nz;
     for (int c = 0; c < C; ++c)
@@ -130,6 +138,7 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     x.aAz = copy_mat(p->absAz, p->nm, p->nm);
     x.M = copy_mat(p->M, p->nm, p->nq);
     x.nz = nz;
+    x.flux_order = flux_order;
     x.nd = nz > 1 ? 3 : 2;
     x.dz = nz > 1 ? p->dz : 1.0;
     if (x.nd == 3) {

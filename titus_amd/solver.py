@@ -9,6 +9,8 @@ Reference interface (Julia, ``code/SolverCSD.jl``):
   returns the above plus ``[E; rank]``, ``[E eta]``, ``[E bound]`` before ``psi``
 * ``SolveFirstCollisionSourceDLRParallel(obj, recordGif=false)`` -- ``:1726-2008``
   returns the fixed-rank tuple with ``rVec`` (2 x nEnergies) before ``psi``
+* ``CSolveFirstCollisionSourceDLR(obj)``               -- ``:1122-1360``
+  the fixed-rank solve with the limited flux ``solveFlux!``; same tuple
 * ``SolveFirstCollisionSource(obj)``                   -- ``:465-586``
   returns ``(0.5*sqrt(gamma[1])*u, dose, psi)`` (the full-rank P_N solve)
 
@@ -103,6 +105,27 @@ def SolveFirstCollisionSourceDLR(obj):
     """Fixed-rank first-collision-source DLRA solve on the GPU."""
     P = obj.problem(INTEGRATOR_FIXED)
     ctx = obj.context(INTEGRATOR_FIXED)
+    try:
+        ctx.solve(0)
+        st = ctx.state()
+    finally:
+        ctx.close()
+    return _finish(obj, P, st)
+
+
+def CSolveFirstCollisionSourceDLR(obj):
+    """The reference's GPU variant of the fixed-rank solve
+    (``SolverCSD.jl:1122-1360``): ``SolveFirstCollisionSourceDLR`` with the
+    second-order limited uncollided flux ``solveFlux!`` (``:1240``).  It
+    defines ``psi`` only for the lung, lungOrig, liver and validation
+    problems (``:1143-1151``); other problems raise, as there."""
+    if obj.settings.problem not in ("lung", "lungOrig", "liver", "validation"):
+        raise ValueError("CSolveFirstCollisionSourceDLR defines psi only for the lung, lungOrig, "
+                         "liver and validation problems (SolverCSD.jl:1143-1151)")
+    P = dict(obj.problem(INTEGRATOR_FIXED))
+    P["flux_order"] = 2
+    ctx = _lib.Context(P, INTEGRATOR_FIXED, device=obj.device, precision=_PRECISIONS[obj.precision],
+                       rank_tol=obj.tolerance(INTEGRATOR_FIXED))
     try:
         ctx.solve(0)
         st = ctx.state()
