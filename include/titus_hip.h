@@ -268,8 +268,9 @@ int titus_fcs_kernel_stats(const titus_fcs* ctx, const char* name, double* total
                            int64_t* launches, double* bytes, double* flops);
 /* Operator plan of the collided kernel (no reference counterpart): values[0]
  * column blocks per wavefront, [1] block groups, [2] nonzero 4 x 16 operator
- * blocks, [3] dense block count, [4] 1 when chosen by timing at creation,
- * [5] 16-cell tiles per wavefront.  Writes at most n values. */
+ * blocks (matrix-core steps per 16 cells), [3] dense block count, [4] 1 when
+ * chosen by timing at creation, [5] 16-cell tiles per wavefront, [6] 4-moment
+ * K-steps per block.  Writes at most n values. */
 int titus_fcs_get_plan(const titus_fcs* ctx, int32_t* values, int32_t n);
 void titus_fcs_destroy(titus_fcs* ctx);
 

@@ -391,11 +391,11 @@ class FcsContext:
 
     def plan(self):
         """Operator plan of the collided kernel."""
-        v = np.zeros(6, dtype=np.int32)
-        check(self._lib.titus_fcs_get_plan(self.h, _ip(v), 6))
+        v = np.zeros(7, dtype=np.int32)
+        check(self._lib.titus_fcs_get_plan(self.h, _ip(v), 7))
         return {"column_blocks_per_wave": int(v[0]), "groups": int(v[1]),
                 "nonzero_blocks": int(v[2]), "dense_blocks": int(v[3]), "tuned": bool(v[4]),
-                "cell_tiles_per_wave": int(v[5])}
+                "cell_tiles_per_wave": int(v[5]), "k_steps_per_block": int(v[6])}
 
     def close(self):
         if getattr(self, "h", None):

@@ -57,18 +57,19 @@ std::vector<int> moment_order(int nm, const std::vector<const double*>& g1,
 // Block-sparse plan for `nbw` column blocks per wavefront (layout of
 // dev::PnBlocks).  ops[o](k, m) in the device moment order.
 struct HostPlan {
-  int nbw = 0, ngroups = 0;
+  int nbw = 0, ngroups = 0, ks = 1;
   std::vector<unsigned> mask;
   std::vector<int> start;
   std::vector<double> pk;
   long long nnz_blocks = 0;
 };
 
-HostPlan build_plan(int nm, int nbw, const std::vector<const double*>& ops,
+HostPlan build_plan(int nm, int nbw, int ks, const std::vector<const double*>& ops,
                     const std::vector<int>& perm) {
   HostPlan h;
-  const int nkb = (nm + 3) / 4, nnb = (nm + 15) / 16;
+  const int kr = 4 * ks, nkb = (nm + kr - 1) / kr, nnb = (nm + 15) / 16;
   h.nbw = nbw;
+  h.ks = ks;
   h.ngroups = (nnb + nbw - 1) / nbw;
   h.mask.assign((size_t)h.ngroups * nkb * 4, 0u);
   h.start.assign((size_t)h.ngroups * (nkb + 1), 0);
@@ -84,24 +85,25 @@ HostPlan build_plan(int nm, int nbw, const std::vector<const double*>& ops,
           const int mb = g * nbw + t;
           if (mb >= nnb) continue;
           bool nz = false;
-          for (int l = 0; l < 64 && !nz; ++l) nz = val(o, kb * 4 + l / 16, mb * 16 + l % 16) != 0.0;
+          for (int l = 0; l < 64 * ks && !nz; ++l)
+            nz = val(o, kb * kr + l / 16, mb * 16 + l % 16) != 0.0;
           if (!nz) continue;
           h.mask[((size_t)g * nkb + kb) * 4 + o] |= 1u << t;
-          for (int l = 0; l < 64; ++l) h.pk.push_back(val(o, kb * 4 + l / 16, mb * 16 + l % 16));
+          for (int l = 0; l < 64 * ks; ++l) h.pk.push_back(val(o, kb * kr + l / 16, mb * 16 + l % 16));
           ++blk;
         }
     }
     h.start[(size_t)g * (nkb + 1) + nkb] = blk;
   }
-  h.nnz_blocks = blk;
+  h.nnz_blocks = (long long)blk * ks;          // in 4 x 16 units
   // the kernel reads a K-block's full capacity (4 nbw blocks) from its start
-  h.pk.resize(h.pk.size() + (size_t)4 * nbw * 64, 0.0);
+  h.pk.resize(h.pk.size() + (size_t)4 * nbw * 64 * ks, 0.0);
   return h;
 }
 
 constexpr size_t kTuneMinWork = size_t(1) << 20;   // cells x moments worth timing plans for
 std::mutex g_fcs_tune_mu;
-struct FcsTuned { int nbw, ct; dev::SweepPlan sweep; };
+struct FcsTuned { int nbw, ct, ks; dev::SweepPlan sweep; };
 std::map<std::tuple<int, int, int, int>, FcsTuned> g_fcs_tuned;
 
 }  // namespace
@@ -164,7 +166,8 @@ struct titus_fcs {
     HIPCHK(hipMemcpy(mask, h.mask.data(), sizeof(unsigned) * h.mask.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(start, h.start.data(), sizeof(int) * h.start.size(), hipMemcpyHostToDevice));
     blocks.nm = nm;
-    blocks.nkb = (nm + 3) / 4;
+    blocks.ks = h.ks;
+    blocks.nkb = (nm + 4 * h.ks - 1) / (4 * h.ks);
     blocks.nnb = (nm + 15) / 16;
     blocks.nbw = h.nbw;
     blocks.ngroups = h.ngroups;
@@ -212,7 +215,7 @@ struct titus_fcs {
     int widest = 1;
     while (widest < nnb && widest < dev::kPnMaxNbw) widest *= 2;
     plan = dev::sweep_default(g1, nq, dev::DT::F64);
-    int best_nbw = std::min(widest, 8), best_ct = 1;
+    int best_nbw = std::min(widest, 8), best_ct = 1, best_ks = 1;
     tuned = false;
     const auto key = std::make_tuple(nx, ny, nm, nq);
     std::lock_guard<std::mutex> lock(g_fcs_tune_mu);
@@ -221,6 +224,7 @@ struct titus_fcs {
       if (it != g_fcs_tuned.end()) {
         best_nbw = it->second.nbw;
         best_ct = it->second.ct;
+        best_ks = it->second.ks;
         plan = it->second.sweep;
         tuned = true;
       } else {
@@ -238,14 +242,17 @@ struct titus_fcs {
           return ms;
         };
         float best = 0.f;
-        for (int nbw = 1; nbw <= widest; nbw *= 2) {
-          upload_plan(build_plan(nm, nbw, ops, perm));
-          for (int ct = 1; ct <= (nbw <= 8 ? 2 : 1); ++ct) {
-            blocks.ct = ct;
-            const float ms = time_it([&] { collide(dE[0], sig_d + nsig); });
-            if (!tuned || ms < best) { best = ms; best_nbw = nbw; best_ct = ct; tuned = true; }
+        for (int nbw = 1; nbw <= widest; nbw *= 2)
+          for (int ks = 1; ks <= (nbw <= 8 ? 2 : 1); ++ks) {
+            upload_plan(build_plan(nm, nbw, ks, ops, perm));
+            for (int ct = 1; ct <= (nbw <= 8 ? 2 : 1); ++ct) {
+              blocks.ct = ct;
+              const float ms = time_it([&] { collide(dE[0], sig_d + nsig); });
+              if (!tuned || ms < best) {
+                best = ms; best_nbw = nbw; best_ct = ct; best_ks = ks; tuned = true;
+              }
+            }
           }
-        }
         bool first = true;
         for (const dev::SweepPlan& c : dev::sweep_candidates(g1, nq, dev::DT::F64)) {
           const float ms = time_it([&] {
@@ -256,10 +263,10 @@ struct titus_fcs {
         }
         hipEventDestroy(a);
         hipEventDestroy(b);
-        g_fcs_tuned[key] = {best_nbw, best_ct, plan};
+        g_fcs_tuned[key] = {best_nbw, best_ct, best_ks, plan};
       }
     }
-    upload_plan(build_plan(nm, best_nbw, ops, perm));
+    upload_plan(build_plan(nm, best_nbw, best_ks, ops, perm));
     blocks.ct = best_ct;
     G = dev::sweep_groups(g1, nq, plan);
     // the timing runs wrote into the zero buffers: clear them again
@@ -470,9 +477,10 @@ int titus_fcs_kernel_stats(const titus_fcs* ctx, const char* name, double* total
 int titus_fcs_get_plan(const titus_fcs* ctx, int32_t* values, int32_t n) {
   return guarded([&] {
     require(ctx != nullptr && (values != nullptr || n <= 0), "null argument");
-    const int32_t v[6] = {ctx->blocks.nbw, ctx->blocks.ngroups, (int32_t)ctx->nnz_blocks,
-                          (int32_t)ctx->dense_blocks, ctx->tuned ? 1 : 0, ctx->blocks.ct};
-    for (int i = 0; i < n && i < 6; ++i) values[i] = v[i];
+    const int32_t v[7] = {ctx->blocks.nbw, ctx->blocks.ngroups, (int32_t)ctx->nnz_blocks,
+                          (int32_t)ctx->dense_blocks, ctx->tuned ? 1 : 0, ctx->blocks.ct,
+                          ctx->blocks.ks};
+    for (int i = 0; i < n && i < 7; ++i) values[i] = v[i];
   });
 }
 

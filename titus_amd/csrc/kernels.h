@@ -145,15 +145,17 @@ void launch_sweep_adj(hipStream_t st, const Grid& g, const double* G, const doub
 // ---- full-rank P_N solve (SolveFirstCollisionSource, SolverCSD.jl:465-586) -----
 // Block-sparse moment operators of the collided streaming sum.  The four
 // operators |Ax|, |Az|, Ax, Az (o = 0..3, paired with the stencils L1x, L1y,
-// L2x, L2y) are cut into 4 x 16 blocks: rows k = 4 kb .. 4 kb + 3, columns
-// m = 16 mb .. 16 mb + 15.  Column blocks are dealt to `ngroups` groups of
-// `nbw` (mb = g nbw + t).  mask[(g nkb + kb) 4 + o] bit t marks block
-// (o, kb, g nbw + t) nonzero; the nonzero blocks of (g, kb) are stored from
-// block index start[g (nkb + 1) + kb] on, ordered by o, then t, 64 values each
-// (value l = operator[4 kb + l / 16, 16 mb + l % 16]).
+// L2x, L2y) are cut into (4 ks) x 16 blocks: rows k = 4 ks kb .. 4 ks kb +
+// 4 ks - 1, columns m = 16 mb .. 16 mb + 15.  Column blocks are dealt to
+// `ngroups` groups of `nbw` (mb = g nbw + t).  mask[(g nkb + kb) 4 + o] bit t
+// marks block (o, kb, g nbw + t) nonzero; the nonzero blocks of (g, kb) are
+// stored from block index start[g (nkb + 1) + kb] on, ordered by o, then t,
+// 64 ks values each (value s 64 + l = operator[4 ks kb + 4 s + l / 16,
+// 16 mb + l % 16]), followed by 4 nbw zero blocks.
 struct PnBlocks {
   int nm = 0, nkb = 0, nnb = 0, nbw = 0, ngroups = 0;
   int ct = 1;                 // 16-cell tiles per wavefront (1, or 2 when nbw <= 8)
+  int ks = 1;                 // K-steps of 4 moments per block (1, or 2 when nbw <= 8)
   const unsigned* mask = nullptr;
   const int* start = nullptr;
   const double* pk = nullptr;

@@ -1213,20 +1213,20 @@ void launch_sweep_adj(hipStream_t st, const Grid& g, const double* G, const doub
 // of one block group.  For each K-block of 4 moments the stencils of u/rho
 // are formed in registers as the B operands -- lane l: cell l % 16 of each
 // tile, moment 4 kb + l / 16, from five loads (c, c -+ ny, c -+ 1) -- and the
-// nonzero operator blocks of that K-block, staged in LDS by the whole
-// workgroup, are the A operands; each A operand read serves the CT tiles.
-// Software pipeline over the K-blocks: while K-block kb runs on the matrix
-// cores, the operator blocks of kb+1 (loaded one K-block earlier) go to the
-// other LDS buffer and the loads of kb+2 (operator blocks, u values, masks)
-// are issued, so no wait falls on a load younger than one K-block; the u
-// values alternate between two register sets (even / odd K-blocks).  Loads
-// and LDS stores carry no conditions: the packed blocks are followed by
-// 4 NBW zero blocks (a K-block's full capacity can always be read), a ghost
-// or padding cell reads a valid cell whose values its zero 1/rho cancels, and
-// rows k >= nm meet zero operator rows.  The in-scattering source
+// nonzero operator blocks of that K-block are the A operands, read by each
+// wavefront straight from the packed array (it stays in L2: every cell tile
+// reads the same blocks), running two values ahead of the matrix-core step
+// that consumes them; each A operand serves the CT tiles.  The u values of
+// K-block kb+2 are loaded while kb runs, alternating between two register
+// sets (even / odd K-blocks), so no wait falls on a load younger than one
+// K-block.  No barrier: the wavefronts of a workgroup share nothing.  Loads
+// carry no conditions: the packed blocks are followed by 4 NBW zero blocks (a
+// K-block's full capacity can always be read), a ghost or padding cell reads
+// a valid cell whose values its zero 1/rho cancels, and rows k >= nm meet zero
+// operator rows.  The in-scattering source
 // -Dvec[m] M[m, q] psi[c, q] follows as a dense K-loop over the ordinates.
 // Epilogue: the implicit per-cell scattering solve, zero on ghost cells.
-template <int NBW, int CT>
+template <int NBW, int CT, int KS>
 __global__ __launch_bounds__(kBlock) void k_pn_collide(
     const double* __restrict__ u, double* __restrict__ un, int C, int nx, int ny, double idx_,
     double idy_, const double* __restrict__ irho, int nm, int nkb, int ngroups,
@@ -1235,8 +1235,6 @@ __global__ __launch_bounds__(kBlock) void k_pn_collide(
     int nq, const double* __restrict__ M, const int* __restrict__ degree,
     const double* __restrict__ sig, double dE) {
   using MF = Mfma<double>;
-  constexpr int kCap = 4 * NBW * 64;           // operator values of one K-block
-  __shared__ double sa[2][kCap];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int L = xcd_block(blockIdx.x, gridDim.x);
   const int grp = L % ngroups, tile = L / ngroups;
@@ -1259,61 +1257,52 @@ __global__ __launch_bounds__(kBlock) void k_pn_collide(
   }
   const unsigned* mk = mask + (size_t)grp * nkb * 4;
   const int* st = start + (size_t)grp * (nkb + 1);
-  double pf[NBW];
-  double vA[CT][5], vB[CT][5];
+  constexpr int KR = 4 * KS;                  // moments (K rows) per K-block
+  double vA[KS][CT][5], vB[KS][CT][5];
   uint4 mA, mB;
-  auto load_blocks = [&](int kb) {
+  auto load_u = [&](int kb, double (*v)[CT][5], uint4& m) {
     if (kb >= nkb) return;
-    const double* src = pk + (size_t)st[kb] * 64 + tid;
 #pragma unroll
-    for (int e = 0; e < NBW; ++e) pf[e] = src[e * kBlock];
-  };
-  auto load_u = [&](int kb, double (*v)[5], uint4& m) {
-    if (kb >= nkb) return;
-    const double* uk = u + (size_t)min(kb * 4 + kg, nm - 1) * C;
+    for (int ks = 0; ks < KS; ++ks) {
+      const double* uk = u + (size_t)min(kb * KR + ks * 4 + kg, nm - 1) * C;
 #pragma unroll
-    for (int s = 0; s < CT; ++s) {
-      const double* p = uk + cu[s];
-      v[s][0] = p[0];
-      v[s][1] = p[-ny];
-      v[s][2] = p[ny];
-      v[s][3] = p[-1];
-      v[s][4] = p[1];
+      for (int s = 0; s < CT; ++s) {
+        const double* p = uk + cu[s];
+        v[ks][s][0] = p[0];
+        v[ks][s][1] = p[-ny];
+        v[ks][s][2] = p[ny];
+        v[ks][s][3] = p[-1];
+        v[ks][s][4] = p[1];
+      }
     }
     m = *reinterpret_cast<const uint4*>(mk + kb * 4);
-  };
-  auto stage = [&](int kb, int buf) {
-    if (kb >= nkb) return;
-#pragma unroll
-    for (int e = 0; e < NBW; ++e) sa[buf][tid + e * kBlock] = pf[e];
   };
   f64x4 acc[CT][NBW];
 #pragma unroll
   for (int s = 0; s < CT; ++s)
 #pragma unroll
     for (int t = 0; t < NBW; ++t) acc[s][t] = f64x4{0.0, 0.0, 0.0, 0.0};
-  auto iter = [&](int kb, double (*v)[5], uint4& m) {
-    const int buf = kb & 1;
-    double b[CT][4];
+  auto iter = [&](int kb, double (*v)[CT][5], uint4& m) {
+    double b[KS][CT][4];
 #pragma unroll
-    for (int s = 0; s < CT; ++s) {
-      double y[5];
+    for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
-      for (int d = 0; d < 5; ++d) y[d] = v[s][d] * ir[s][d];
-      b[s][0] = (y[0] - 0.5 * y[1] - 0.5 * y[2]) * idx_;     // L1x
-      b[s][1] = (y[0] - 0.5 * y[3] - 0.5 * y[4]) * idy_;     // L1y
-      b[s][2] = (y[2] - y[1]) * (0.5 * idx_);                // L2x
-      b[s][3] = (y[4] - y[3]) * (0.5 * idy_);                // L2y
-    }
+      for (int s = 0; s < CT; ++s) {
+        double y[5];
+#pragma unroll
+        for (int d = 0; d < 5; ++d) y[d] = v[ks][s][d] * ir[s][d];
+        b[ks][s][0] = (y[0] - 0.5 * y[1] - 0.5 * y[2]) * idx_;     // L1x
+        b[ks][s][1] = (y[0] - 0.5 * y[3] - 0.5 * y[4]) * idy_;     // L1y
+        b[ks][s][2] = (y[2] - y[1]) * (0.5 * idx_);                // L2x
+        b[ks][s][3] = (y[4] - y[3]) * (0.5 * idy_);                // L2y
+      }
     const uint4 mcur = m;
-    stage(kb + 1, buf ^ 1);
-    load_blocks(kb + 2);
     load_u(kb + 2, v, m);
-    // the K-block's operator blocks sit in LDS in the order of their mask
-    // bits; each LDS read runs two blocks ahead of the matrix-core steps that
-    // consume it (indices clamped to the buffer, surplus reads unused)
-    const double* a = sa[buf] + lane;
-    int j = 0;
+    // the K-block's operator blocks follow each other in the order of their
+    // mask bits, KS K-steps of 64 values each; reads run two values ahead
+    // (indices clamped to the K-block's capacity, surplus reads unused)
+    const double* a = pk + (size_t)st[kb] * (64 * KS) + lane;
+    int e = 0;
     double a0 = a[0], a1 = a[64];
 #pragma unroll
     for (int o = 0; o < 4; ++o) {
@@ -1321,22 +1310,20 @@ __global__ __launch_bounds__(kBlock) void k_pn_collide(
 #pragma unroll
       for (int t = 0; t < NBW; ++t)
         if (bits & (1u << t)) {
-          const double cur = a0;
-          a0 = a1;
-          ++j;
-          a1 = a[64 * min(j + 1, 4 * NBW - 1)];
 #pragma unroll
-          for (int s = 0; s < CT; ++s) acc[s][t] = MF::step(cur, b[s][o], acc[s][t]);
+          for (int ks = 0; ks < KS; ++ks) {
+            const double cur = a0;
+            a0 = a1;
+            ++e;
+            a1 = a[64 * min(e + 1, 4 * NBW * KS - 1)];
+#pragma unroll
+            for (int s = 0; s < CT; ++s) acc[s][t] = MF::step(cur, b[ks][s][o], acc[s][t]);
+          }
         }
     }
-    __syncthreads();
   };
-  load_blocks(0);
   load_u(0, vA, mA);
-  stage(0, 0);
-  load_blocks(1);
   load_u(1, vB, mB);
-  __syncthreads();
   for (int kb = 0; kb < nkb; kb += 2) {
     iter(kb, vA, mA);
     if (kb + 1 < nkb) iter(kb + 1, vB, mB);
@@ -1376,23 +1363,32 @@ __global__ __launch_bounds__(kBlock) void k_pn_collide(
   }
 }
 
-template <int NBW, int CT>
+template <int NBW, int CT, int KS>
 void pn_collide_launch(hipStream_t st, const Grid& g, const PnBlocks& b, const double* u, double* un,
                        const double* psi, int nq, const double* M, const int* degree,
                        const double* sig, double dE) {
   const int cells = 64 * CT;
   const unsigned blocks = (unsigned)(((g.C + cells - 1) / cells) * b.ngroups);
-  hipLaunchKernelGGL((k_pn_collide<NBW, CT>), dim3(blocks), dim3(kBlock), 0, st, u, un, g.C, g.nx,
-                     g.ny, 1.0 / g.dx, 1.0 / g.dy, g.irho, b.nm, b.nkb, b.ngroups, b.mask, b.start,
-                     b.pk, psi, nq, M, degree, sig, dE);
+  hipLaunchKernelGGL((k_pn_collide<NBW, CT, KS>), dim3(blocks), dim3(kBlock), 0, st, u, un, g.C,
+                     g.nx, g.ny, 1.0 / g.dx, 1.0 / g.dy, g.irho, b.nm, b.nkb, b.ngroups, b.mask,
+                     b.start, b.pk, psi, nq, M, degree, sig, dE);
+}
+
+template <int NBW, int CT>
+void pn_collide_ks(hipStream_t st, const Grid& g, const PnBlocks& b, const double* u, double* un,
+                   const double* psi, int nq, const double* M, const int* degree,
+                   const double* sig, double dE) {
+  if (b.ks == 1) pn_collide_launch<NBW, CT, 1>(st, g, b, u, un, psi, nq, M, degree, sig, dE);
+  else if (b.ks == 2 && NBW <= 8) pn_collide_launch<NBW, CT, (NBW <= 8 ? 2 : 1)>(st, g, b, u, un, psi, nq, M, degree, sig, dE);
+  else throw std::runtime_error("pn_collide: K-steps per block must be 1, or 2 with at most 8 column blocks");
 }
 
 template <int NBW>
 void pn_collide_nbw(hipStream_t st, const Grid& g, const PnBlocks& b, const double* u, double* un,
                     const double* psi, int nq, const double* M, const int* degree,
                     const double* sig, double dE) {
-  if (b.ct == 2 && NBW <= 8) pn_collide_launch<NBW, (NBW <= 8 ? 2 : 1)>(st, g, b, u, un, psi, nq, M, degree, sig, dE);
-  else if (b.ct == 1) pn_collide_launch<NBW, 1>(st, g, b, u, un, psi, nq, M, degree, sig, dE);
+  if (b.ct == 2 && NBW <= 8) pn_collide_ks<NBW, (NBW <= 8 ? 2 : 1)>(st, g, b, u, un, psi, nq, M, degree, sig, dE);
+  else if (b.ct == 1) pn_collide_ks<NBW, 1>(st, g, b, u, un, psi, nq, M, degree, sig, dE);
   else throw std::runtime_error("pn_collide: cell tiles per wavefront must be 1, or 2 with at most 8 column blocks");
 }
 
@@ -1400,7 +1396,7 @@ void launch_pn_collide(hipStream_t st, const Grid& g, const PnBlocks& b, const d
                        const double* psi, int nq, const double* M, const int* degree,
                        const double* sig, double dE) {
   if (g.ndim() != 2) throw std::runtime_error("pn_collide: 2D grids only");
-  if (b.ngroups * b.nbw < b.nnb || b.nkb * 4 < b.nm)
+  if (b.ngroups * b.nbw < b.nnb || b.nkb * 4 * b.ks < b.nm)
     throw std::runtime_error("pn_collide: block plan does not cover the moments");
   switch (b.nbw) {
     case 1: pn_collide_nbw<1>(st, g, b, u, un, psi, nq, M, degree, sig, dE); break;
