@@ -360,7 +360,7 @@ struct titus_ctx {
   template <class F>
   void for_ops(size_t work, F&& f) const {
     const int n = (int)sA.size();
-    if (!host_parallel || work < (size_t(1) << 18) || n < 2) {
+    if (!host_parallel || work < (size_t(1) << 20) || n < 2) {
       for (int o = 0; o < n; ++o) f(o);
       return;
     }
