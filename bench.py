@@ -50,7 +50,7 @@ PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 MEASURED_TRAFFIC = {"gram": 3.19e6}
 # HBM bytes per launch of the full-rank solver's collided kernel on the
 # phantom (FETCH_SIZE x 2 + WRITE_SIZE x 1, profiles/fcs_collide_phantom200_p21.md)
-MEASURED_TRAFFIC_FULL_PN = {"collide": 2.625e8}
+MEASURED_TRAFFIC_FULL_PN = {"collide": 2.698e8}
 
 
 def build_workload(n=200, N=21, cap=40, seed=0):
