@@ -12,6 +12,19 @@
 
 namespace titus {
 
+static inline double dot(const double* a, const double* b, int n) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int i = 0;
+  for (; i + 4 <= n; i += 4) {
+    s0 += a[i] * b[i];
+    s1 += a[i + 1] * b[i + 1];
+    s2 += a[i + 2] * b[i + 2];
+    s3 += a[i + 3] * b[i + 3];
+  }
+  for (; i < n; ++i) s0 += a[i] * b[i];
+  return (s0 + s1) + (s2 + s3);
+}
+
 HMat HMat::eye(int m, int n) {
   HMat I(m, n);
   for (int i = 0; i < std::min(m, n); ++i) I(i, i) = 1.0;
@@ -96,6 +109,13 @@ HMat matmul(const HMat& A, const HMat& B, bool ta, bool tb) {
   const int m = ta ? A.n : A.m, k = ta ? A.m : A.n;
   const int k2 = tb ? B.n : B.m, n = tb ? B.m : B.n;
   if (k != k2) throw std::runtime_error("matmul: inner dimensions differ");
+  if (ta && !tb) {
+    // A^T B: every entry is a dot product of two contiguous columns
+    HMat C(m, n);
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < m; ++i) C(i, j) = dot(A.col(i), B.col(j), k);
+    return C;
+  }
   // transposed operands are materialised once, so every product runs as
   // column updates c += a_l * b_lj, vectorised over the rows; each entry is
   // still summed over l in ascending order
@@ -146,18 +166,6 @@ double frob(const HMat& A) {
 
 // Dot product with four independent partial sums (vectorisable; the sum
 // order is fixed, so results are reproducible).
-static inline double dot(const double* a, const double* b, int n) {
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int i = 0;
-  for (; i + 4 <= n; i += 4) {
-    s0 += a[i] * b[i];
-    s1 += a[i + 1] * b[i + 1];
-    s2 += a[i + 2] * b[i + 2];
-    s3 += a[i + 3] * b[i + 3];
-  }
-  for (; i < n; ++i) s0 += a[i] * b[i];
-  return (s0 + s1) + (s2 + s3);
-}
 
 // ---------------------------------------------------------------------------
 // Householder QR of an m x n matrix, m >= n.  On return A holds R in its upper
