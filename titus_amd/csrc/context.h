@@ -342,9 +342,18 @@ struct titus_ctx {
       dev::launch_gemm(st, C, A, C, k, Td, n, alpha, beta, out, C, mask);
     });
   }
+  // small coefficient blocks travel in the kernel arguments, larger ones
+  // through the upload ring
   void gemm(const double* A, int k, const HMat& T, double alpha, double beta, double* out,
             const unsigned char* mask) {
-    gemm_dev(A, k, upload(T), T.n, alpha, beta, out, mask);
+    if (T.a.size() > (size_t)dev::kArgMat) {
+      gemm_dev(A, k, upload(T), T.n, alpha, beta, out, mask);
+      return;
+    }
+    timed("gemm", 8.0 * C * (k + T.n * (beta != 0.0 ? 2 : 1)), 2.0 * C * k * T.n, [&] {
+      dev::launch_gemm_args(st, C, A, dev::DT::F64, C, k, T.a.data(), T.n, alpha, beta, out, C,
+                            mask, false);
+    });
   }
 
   // ---- host building blocks (angular side) -------------------------------------
@@ -458,7 +467,13 @@ struct titus_ctx {
     return B;
   }
   void stream_k(int r, const std::vector<HMat>& WAW) {
-    const double* Zd = upload(stream_Z(S, WAW));
+    const HMat Z = stream_Z(S, WAW);
+    if (Z.a.size() <= (size_t)dev::kArgMat) {
+      timed("stream_K", 8.0 * C * (2 * r + 1), 2.0 * (2 * nd + 2) * C * r * r,
+            [&] { dev::launch_stream_k_args(st, g, X, C, r, Z.a.data(), r, dE, K, C, full32); });
+      return;
+    }
+    const double* Zd = upload(Z);
     timed("stream_K", 8.0 * C * (2 * r + 1), 2.0 * (2 * nd + 2) * C * r * r,
           [&] { dev::launch_stream_k(st, g, X, C, r, Zd, r, dE, K, C, full32); });
   }
@@ -663,8 +678,15 @@ struct titus_ctx {
     gemm(X, rank, S, 1.0, 0.0, K, nullptr);
     HMat T = MtDW;
     for (double& v : T.a) v *= dE;
-    const double* Td = upload(T);
     const double ps = mixed ? 4.0 : 8.0;
+    if (T.a.size() <= (size_t)dev::kArgMat) {
+      timed("gemm", ps * C * nq + 16.0 * C * T.n, 2.0 * C * nq * T.n, [&] {
+        dev::launch_gemm_args(st, C, psi, tpsi, C, nq, T.a.data(), T.n, 1.0, 1.0, K, C, bmask,
+                              full32);
+      });
+      return;
+    }
+    const double* Td = upload(T);
     timed("gemm", ps * C * nq + 16.0 * C * T.n, 2.0 * C * nq * T.n, [&] {
       dev::launch_gemm(st, C, psi, tpsi, C, nq, Td, T.n, 1.0, 1.0, K, C, bmask, full32);
     });

@@ -114,6 +114,17 @@ inline void launch_gemm(hipStream_t st, int C, const double* A, int lda, int k, 
   launch_gemm(st, C, A, DT::F64, lda, k, T, n, alpha, beta, out, ldo, zero_mask, false);
 }
 
+// The same two products with their coefficients (T: k x n, Z: the 2 nd + 2
+// blocks of r x n) taken from host memory and carried in the kernel
+// arguments, when they hold at most kArgMat values: no staging copy on the
+// stream.  The host buffer may be reused as soon as the call returns.
+constexpr int kArgMat = 448;
+void launch_gemm_args(hipStream_t st, int C, const void* A, DT ta, int lda, int k,
+                      const double* T_host, int n, double alpha, double beta, double* out, int ldo,
+                      const unsigned char* zero_mask, bool fp32);
+void launch_stream_k_args(hipStream_t st, const Grid& g, const double* X, int ldx, int r,
+                          const double* Z_host, int n, double dE, double* K, int ldk, bool fp32);
+
 // out[t + k*nrows] = A[rows[t] + k*lda], k < ncols.
 void launch_gather_rows(hipStream_t st, const double* A, int lda, int ncols, const int* rows,
                         int nrows, double* out);

@@ -2,6 +2,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -42,6 +43,17 @@ __device__ inline int xcd_block(int b, int nb) {
   const int x = b % kXcds, k = b / kXcds;
   return x * per + min(x, rem) + k;
 }
+
+// Coefficient blocks of the cell-blocked kernels: read from device memory, or
+// carried in the kernel arguments when small (no staging copy on the stream).
+struct GlobalSrc {
+  const double* p;
+  __device__ double operator[](size_t i) const { return p[i]; }
+};
+struct ArgSrc {
+  double v[kArgMat];
+  __device__ double operator[](size_t i) const { return v[i]; }
+};
 
 // psi at the boundary cells for every ordinate, and the l = 0 moment of those
 // cells: u0p[0][cell] = sum_q psi[cell, q] m0[q] (the other partials are 0).
@@ -273,10 +285,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep_lim(
 // neighbours d that are interior cells, K[c,b] = X[c,:] Z0[:,b] - dE * sum_d
 // [d interior] X[d,:] Zd[:,b], with no 1/rho (the adjoint applies it to the
 // result); the centre term counts when c itself is interior.
-template <class TC, int NB, int ND, bool TR>
+template <class TC, int NB, int ND, bool TR, class Src>
 __global__ __launch_bounds__(kBlock) void k_stream_k(
     const double* __restrict__ X, int ldx, int r, int nx, int ny, int nz,
-    const double* __restrict__ rho, const double* __restrict__ Z, int n, double dE,
+    const double* __restrict__ rho, const Src Z, int n, double dE,
     double* __restrict__ K, int ldk) {
   constexpr int NZB = 2 * ND + 2;
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -703,9 +715,9 @@ __global__ void k_chol_inv(const double* __restrict__ G, int k, int allow_plain,
 // column block and multiplied into NB accumulators; T's rows a0..a0+kRowsKT
 // are staged in LDS as Ts[a*NB + k] (broadcast reads).  Sums run over a in
 // ascending order.
-template <class TA, class TC, int NB>
+template <class TA, class TC, int NB, class Src>
 __global__ __launch_bounds__(kBlock) void k_gemm(int C, const TA* __restrict__ A, int lda, int k,
-                                                 const double* __restrict__ T, int n, double alpha,
+                                                 const Src T, int n, double alpha,
                                                  double beta, double* __restrict__ out, int ldo,
                                                  const unsigned char* __restrict__ mask) {
   __shared__ TC Ts[kRowsKT * NB];
@@ -914,25 +926,23 @@ void launch_sweep(hipStream_t st, const Grid& g, const void* pin, void* pout, DT
 }
 
 namespace {
-template <class TC, int NB, int ND, bool TR = false>
-void stream_k_nd(hipStream_t st, const Grid& g, const double* X, int ldx, int r, const double* Z,
+template <class TC, int NB, int ND, bool TR = false, class Src = GlobalSrc>
+void stream_k_nd(hipStream_t st, const Grid& g, const double* X, int ldx, int r, const Src& Z,
                  int n, double dE, double* K, int ldk) {
   const dim3 grid(blocks_for(g.C), (n + NB - 1) / NB);
   const size_t lds = (size_t)r * (2 * ND + 2) * NB * sizeof(TC);
-  hipLaunchKernelGGL((k_stream_k<TC, NB, ND, TR>), grid, dim3(kBlock), lds, st, X, ldx, r, g.nx,
-                     g.ny, g.nz, g.rho, Z, n, dE, K, ldk);
+  hipLaunchKernelGGL((k_stream_k<TC, NB, ND, TR, Src>), grid, dim3(kBlock), lds, st, X, ldx, r,
+                     g.nx, g.ny, g.nz, g.rho, Z, n, dE, K, ldk);
 }
-template <class TC, int NB>
-void stream_k_nb(hipStream_t st, const Grid& g, const double* X, int ldx, int r, const double* Z,
+template <class TC, int NB, class Src>
+void stream_k_nb(hipStream_t st, const Grid& g, const double* X, int ldx, int r, const Src& Z,
                  int n, double dE, double* K, int ldk) {
-  if (g.ndim() == 3) stream_k_nd<TC, NB, 3>(st, g, X, ldx, r, Z, n, dE, K, ldk);
-  else stream_k_nd<TC, NB, 2>(st, g, X, ldx, r, Z, n, dE, K, ldk);
+  if (g.ndim() == 3) stream_k_nd<TC, NB, 3, false, Src>(st, g, X, ldx, r, Z, n, dE, K, ldk);
+  else stream_k_nd<TC, NB, 2, false, Src>(st, g, X, ldx, r, Z, n, dE, K, ldk);
 }
-}  // namespace
-
-void launch_stream_k(hipStream_t st, const Grid& g, const double* X, int ldx, int r,
-                     const double* Z, int n, double dE, double* K, int ldk, bool fp32) {
-  if (n == 0) return;
+template <class Src>
+void stream_k_any(hipStream_t st, const Grid& g, const double* X, int ldx, int r, const Src& Z,
+                  int n, double dE, double* K, int ldk, bool fp32) {
   // column blocks of NB = 16 (8 when r is wide, to bound the LDS footprint)
   const bool narrow = r > 48;
   if (fp32) {
@@ -943,6 +953,23 @@ void launch_stream_k(hipStream_t st, const Grid& g, const double* X, int ldx, in
     else stream_k_nb<double, 16>(st, g, X, ldx, r, Z, n, dE, K, ldk);
   }
   check_launch("stream_K");
+}
+}  // namespace
+
+void launch_stream_k(hipStream_t st, const Grid& g, const double* X, int ldx, int r,
+                     const double* Z, int n, double dE, double* K, int ldk, bool fp32) {
+  if (n == 0) return;
+  stream_k_any(st, g, X, ldx, r, GlobalSrc{Z}, n, dE, K, ldk, fp32);
+}
+
+void launch_stream_k_args(hipStream_t st, const Grid& g, const double* X, int ldx, int r,
+                          const double* Z_host, int n, double dE, double* K, int ldk, bool fp32) {
+  if (n == 0) return;
+  const size_t sz = (size_t)r * (2 * g.ndim() + 2) * n;
+  if (sz > (size_t)kArgMat) throw std::runtime_error("stream_K: coefficients too large for the arguments");
+  ArgSrc src;
+  std::memcpy(src.v, Z_host, sizeof(double) * sz);
+  stream_k_any(st, g, X, ldx, r, src, n, dE, K, ldk, fp32);
 }
 
 size_t tn_partial_size() { return (size_t)kMaxTnSlots * 6 * 256; }
@@ -1001,26 +1028,25 @@ void launch_chol_inv(hipStream_t st, const double* G, int k, bool allow_plain, d
 
 namespace {
 // column-block width by n: 4, 8 or 16 accumulators per thread
-template <class TA, class TC>
-void gemm_nb(hipStream_t st, int C, const void* A, int lda, int k, const double* T, int n,
+template <class TA, class TC, class Src>
+void gemm_nb(hipStream_t st, int C, const void* A, int lda, int k, const Src& T, int n,
              double alpha, double beta, double* out, int ldo, const unsigned char* mask) {
   const TA* a = static_cast<const TA*>(A);
   if (n <= 4)
-    hipLaunchKernelGGL((k_gemm<TA, TC, 4>), dim3(blocks_for(C), 1), dim3(kBlock), 0, st, C, a, lda,
-                       k, T, n, alpha, beta, out, ldo, mask);
+    hipLaunchKernelGGL((k_gemm<TA, TC, 4, Src>), dim3(blocks_for(C), 1), dim3(kBlock), 0, st, C, a,
+                       lda, k, T, n, alpha, beta, out, ldo, mask);
   else if (n <= 8)
-    hipLaunchKernelGGL((k_gemm<TA, TC, 8>), dim3(blocks_for(C), 1), dim3(kBlock), 0, st, C, a, lda,
-                       k, T, n, alpha, beta, out, ldo, mask);
+    hipLaunchKernelGGL((k_gemm<TA, TC, 8, Src>), dim3(blocks_for(C), 1), dim3(kBlock), 0, st, C, a,
+                       lda, k, T, n, alpha, beta, out, ldo, mask);
   else
-    hipLaunchKernelGGL((k_gemm<TA, TC, 16>), dim3(blocks_for(C), (n + 15) / 16), dim3(kBlock), 0,
-                       st, C, a, lda, k, T, n, alpha, beta, out, ldo, mask);
+    hipLaunchKernelGGL((k_gemm<TA, TC, 16, Src>), dim3(blocks_for(C), (n + 15) / 16), dim3(kBlock),
+                       0, st, C, a, lda, k, T, n, alpha, beta, out, ldo, mask);
 }
-}  // namespace
 
-void launch_gemm(hipStream_t st, int C, const void* A, DT ta, int lda, int k, const double* T,
-                 int n, double alpha, double beta, double* out, int ldo,
-                 const unsigned char* zero_mask, bool fp32) {
-  if (n == 0) return;
+template <class Src>
+void gemm_any(hipStream_t st, int C, const void* A, DT ta, int lda, int k, const Src& T, int n,
+              double alpha, double beta, double* out, int ldo, const unsigned char* zero_mask,
+              bool fp32) {
   if (ta == DT::F64) {
     if (fp32) gemm_nb<double, float>(st, C, A, lda, k, T, n, alpha, beta, out, ldo, zero_mask);
     else gemm_nb<double, double>(st, C, A, lda, k, T, n, alpha, beta, out, ldo, zero_mask);
@@ -1029,6 +1055,24 @@ void launch_gemm(hipStream_t st, int C, const void* A, DT ta, int lda, int k, co
     else gemm_nb<float, double>(st, C, A, lda, k, T, n, alpha, beta, out, ldo, zero_mask);
   }
   check_launch("gemm");
+}
+}  // namespace
+
+void launch_gemm(hipStream_t st, int C, const void* A, DT ta, int lda, int k, const double* T,
+                 int n, double alpha, double beta, double* out, int ldo,
+                 const unsigned char* zero_mask, bool fp32) {
+  if (n == 0) return;
+  gemm_any(st, C, A, ta, lda, k, GlobalSrc{T}, n, alpha, beta, out, ldo, zero_mask, fp32);
+}
+
+void launch_gemm_args(hipStream_t st, int C, const void* A, DT ta, int lda, int k,
+                      const double* T_host, int n, double alpha, double beta, double* out, int ldo,
+                      const unsigned char* zero_mask, bool fp32) {
+  if (n == 0) return;
+  if ((size_t)k * n > (size_t)kArgMat) throw std::runtime_error("gemm: coefficients too large for the arguments");
+  ArgSrc src;
+  std::memcpy(src.v, T_host, sizeof(double) * (size_t)k * n);
+  gemm_any(st, C, A, ta, lda, k, src, n, alpha, beta, out, ldo, zero_mask, fp32);
 }
 
 void launch_gather_rows(hipStream_t st, const double* A, int lda, int ncols, const int* rows,
@@ -1055,11 +1099,11 @@ void launch_stream_kt(hipStream_t st, const Grid& g, const double* X, int ldx, i
                       const double* Z, int n, double dE, double* K, int ldk) {
   if (n == 0) return;
   if (r > 48) {
-    if (g.ndim() == 3) stream_k_nd<double, 8, 3, true>(st, g, X, ldx, r, Z, n, dE, K, ldk);
-    else stream_k_nd<double, 8, 2, true>(st, g, X, ldx, r, Z, n, dE, K, ldk);
+    if (g.ndim() == 3) stream_k_nd<double, 8, 3, true>(st, g, X, ldx, r, GlobalSrc{Z}, n, dE, K, ldk);
+    else stream_k_nd<double, 8, 2, true>(st, g, X, ldx, r, GlobalSrc{Z}, n, dE, K, ldk);
   } else {
-    if (g.ndim() == 3) stream_k_nd<double, 16, 3, true>(st, g, X, ldx, r, Z, n, dE, K, ldk);
-    else stream_k_nd<double, 16, 2, true>(st, g, X, ldx, r, Z, n, dE, K, ldk);
+    if (g.ndim() == 3) stream_k_nd<double, 16, 3, true>(st, g, X, ldx, r, GlobalSrc{Z}, n, dE, K, ldk);
+    else stream_k_nd<double, 16, 2, true>(st, g, X, ldx, r, GlobalSrc{Z}, n, dE, K, ldk);
   }
   check_launch("stream_kt");
 }
