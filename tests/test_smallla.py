@@ -20,6 +20,7 @@ def lib():
         pytest.skip("host LA library not built")
     L = ctypes.CDLL(LIB)
     L.smallla_svd.argtypes = [ctypes.c_int, ctypes.c_int, P, P, P, P]
+    L.smallla_svd_left.argtypes = [ctypes.c_int, ctypes.c_int, P, P, P]
     L.smallla_range_basis.argtypes = [ctypes.c_int, ctypes.c_int, P, ctypes.c_int, ctypes.c_int, P,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                       PI, P, PI]
@@ -53,6 +54,33 @@ def test_svd_matches_numpy(lib, shape):
     np.testing.assert_allclose(s, s_ref, rtol=1e-12, atol=1e-15 * s_ref[0])
     np.testing.assert_allclose(U @ np.diag(s) @ V.T, A, atol=1e-13 * s_ref[0])
     np.testing.assert_allclose(V.T @ V, np.eye(V.shape[1]), atol=1e-13)
+
+
+def exactly_deficient(rng, m, n, rank):
+    """Exact zeros: zero columns, a repeated column and a zero block."""
+    A = rng.standard_normal((m, rank)) @ rng.standard_normal((rank, n))
+    A[:, 1] = 0.0
+    A[:, -1] = A[:, 0]
+    A[:, 2:4] = 0.0
+    return A
+
+
+@pytest.mark.parametrize("shape", [(40, 7), (136, 30), (12, 12), (5, 9)])
+def test_svd_left_vectors_do_not_depend_on_want_v(lib, shape):
+    """The adjoint replays a segment with right vectors requested; its bases
+    must equal the forward's (computed without them) bit for bit."""
+    rng = np.random.default_rng(shape[0] + shape[1])
+    m, n = shape
+    k = min(m, n)
+    for A in (rng.standard_normal(shape) * np.logspace(0, -14, n)[None, :],
+              exactly_deficient(rng, m, n, min(3, k)), np.zeros(shape)):
+        U, s, V = csvd(lib, A)
+        U2, s2 = f(np.zeros((m, k))), np.zeros(k)
+        assert lib.smallla_svd_left(m, n, dp(f(A)), dp(U2), dp(s2)) == 0
+        assert np.array_equal(U, U2) and np.array_equal(s, s2)
+        assert np.all(np.isfinite(V))
+        np.testing.assert_allclose(V.T @ V, np.eye(k), atol=1e-12)
+        np.testing.assert_allclose(U @ np.diag(s) @ V.T, A, atol=1e-13 * max(s[0], 1.0))
 
 
 def crange(lib, A, ncols, against=None, scale=-1.0, greedy=False, cand=None, tol=1e-7):

@@ -61,7 +61,7 @@ struct AdjointStore {
     return p;
   }
   ~AdjointStore() {
-    for (double* p : owned) hipFree(p);
+    for (double* p : owned) (void)hipFree(p);
   }
 };
 

@@ -124,8 +124,8 @@ struct Timing {
     pending.clear();
   }
   ~Timing() {
-    for (auto& r : pending) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
-    for (auto e : pool) hipEventDestroy(e);
+    for (auto& r : pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    for (auto e : pool) (void)hipEventDestroy(e);
   }
 };
 
@@ -256,10 +256,10 @@ struct titus_ctx {
   ~titus_ctx() {
     void* ptrs[] = {rho, irho, omega, bc_base, m0, psi, psi2, dose, u0p, X, Xn, K, Kc, Q1, Q2, partial, arena,
                     upl, bcells, cand_d, bmask, restore};
-    for (void* p : ptrs) if (p) hipFree(p);
+    for (void* p : ptrs) if (p) (void)hipFree(p);
     void* hptrs[] = {h_arena, h_upl};
-    for (void* p : hptrs) if (p) hipHostFree(p);
-    if (st) hipStreamDestroy(st);
+    for (void* p : hptrs) if (p) (void)hipHostFree(p);
+    if (st) (void)hipStreamDestroy(st);
   }
 
   // ---- timing ---------------------------------------------------------------

@@ -133,14 +133,14 @@ struct titus_fcs {
   Timing timing;
 
   ~titus_fcs() {
-    if (device >= 0) hipSetDevice(device);
-    if (st) hipStreamSynchronize(st);
+    if (device >= 0) (void)hipSetDevice(device);
+    if (st) (void)hipStreamSynchronize(st);
     void* ptrs[] = {rho, irho, ones, u, un, psi, psi2, dose, u0p, omega, bc_base, m0, Mp,
                     sig_d, one, pk, degp, bcells, start, mask, restore};
     for (void* p : ptrs)
-      if (p) hipFree(p);
+      if (p) (void)hipFree(p);
     timing.pending.clear();
-    if (st) hipStreamDestroy(st);
+    if (st) (void)hipStreamDestroy(st);
   }
 
   template <class F>
@@ -261,8 +261,8 @@ struct titus_fcs {
           });
           if (first || ms < best) { best = ms; plan = c; first = false; }
         }
-        hipEventDestroy(a);
-        hipEventDestroy(b);
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
         g_fcs_tuned[key] = {best_nbw, best_ct, best_ks, plan};
       }
     }

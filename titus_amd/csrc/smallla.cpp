@@ -164,37 +164,7 @@ double frob(const HMat& A) {
   return std::sqrt(s);
 }
 
-// Dot product with four independent partial sums (vectorisable; the sum
-// order is fixed, so results are reproducible).
-
 // ---------------------------------------------------------------------------
-// Householder QR of an m x n matrix, m >= n.  On return A holds R in its upper
-// triangle and the reflector vectors (unit leading entry implied) below.
-static void householder_qr(HMat& A, std::vector<double>& tau) {
-  const int m = A.m, n = A.n;
-  tau.assign(n, 0.0);
-  for (int j = 0; j < n; ++j) {
-    double* a = A.col(j);
-    const double xnorm2 = dot(a + j + 1, a + j + 1, m - j - 1);
-    const double alpha = a[j];
-    if (xnorm2 == 0.0) { tau[j] = 0.0; continue; }
-    const double nrm = std::sqrt(alpha * alpha + xnorm2);
-    const double beta = alpha >= 0.0 ? -nrm : nrm;
-    tau[j] = (beta - alpha) / beta;
-    const double scal = 1.0 / (alpha - beta);
-    for (int i = j + 1; i < m; ++i) a[i] *= scal;
-    a[j] = beta;
-    // apply H = I - tau v v^T to the trailing columns
-    for (int k = j + 1; k < n; ++k) {
-      double* b = A.col(k);
-      double w = b[j] + dot(a + j + 1, b + j + 1, m - j - 1);
-      w *= tau[j];
-      b[j] -= w;
-      for (int i = j + 1; i < m; ++i) b[i] -= w * a[i];
-    }
-  }
-}
-
 // Q * B for the reflectors stored in QR (m x n), B has m rows (in place).
 static void apply_q(const HMat& QR, const std::vector<double>& tau, HMat& B) {
   const int m = QR.m, n = QR.n;
@@ -260,7 +230,8 @@ static void jacobi_square(HMat& B, HMat& V, bool want_v) {
 
 // Householder QR with column pivoting (the remaining column of largest norm
 // first, ties to the lowest index): A P = Q R, perm[j] = original column of
-// column j.
+// column j.  On return A holds R in its upper triangle and the reflector
+// vectors (unit leading entry implied) below.
 static void householder_qrcp(HMat& A, std::vector<double>& tau, std::vector<int>& perm) {
   const int m = A.m, n = A.n;
   tau.assign(n, 0.0);
@@ -300,44 +271,18 @@ static void householder_qrcp(HMat& A, std::vector<double>& tau, std::vector<int>
   }
 }
 
-// Unpivoted form: Householder QR, one-sided Jacobi on R.  Used when the
-// preconditioned form meets an exactly zero singular value (its right vectors
-// would be undetermined there).
-static void svd_plain(const HMat& A0, HMat& U, std::vector<double>& s, HMat& V, bool want_v) {
-  const int m = A0.m, n = A0.n;
-  HMat QR = A0;
-  std::vector<double> tau;
-  householder_qr(QR, tau);
-  HMat R(n, n);
-  for (int j = 0; j < n; ++j)
-    for (int i = 0; i <= j; ++i) R(i, j) = QR(i, j);
-  HMat Vr;
-  jacobi_square(R, Vr, want_v);
-  std::vector<double> sv(n);
-  for (int j = 0; j < n; ++j) sv[j] = std::sqrt(dot(R.col(j), R.col(j), n));
-  std::vector<int> ord(n);
-  std::iota(ord.begin(), ord.end(), 0);
-  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return sv[a] > sv[b]; });
-  HMat Ur(m, n);   // padded to m rows for apply_q
-  V = want_v ? HMat(n, n) : HMat();
-  s.assign(n, 0.0);
-  for (int k = 0; k < n; ++k) {
-    const int j = ord[k];
-    s[k] = sv[j];
-    if (sv[j] > 0.0)
-      for (int i = 0; i < n; ++i) Ur(i, k) = R(i, j) / sv[j];
-    if (want_v)
-      for (int i = 0; i < n; ++i) V(i, k) = Vr(i, j);
-  }
-  apply_q(QR, tau, Ur);
-  U = Ur;
-}
-
 // Preconditioned one-sided Jacobi (Drmac and Veselic, SIAM J. Matrix Anal.
 // Appl. 29, 2008): A P = Q R with column pivoting, then one-sided Jacobi on
 // R^T, X = R^T J with orthogonal columns.  R = J Sigma Ut^T gives A = (Q J)
 // Sigma (P Ut)^T.  On the graded factors of a low-rank solve (rapidly
 // decaying singular values) it converges in 3-5 sweeps instead of 6-8.
+//
+// U and s do not depend on want_v (the same operations run either way): the
+// adjoint replays a solve segment from a checkpoint with want_v set, and its
+// bases must be the forward's to the last bit, since the adjoint state
+// handed across a checkpoint is expressed in the forward's basis.  A right
+// vector of an exactly zero singular value is undetermined; those columns of
+// V are an orthonormal completion of the others (their weight in A is zero).
 void svd(const HMat& A0, HMat& U, std::vector<double>& s, HMat& V, bool want_v) {
   if (A0.m < A0.n) {
     HMat Ut, Vt;
@@ -358,27 +303,48 @@ void svd(const HMat& A0, HMat& U, std::vector<double>& s, HMat& V, bool want_v) 
   jacobi_square(X, J, true);
   std::vector<double> sv(n);
   for (int j = 0; j < n; ++j) sv[j] = std::sqrt(dot(X.col(j), X.col(j), n));
-  if (want_v)
-    for (int j = 0; j < n; ++j)
-      if (sv[j] == 0.0) {
-        svd_plain(A0, U, s, V, want_v);
-        return;
-      }
   std::vector<int> ord(n);
   std::iota(ord.begin(), ord.end(), 0);
   std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return sv[a] > sv[b]; });
   HMat Ul(m, n);   // padded to m rows for apply_q
   V = want_v ? HMat(n, n) : HMat();
   s.assign(n, 0.0);
+  int nz = 0;      // columns with a nonzero singular value (they sort first)
   for (int k = 0; k < n; ++k) {
     const int j = ord[k];
     s[k] = sv[j];
     for (int i = 0; i < n; ++i) Ul(i, k) = J(i, j);
-    if (want_v)
-      for (int i = 0; i < n; ++i) V(perm[i], k) = X(i, j) / sv[j];
+    if (sv[j] > 0.0) {
+      ++nz;
+      if (want_v)
+        for (int i = 0; i < n; ++i) V(perm[i], k) = X(i, j) / sv[j];
+    }
   }
   apply_q(QR, tau, Ul);
   U = Ul;
+  if (want_v && nz < n) {
+    // completion: per column, the unit vector e_t with the largest part
+    // outside the span so far (the first on ties), projected twice, normalized
+    for (int k = nz; k < n; ++k) {
+      int best = 0;
+      double bn = -1.0;
+      for (int t = 0; t < n; ++t) {
+        double r = 1.0;
+        for (int c = 0; c < k; ++c) r -= V(t, c) * V(t, c);
+        if (r > bn) { bn = r; best = t; }
+      }
+      double* v = V.col(k);
+      std::fill(v, v + n, 0.0);
+      v[best] = 1.0;
+      for (int pass = 0; pass < 2; ++pass)
+        for (int c = 0; c < k; ++c) {
+          const double w = dot(V.col(c), v, n);
+          for (int i = 0; i < n; ++i) v[i] -= w * V(i, c);
+        }
+      const double nv = std::sqrt(dot(v, v, n));
+      for (int i = 0; i < n; ++i) v[i] /= nv;
+    }
+  }
 }
 
 bool cholesky_upper(const HMat& A, HMat& R) {

@@ -48,8 +48,9 @@ double frob(const HMat& A);
 // V is n x k, k = min(m, n), singular values descending.  Householder QR
 // followed by one-sided (Hestenes) Jacobi on the triangular factor: singular
 // values carry errors of order eps * s_1, singular vectors of order
-// eps * s_1 / gap, as LAPACK's gesvd.
-// V is left empty when want_v is false (m >= n only).
+// eps * s_1 / gap, as LAPACK's gesvd.  The QR is column-pivoted and Jacobi
+// runs on R^T (fewer sweeps on graded factors).  U and s are bit-identical
+// with and without want_v; V is left empty without it (m >= n only).
 void svd(const HMat& A, HMat& U, std::vector<double>& s, HMat& V, bool want_v = true);
 
 // Cholesky A = R^T R of a symmetric positive definite matrix; returns false

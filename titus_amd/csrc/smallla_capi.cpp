@@ -26,6 +26,21 @@ int smallla_svd(int m, int n, const double* A, double* U, double* s, double* V) 
   }
 }
 
+// The same SVD without right vectors (U m x k, s k).
+int smallla_svd_left(int m, int n, const double* A, double* U, double* s) {
+  try {
+    HMat Am(m, n), Um, Vm;
+    std::memcpy(Am.a.data(), A, sizeof(double) * m * n);
+    std::vector<double> sv;
+    titus::svd(Am, Um, sv, Vm, false);
+    std::memcpy(U, Um.a.data(), sizeof(double) * Um.a.size());
+    std::memcpy(s, sv.data(), sizeof(double) * sv.size());
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
 // range_basis (see smallla.h); against may be null (q = 0); candidates may be
 // null (all rows).  Writes m x ncols to out and the rank to *rank.
 int smallla_range_basis(int m, int k, const double* A, int ncols, int q, const double* against,

@@ -405,8 +405,8 @@ int titus_get_decisions(const titus_ctx* ctx, uint64_t* hash, int64_t* count) {
 
 void titus_destroy(titus_ctx* ctx) {
   if (!ctx) return;
-  hipSetDevice(ctx->device);
-  if (ctx->st) hipStreamSynchronize(ctx->st);
+  (void)hipSetDevice(ctx->device);
+  if (ctx->st) (void)hipStreamSynchronize(ctx->st);
   delete ctx;
 }
 
