@@ -232,6 +232,8 @@ struct titus_ctx {
   // pinned host mirrors
   double *h_arena = nullptr, *h_upl = nullptr;
   size_t arena_n = 0, arena_top = 0, upl_n = 0, upl_top = 0;
+  size_t fetched_top = 0;                   // arena prefix already copied to h_arena
+  hipEvent_t ev_part = nullptr;             // end of the last prefix copy
   dev::Grid g{};
   int nb = 0;
   // state
@@ -259,6 +261,7 @@ struct titus_ctx {
     for (void* p : ptrs) if (p) (void)hipFree(p);
     void* hptrs[] = {h_arena, h_upl};
     for (void* p : hptrs) if (p) (void)hipHostFree(p);
+    if (ev_part) (void)hipEventDestroy(ev_part);
     if (st) (void)hipStreamDestroy(st);
   }
 
@@ -283,14 +286,31 @@ struct titus_ctx {
     if (arena_top > arena_n) throw TitusError(TITUS_E_ARG, "result arena too small");
     return off;
   }
-  void areset() { arena_top = 0; }
-  // one transfer of everything the arena holds, then wait
+  void areset() { arena_top = fetched_top = 0; }
+  // one transfer of what the arena holds beyond the prefix already copied,
+  // then wait for the stream
   void fetch() {
-    if (arena_top)
-      HIPCHK(hipMemcpyAsync(h_arena, arena, arena_top * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (arena_top > fetched_top)
+      HIPCHK(hipMemcpyAsync(h_arena + fetched_top, arena + fetched_top,
+                            (arena_top - fetched_top) * sizeof(double), hipMemcpyDeviceToHost, st));
+    fetched_top = arena_top;
     HostScope hs(timing, "host:wait");
     HIPCHK(hipStreamSynchronize(st));
     upl_top = 0;          // every queued upload has been consumed
+  }
+  // the arena so far goes to the host behind the work queued until now;
+  // wait_part() waits for that copy only, so the host can use those results
+  // while later work runs on the device
+  void fetch_part() {
+    if (arena_top > fetched_top)
+      HIPCHK(hipMemcpyAsync(h_arena + fetched_top, arena + fetched_top,
+                            (arena_top - fetched_top) * sizeof(double), hipMemcpyDeviceToHost, st));
+    fetched_top = arena_top;
+    HIPCHK(hipEventRecord(ev_part, st));
+  }
+  void wait_part() {
+    HostScope hs(timing, "host:wait");
+    HIPCHK(hipEventSynchronize(ev_part));
   }
   HMat view(size_t off, int m, int n) const {
     HMat A(m, n);
@@ -836,26 +856,30 @@ struct titus_ctx {
       const int r = rank;
       areset();
       const size_t gx = galerkin_x(X, r, X, r);
+      fetch_part();
       const AWs aw = apply_a(W);
       stream_k(r, galerkin_from(W, aw));
       copy_x_to_aug(r);
       RangeJob j = range_launch(K, r, X, r);
-      fetch();
+      // the L-step (old X) and the new angular basis on the host while the
+      // device runs the K-step and its range job
+      wait_part();
       std::vector<HMat> XLX = galerkin_view(gx, r, r);
+      HMat W1 = augment_w(stream_L(aw, XLX));
+      HMat Nu = titus::matmul(W1, W, true, false);
+      fetch();
       range_finish(j, r, Xn + (size_t)r * C);               // Xn = X1 = [X | Xt]
-      // products of X1 on the device while the host augments W
+      // products of X1 on the device while the host forms W1^T A W1
       areset();
       const size_t mu = tn(Xn, 2 * r, X, r, dev::OP_NONE);
       const size_t g1 = galerkin_x(Xn, 2 * r, Xn, 2 * r);
-      HMat W1 = augment_w(stream_L(aw, XLX));
-      HMat Nu = titus::matmul(W1, W, true, false);
+      // A W1 = [A W, A Wt]
+      std::vector<HMat> WAW =
+          galerkin_from(W1, hcat_all(aw, apply_a(titus::cols(W1, r, W1.n))));
       fetch();
       HMat Mu = view(mu, 2 * r, r);
       XLX = galerkin_view(g1, 2 * r, 2 * r);
       HMat S1 = titus::matmul(titus::matmul(Mu, S), Nu, false, true);   // 2r x 2r
-      // A W1 = [A W, A Wt]
-      std::vector<HMat> WAW =
-          galerkin_from(W1, hcat_all(aw, apply_a(titus::cols(W1, r, W1.n))));
       HMat Sup = stream_S(S1, XLX, WAW);
       // eta diagnostic (:2210-2220): rows r.. of X1^T L X1, columns r.. of W1^T A W1
       HMat acc(r, r);
@@ -875,18 +899,16 @@ struct titus_ctx {
     const int r = rank;
     areset();
     const size_t xtp = tn_psi(X, r);
+    fetch_part();
     out_scatter(Dout);
     // in-scattering (:2252-2306)
     scatter_K(mtdw(Dvec, W));
     copy_x_to_aug(r);
     RangeJob j = range_launch(K, r, X, r);
-    fetch();
+    // the L-step and the new angular basis on the host while the device runs
+    // the scattering and the K range job
+    wait_part();
     HMat Xtpsi = view(xtp, r, nq);
-    range_finish(j, r, Xn + (size_t)r * C);
-    // products of X1 on the device while the host augments W
-    areset();
-    const size_t mu = tn(Xn, 2 * r, X, r, dev::OP_NONE);
-    const size_t x1p = tn_psi(Xn, 2 * r);
     HMat L = titus::matmul(W, S, false, true);
     HMat MX = titus::matmul(M, Xtpsi, false, true);
     for (int c = 0; c < r; ++c)
@@ -894,9 +916,16 @@ struct titus_ctx {
     HMat W1 = augment_w(L);
     HMat Nu = titus::matmul(W1, W, true, false);
     fetch();
+    range_finish(j, r, Xn + (size_t)r * C);
+    // products of X1 on the device while the host forms M^T D W1
+    areset();
+    const size_t mu = tn(Xn, 2 * r, X, r, dev::OP_NONE);
+    const size_t x1p = tn_psi(Xn, 2 * r);
+    const HMat MtDW1 = mtdw(Dvec, W1);
+    fetch();
     HMat Mu = view(mu, 2 * r, r);
     HMat S1 = titus::matmul(titus::matmul(Mu, S), Nu, false, true);
-    HMat add = titus::matmul(view(x1p, 2 * r, nq), mtdw(Dvec, W1));
+    HMat add = titus::matmul(view(x1p, 2 * r, nq), MtDW1);
     for (size_t i = 0; i < S1.a.size(); ++i) S1.a[i] += dE * add.a[i];
     truncate_to(S1, Xn, W1);
     dose_half(stopping[s]);

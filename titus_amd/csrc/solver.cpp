@@ -111,6 +111,7 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     x.device = device;
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&x.st, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&x.ev_part, hipEventDisableTiming));
     x.nx = p->nx; x.ny = p->ny; x.C = C; x.nm = p->nm; x.nq = p->nq; x.ne = p->ne;
     x.nsig = p->nsigma; x.integrator = p->integrator; x.precision = p->precision;
     x.mixed = p->precision != TITUS_PRECISION_FP64;
