@@ -740,21 +740,26 @@ struct titus_ctx {
       // K-step and old Galerkin products (SolverCSD.jl:1019-1038)
       areset();
       const size_t gx = galerkin_x(X, r, X, r);
+      fetch_part();
       const AWs aw = apply_a(W);
       const std::vector<HMat> WAW = galerkin_from(W, aw);
       stream_k(r, WAW);
       RangeJob j = range_launch(K, r, nullptr, 0);
-      fetch();
+      // the L-step (:1033-1046) on the host while the device runs the K-step
+      // and its range job
+      wait_part();
       std::vector<HMat> XLX = galerkin_view(gx, r, r);
-      range_finish(j, r, Xn, tape ? &tape->Rx1 : nullptr);
-      // the device writes XNew and its products while the host runs the
-      // L-step (:1033-1046)
-      areset();
-      const size_t mu = tn(Xn, r, X, r, dev::OP_NONE);
-      const size_t gn = galerkin_x(Xn, r, Xn, r);
       HMat L = stream_L(aw, XLX);
       HMat WNew = range_w(L, r, nullptr, -1.0, tape ? &tape->Rw1 : nullptr);
       HMat NUp = titus::matmul(WNew, W, true, false);
+      fetch();
+      range_finish(j, r, Xn, tape ? &tape->Rx1 : nullptr);
+      // the device writes XNew and its products while the host forms the
+      // new W's Galerkin products
+      areset();
+      const size_t mu = tn(Xn, r, X, r, dev::OP_NONE);
+      const size_t gn = galerkin_x(Xn, r, Xn, r);
+      const std::vector<HMat> WAW1 = galerkin_w(WNew, WNew);
       fetch();
       HMat MUp = view(mu, r, r);
       if (tape) {
@@ -768,9 +773,9 @@ struct titus_ctx {
       // device during the host S-step and out-scattering
       areset();
       xtp0 = tn_psi(X, r);
+      fetch_part();
       // S-step (:1051-1063)
       S = titus::matmul(titus::matmul(MUp, S), NUp, false, true);
-      const std::vector<HMat> WAW1 = galerkin_w(W, W);
       if (tape) {
         tape->S1 = S; tape->XLX1 = XLX; tape->WAW1 = WAW1;
         HIPCHK(hipMemcpyAsync(tape->X1, X, cr, hipMemcpyDeviceToDevice, st));
@@ -779,19 +784,16 @@ struct titus_ctx {
     } else {
       areset();
       xtp0 = tn_psi(X, r);
+      fetch_part();
     }
     out_scatter(Dout);
     // in-scattering (:1079-1102)
     const HMat MtDW = mtdw(Dvec, W);
     scatter_K(MtDW);
     RangeJob j = range_launch(K, r, nullptr, 0);
-    fetch();
+    // the new W on the host while the device runs the K range job
+    wait_part();
     HMat PtX = titus::transpose(view(xtp0, r, nq));
-    range_finish(j, r, Xn, tape ? &tape->Rx3 : nullptr);
-    // products of XNew on the device while the host builds the new W
-    areset();
-    const size_t mu = tn(Xn, r, X, r, dev::OP_NONE);
-    const size_t xtp = tn_psi(Xn, r);
     HMat L = titus::matmul(W, S, false, true);
     HMat MP = titus::matmul(M, PtX);
     for (int c = 0; c < r; ++c)
@@ -799,12 +801,18 @@ struct titus_ctx {
     HMat WNew = range_w(L, r, nullptr, -1.0, tape ? &tape->Rw3 : nullptr);
     HMat NUp = titus::matmul(WNew, W, true, false);
     fetch();
+    range_finish(j, r, Xn, tape ? &tape->Rx3 : nullptr);
+    // products of XNew on the device while the host forms M^T D WNew
+    areset();
+    const size_t mu = tn(Xn, r, X, r, dev::OP_NONE);
+    const size_t xtp = tn_psi(Xn, r);
+    const HMat B3 = mtdw(Dvec, WNew);
+    fetch();
     HMat MUp = view(mu, r, r);
     HMat XtP = view(xtp, r, nq);
     W = WNew;
     std::swap(X, Xn);
     S = titus::matmul(titus::matmul(MUp, S), NUp, false, true);
-    const HMat B3 = mtdw(Dvec, W);
     HMat add = titus::matmul(XtP, B3);
     for (size_t i = 0; i < S.a.size(); ++i) S.a[i] += dE * add.a[i];
     if (tape) {
@@ -957,20 +965,24 @@ struct titus_ctx {
       const int r = rank;
       areset();
       const size_t gx = galerkin_x(X, r, X, r);
+      fetch_part();
       const AWs aw = apply_a(W);
       const std::vector<HMat> WAW = galerkin_from(W, aw);
       stream_k(r, WAW);                                   // K (:1843-1846)
       copy_x_to_aug(r);
       HIPCHK(hipMemcpyAsync(Kc, K, sizeof(double) * (size_t)C * r, hipMemcpyDeviceToDevice, st));
       RangeJob j = range_launch(K, r, X, r);              // K is left projected off X
-      fetch();
+      // L- and S-steps from the old factors on the host while the device runs
+      // the K range job
+      wait_part();
       std::vector<HMat> XLX = galerkin_view(gx, r, r);
-      range_finish(j, r, Xn + (size_t)r * C);             // Xn = [X | Xt]
-      areset();
-      const size_t xk = tn(Xn + (size_t)r * C, r, Kc, r, dev::OP_NONE);  // Xt^T K
       HMat L = stream_L(aw, XLX);                          // (:1850-1854)
       HMat W1 = augment_w(L);
       HMat Sn = stream_S(S, XLX, WAW);                     // old Galerkin (:1859)
+      fetch();
+      range_finish(j, r, Xn + (size_t)r * C);             // Xn = [X | Xt]
+      areset();
+      const size_t xk = tn(Xn + (size_t)r * C, r, Kc, r, dev::OP_NONE);  // Xt^T K
       fetch();
       truncate_to(augmented_s(Sn, view(xk, r, r), L, W1), Xn, W1);
     }
@@ -978,6 +990,7 @@ struct titus_ctx {
     const int r = rank;
     areset();
     const size_t xtp = tn_psi(X, r);
+    fetch_part();
     out_scatter(Dout);
     // in-scattering (:1900-1940)
     const HMat MtDW = mtdw(Dvec, W);
@@ -985,11 +998,9 @@ struct titus_ctx {
     copy_x_to_aug(r);
     HIPCHK(hipMemcpyAsync(Kc, K, sizeof(double) * (size_t)C * r, hipMemcpyDeviceToDevice, st));
     RangeJob j = range_launch(K, r, X, r);
-    fetch();
+    // L- and S-steps on the host while the device runs the K range job
+    wait_part();
     const HMat Xtpsi = view(xtp, r, nq);
-    range_finish(j, r, Xn + (size_t)r * C);
-    areset();
-    const size_t xk = tn(Xn + (size_t)r * C, r, Kc, r, dev::OP_NONE);
     HMat L = titus::matmul(W, S, false, true);
     const HMat MX = titus::matmul(M, Xtpsi, false, true);
     for (int c = 0; c < r; ++c)
@@ -997,6 +1008,10 @@ struct titus_ctx {
     HMat W1 = augment_w(L);
     const HMat add = titus::matmul(Xtpsi, MtDW);
     for (size_t i = 0; i < S.a.size(); ++i) S.a[i] += dE * add.a[i];
+    fetch();
+    range_finish(j, r, Xn + (size_t)r * C);
+    areset();
+    const size_t xk = tn(Xn + (size_t)r * C, r, Kc, r, dev::OP_NONE);
     fetch();
     const HMat Sn = augmented_s(S, view(xk, r, r), L, W1);
     dose_half(stopping[s]);                                // before the truncation (:1944-1950)
