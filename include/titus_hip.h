@@ -135,7 +135,8 @@ int titus_solve(struct titus_ctx* ctx, int32_t last_step);
  * the end of their energy grids, `nthreads` host workers at a time (<= 0:
  * one per hardware thread, at most n).  Each context keeps its own stream, so
  * the beamlets' kernels and host algebra overlap.  status[i] (optional)
- * receives each solve's return code; the first failure is returned.
+ * receives each solve's return code; the first failure is returned.  A
+ * context may appear only once (TITUS_E_ARG otherwise, before any solve).
  * No reference counterpart: the reference solves one beam per call
  * (code/SolverCSD.jl:891, 2011). */
 int titus_solve_batch(struct titus_ctx** ctxs, int32_t n, int32_t nthreads, int32_t* status);

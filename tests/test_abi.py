@@ -70,3 +70,21 @@ def test_create_reports_missing_device_or_bad_args_without_compute():
     h = ctypes.c_void_p()
     assert _lib.load().titus_create(ctypes.byref(h), ctypes.byref(st), 0) == -1
     assert b"version" in _lib.load().titus_last_error()
+
+
+def test_batches_refuse_a_repeated_context():
+    """Two host threads must never drive one context: a batch naming a
+    context twice is refused before any context is touched."""
+    lib = _lib.load()
+    dummy = ctypes.create_string_buffer(64)               # never dereferenced
+    ptrs = (ctypes.c_void_p * 2)(ctypes.addressof(dummy), ctypes.addressof(dummy))
+    status = (ctypes.c_int32 * 2)()
+    ctx_array = ctypes.cast(ptrs, ctypes.POINTER(ctypes.c_void_p))
+    assert lib.titus_solve_batch(ctx_array, 2, 2, status) == -1
+    assert b"twice" in lib.titus_last_error()
+    P = ctypes.POINTER(ctypes.c_double)
+    bufs = [(ctypes.c_double * 4)() for _ in range(2)]
+    bars = (P * 2)(*[ctypes.cast(b, P) for b in bufs])
+    grads = (P * 2)(*[ctypes.cast(b, P) for b in bufs])
+    assert lib.titus_adjoint_batch(ctx_array, 2, 2, bars, grads, status) == -1
+    assert b"twice" in lib.titus_last_error()

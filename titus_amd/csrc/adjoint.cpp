@@ -486,6 +486,10 @@ int titus_adjoint_batch(titus_ctx** ctxs, int32_t n, int32_t nthreads, const dou
     g_last_error = "bad argument";
     return TITUS_E_ARG;
   }
+  if (batch_has_duplicates(ctxs, n)) {
+    g_last_error = "a context appears twice in the batch";
+    return TITUS_E_ARG;
+  }
   int nt = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
   nt = std::max(1, std::min<int>(nt, n));
   std::vector<int> codes(n, TITUS_OK);

@@ -62,6 +62,16 @@ struct TitusError : std::runtime_error {
       throw TitusError(TITUS_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// True when a context appears more than once in a batch (two host threads
+// would then drive one context and its stream).
+inline bool batch_has_duplicates(titus_ctx* const* ctxs, int n) {
+  std::vector<const void*> v(ctxs, ctxs + n);
+  std::sort(v.begin(), v.end());
+  for (int i = 1; i < n; ++i)
+    if (v[i] && v[i] == v[i - 1]) return true;
+  return false;
+}
+
 template <class T>
 T* dalloc(size_t n) {
   void* p = nullptr;
