@@ -74,6 +74,17 @@ def test_fcs_p21_several_groups():
     check(g, fullpn.solve_fcs(P, steps=20), "fcs P21")
 
 
+def test_fcs_tuned_plan():
+    """A grid large enough (cells x moments >= 2^20) for the creation-time
+    tuner: whichever kernel plan it picks is checked against the oracle."""
+    P = cases.phantom_fcs(n=72, N=21, seed=2)
+    announce("fcs phantom 72x72 P21, tuned", P)
+    assert P["C"] * P["Nm"] >= 2 ** 20
+    g = run_fcs(P, 6)
+    assert g["plan"]["tuned"]
+    check(g, fullpn.solve_fcs(P, steps=6), "fcs tuned")
+
+
 def test_fcs_full_moment_set():
     """(N+1)^2 moments: sine harmonics as separate blocks of the plan."""
     P = cases.small_fcs(n=16, N=5, moments="full")
