@@ -107,3 +107,18 @@ def test_3d_tuned_sweep():
     o = dlra.solve_fixed(P, steps=3)
     assert rel(g["psi"], o["psi"]) <= 1e-13
     assert rel(g["dose"], o["dose"]) <= 1e-10
+
+
+@pytest.mark.parametrize("integ,r", [(INTEGRATOR_FIXED, 1), (INTEGRATOR_BUG, 2)])
+def test_3d_smallest_cube_lowest_order(integ, r):
+    """A 5^3 grid (27 interior cells) with P_1 (four moments): rank 1 fixed,
+    and the BUG with cap 2 (augmented bases of four columns = all moments)."""
+    P = broad_cube(n=5, N=1, r=r, integ=integ, seed=4)
+    announce(f"3d 5^3 P_1, integrator {integ}, r {r}", P)
+    g = run(P, integ, 20)
+    o = dlra.solve_bug(P, steps=20) if integ == INTEGRATOR_BUG else dlra.solve_fixed(P, steps=20)
+    ed = rel(g["dose"], o["dose"])
+    print(f"[3d smallest] ranks {list(g['ranks'])} dose {ed:.2e}", flush=True)
+    assert np.abs(o["dose"]).max() > 0 and ed <= 1e-10
+    if integ == INTEGRATOR_BUG:
+        assert np.array_equal(g["ranks"], np.asarray(o["ranks"])[:len(g["ranks"])])
