@@ -61,7 +61,7 @@ enum { TITUS_PRECISION_FP64 = 0, TITUS_PRECISION_MIXED = 1, TITUS_PRECISION_MIXE
  * ordinate reduction and initial factors of the Solve* functions
  * (code/SolverCSD.jl:63-146, 904-961, 2025-2082). */
 typedef struct titus_problem_v1 {
-  int32_t version;            /* TITUS_ABI_VERSION (1 and 2 accepted: fields they lack default) */
+  int32_t version;            /* TITUS_ABI_VERSION (1..3 accepted: fields they lack default) */
   int32_t nx, ny;             /* NCellsX, NCellsY */
   double dx, dy;              /* settings.dx, settings.dy */
   const double* density;      /* C = nx*ny*nz, densityVec (SolverCSD.jl:143) */
@@ -120,8 +120,18 @@ typedef struct titus_problem_v1 {
   int32_t flux_order;
 } titus_problem_v1;
 
-/* SolverCSD(settings): validate, copy operands to device `device`. */
+/* SolverCSD(settings): validate, copy operands to device `device`.
+ * Refused with TITUS_E_ARG (before any device call): negative counts, a
+ * missing operand (restore_cells when nrestore > 0 included), a density that
+ * is not positive and finite, a non-finite initial flux, cell indices out of
+ * range, a non-positive energy step. */
 int titus_create(struct titus_ctx** out, const titus_problem_v1* problem, int32_t device);
+
+/* The explicit scheme's CFL number dE / min(dx, dy[, dz]) * max(1 / rho),
+ * which the reference prints before its energy loop (SolverCSD.jl:963, 2084).
+ * Above 1 the explicit upwind/Roe update can grow without bound; the library
+ * still runs (as the reference does) and a caller decides whether to warn. */
+int titus_get_cfl(const struct titus_ctx* ctx, double* cfl);
 
 /* One iteration `for n = ...` of the energy loop (SolverCSD.jl:972-1114 /
  * 2102-2369).  Steps run in order 1..ne-1; `step` must be the next one. */
@@ -200,8 +210,10 @@ int titus_set_checkpoints(struct titus_ctx* ctx, int32_t every);
 /* Gradient of <dose_bar, dose> with respect to the density (C values each):
  * d/d rho_c.  Recomputes each checkpointed segment once with its intermediates
  * kept, then runs it backwards; the forward state (dose, factors, psi) is
- * unchanged afterwards.  TITUS_E_STATE without checkpoints or steps.  The
- * rrule of a dose objective J(dose) calls it with dose_bar = dJ/d dose. */
+ * unchanged afterwards, also when the call fails (TITUS_E_NONFINITE when the
+ * reverse pass overflows).  TITUS_E_STATE without checkpoints or steps;
+ * TITUS_E_ARG for a non-finite dose_bar.  The rrule of a dose objective
+ * J(dose) calls it with dose_bar = dJ/d dose. */
 int titus_adjoint(struct titus_ctx* ctx, const double* dose_bar, double* grad_density);
 
 /* titus_adjoint over n contexts on up to nthreads host threads (as

@@ -88,3 +88,85 @@ def test_batches_refuse_a_repeated_context():
     grads = (P * 2)(*[ctypes.cast(b, P) for b in bufs])
     assert lib.titus_adjoint_batch(ctx_array, 2, 2, bars, grads, status) == -1
     assert b"twice" in lib.titus_last_error()
+
+
+def _refused(mutate, expect):
+    """titus_create refuses the mutated problem with TITUS_E_ARG before any
+    device call, naming `expect` in its message."""
+    import numpy as np
+    import cases
+    P = cases.small_fixed(n=12, N=3, r=2, steps=4)
+    st, keep = _lib.make_problem(P, _lib.INTEGRATOR_FIXED)
+    extra = mutate(st, keep, np)
+    h = ctypes.c_void_p()
+    code = _lib.load().titus_create(ctypes.byref(h), ctypes.byref(st), 0)
+    msg = _lib.load().titus_last_error().decode()
+    del extra
+    assert code == -1, (code, msg)
+    assert expect in msg, msg
+    assert not h.value
+
+
+def _set_array(st, keep, np, name, key, fn):
+    a = keep[key].copy()
+    fn(a)
+    keep[key + "_mut"] = a
+    setattr(st, name, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+
+
+@pytest.mark.parametrize("case", ["zero_density", "negative_density", "nan_density", "inf_density"])
+def test_create_refuses_bad_density(case):
+    val = {"zero_density": 0.0, "negative_density": -1.0, "nan_density": float("nan"),
+           "inf_density": float("inf")}[case]
+
+    def mutate(st, keep, np):
+        _set_array(st, keep, np, "density", "density", lambda a: a.__setitem__(7, val))
+    _refused(mutate, "density")
+
+
+def test_create_refuses_nonfinite_initial_flux():
+    def mutate(st, keep, np):
+        psi0 = np.zeros((st.nx * st.ny, st.nq), order="F")
+        psi0[5, 0] = float("nan")
+        keep["psi0_mut"] = psi0
+        st.psi0 = psi0.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        return psi0
+    _refused(mutate, "initial flux")
+
+
+@pytest.mark.parametrize("field", ["nb", "nrestore", "ncand"])
+def test_create_refuses_negative_counts(field):
+    def mutate(st, keep, np):
+        setattr(st, field, -3)
+    _refused(mutate, "negative")
+
+
+def test_create_refuses_missing_restore_cells():
+    def mutate(st, keep, np):
+        st.restore_cells = ctypes.POINTER(ctypes.c_int32)()
+    _refused(mutate, "restore_cells")
+
+
+def test_create_refuses_bad_energy_step_and_cells():
+    def step(st, keep, np):
+        st.dE = float("nan")
+    _refused(step, "energy step")
+
+    def cell(st, keep, np):
+        a = keep["bcells"].copy()
+        a[0] = st.nx * st.ny
+        keep["bc_mut"] = a
+        st.bcells = a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    _refused(cell, "ghost cell")
+
+
+def test_cfl_number_of_the_reference_grids():
+    """The CFL number the library reports is the one the reference prints,
+    dE / min(dx, dy) * max(1 / rho) (SolverCSD.jl:963); checked here on the
+    host formula for the phantom and slab (the library returns the same
+    value after titus_create, covered by the gpu tests)."""
+    import numpy as np
+    import cases
+    P = cases.phantom_bug(n=64, N=5, cap=8)
+    cfl = P["dE"] / min(P["dx"], P["dy"]) * np.max(1.0 / P["density"])
+    assert 0.1 < cfl < 1.0

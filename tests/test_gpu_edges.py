@@ -134,3 +134,68 @@ def test_full_solver_smallest_grid():
     ed = rel(g["dose"], o["dose"])
     print(f"[fcs smallest] dose rel {ed:.3e}", flush=True)
     assert ed <= TOL and rel(g["u"], o["u"]) <= TOL
+
+
+def test_nonfinite_state_is_reported():
+    """A beam source scaled to 1e300 overflows the state: the solve stops with
+    TITUS_E_NONFINITE instead of carrying inf/NaN on (every loop of the step
+    is bounded, so the overflow only propagates)."""
+    P = dict(beam(24, 3, 2, steps=6))
+    P["bc_scale"] = np.asarray(P["bc_scale"]) * 1e300
+    announce("overflowing beam source", P)
+    ctx = _lib.Context(P, INTEGRATOR_FIXED)
+    try:
+        with pytest.raises(_lib.TitusError) as e:
+            ctx.solve(0)
+        assert e.value.code == -3, e.value
+    finally:
+        ctx.close()
+
+
+def test_cfl_number_matches_the_reference_formula():
+    P = beam(24, 3, 2, steps=4)
+    ctx = _lib.Context(P, INTEGRATOR_FIXED)
+    try:
+        cfl = ctx.cfl()
+    finally:
+        ctx.close()
+    ref = P["dE"] / min(P["dx"], P["dy"]) * np.max(1.0 / P["density"])
+    assert abs(cfl - ref) <= 1e-14 * ref
+
+
+def test_failed_adjoint_leaves_the_forward_state():
+    """An adjoint whose reverse pass overflows (seed 1e308) fails with
+    TITUS_E_NONFINITE and leaves dose, factors, psi and the step count as
+    they were; a NaN seed is refused up front; a later adjoint on the same
+    context equals one on a fresh context bit for bit."""
+    P = beam(24, 5, 6, steps=14)
+    announce("adjoint failure and restore", P)
+    seed = np.random.default_rng(2).standard_normal(P["C"])
+    ctx = _lib.Context(P, INTEGRATOR_FIXED)
+    ref = _lib.Context(P, INTEGRATOR_FIXED)
+    try:
+        for c in (ctx, ref):
+            c.set_checkpoints(4)
+            c.solve(12)
+        before = ctx.state()
+        with pytest.raises(_lib.TitusError) as e:
+            ctx.adjoint(np.full(P["C"], 1e308))
+        assert e.value.code == -3, e.value
+        after = ctx.state()
+        for k in ("dose", "X", "S", "W", "psi"):
+            assert np.array_equal(before[k], after[k]), k
+        assert ctx.progress() == ref.progress()
+        bad = seed.copy()
+        bad[3] = np.nan
+        with pytest.raises(_lib.TitusError) as e:
+            ctx.adjoint(bad)
+        assert e.value.code == -1
+        g1, g2 = ctx.adjoint(seed), ref.adjoint(seed)
+        print(f"[adjoint restore] |g| {np.linalg.norm(g2):.3e}", flush=True)
+        assert np.array_equal(g1, g2)
+        ctx.step(13)
+        ref.step(13)
+        assert np.array_equal(ctx.state()["dose"], ref.state()["dose"])
+    finally:
+        ctx.close()
+        ref.close()

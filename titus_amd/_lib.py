@@ -5,6 +5,7 @@ loudly when it is missing: there is no CPU fallback for the energy step.
 """
 import ctypes
 import os
+import warnings
 
 import numpy as np
 
@@ -24,7 +25,7 @@ ERRORS = {
 }
 
 # every symbol include/titus_hip.h declares
-EXPORTED = ("titus_create", "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
+EXPORTED = ("titus_create", "titus_get_cfl", "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
             "titus_get_state",
             "titus_get_history", "titus_set_timing", "titus_kernel_stats", "titus_get_tuning",
             "titus_get_decisions", "titus_set_checkpoints", "titus_adjoint", "titus_adjoint_batch",
@@ -90,6 +91,7 @@ def load():
     vp = ctypes.c_void_p
     lib.titus_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(ProblemV1), ctypes.c_int32]
     lib.titus_step.argtypes = [vp, ctypes.c_int32]
+    lib.titus_get_cfl.argtypes = [vp, _P]
     lib.titus_solve.argtypes = [vp, ctypes.c_int32]
     lib.titus_solve_batch.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32, _PI]
     lib.titus_progress.argtypes = [vp, _PI, _PI]
@@ -119,7 +121,7 @@ def load():
     lib.titus_fcs_destroy.restype = None
     lib.titus_last_error.restype = ctypes.c_char_p
     lib.titus_abi_version.restype = ctypes.c_int32
-    for name in ("titus_create", "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
+    for name in ("titus_create", "titus_get_cfl", "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
                  "titus_get_state", "titus_get_history", "titus_set_timing",
                  "titus_kernel_stats", "titus_get_tuning", "titus_get_decisions",
                  "titus_set_checkpoints",
@@ -268,6 +270,17 @@ class Context:
         check(lib.titus_create(ctypes.byref(h), ctypes.byref(st), device))
         del keep
         self.h = h
+        cfl = self.cfl()
+        if cfl > 1.0:
+            warnings.warn(f"CFL = {cfl:.3g} > 1: the explicit streaming update may be unstable "
+                          "(the reference prints this number, SolverCSD.jl:963)", RuntimeWarning,
+                          stacklevel=2)
+
+    def cfl(self):
+        """dE / min(dx, dy[, dz]) * max(1 / density) of the problem."""
+        v = ctypes.c_double()
+        check(self._lib.titus_get_cfl(self.h, ctypes.byref(v)))
+        return v.value
 
     def step(self, s):
         check(self._lib.titus_step(self.h, s))

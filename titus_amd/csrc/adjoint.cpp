@@ -359,6 +359,8 @@ void run_adjoint(titus_ctx& x, const double* dose_bar, double* grad) {
   if (x.steps_done < 1) throw TitusError(TITUS_E_STATE, "adjoint: nothing solved yet");
   if (a.cps.empty() || !a.cps.front().saved)
     throw TitusError(TITUS_E_STATE, "adjoint: no checkpoint before the first step");
+  for (int c = 0; c < x.C; ++c)
+    if (!std::isfinite(dose_bar[c])) throw TitusError(TITUS_E_ARG, "adjoint: dose_bar must be finite");
   const int C = x.C, r = x.rank, nq = x.nq;
   const size_t cr = (size_t)C * r, cq = (size_t)C * nq;
   // buffers (first call)
@@ -402,6 +404,21 @@ void run_adjoint(titus_ctx& x, const double* dose_bar, double* grad) {
   const int64_t decisions_was = x.decisions;
   const bool timing_was = x.timing.enabled;
   x.timing.enabled = false;
+  // puts the forward state back as it was before the call: device blocks
+  // from their copies, host factors, counters; runs on success and failure
+  auto restore = [&] {
+    x.tape = nullptr;
+    x.timing.enabled = timing_was;
+    x.decision_hash = hash_was;
+    x.decisions = decisions_was;
+    HIPCHK(hipMemcpyAsync(x.X, a.Xsave, sizeof(double) * cr, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(x.psi, a.psisave, sizeof(double) * cq, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(x.dose, a.dosesave, sizeof(double) * C, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(x.u0p, a.u0save, sizeof(double) * cu, hipMemcpyDeviceToDevice, st));
+    x.S = S_end;
+    x.W = W_end;
+    x.steps_done = done;
+  };
   Reverse rev(x, a);
   try {
     for (int ci = (int)a.cps.size() - 1; ci >= 0; --ci) {
@@ -420,31 +437,31 @@ void run_adjoint(titus_ctx& x, const double* dose_bar, double* grad) {
         x.step_fixed(s);
         x.tape = nullptr;
       }
-      for (int s = s1; s >= cp.s; --s) rev.step(a.tapes[s - cp.s]);
+      for (int s = s1; s >= cp.s; --s) {
+        rev.step(a.tapes[s - cp.s]);
+        // the host-side adjoint coefficients carry every reverse step
+        for (double v : rev.Sb.a)
+          if (!std::isfinite(v)) throw TitusError(TITUS_E_NONFINITE, "adjoint: non-finite coefficients");
+        for (double v : rev.Wb.a)
+          if (!std::isfinite(v)) throw TitusError(TITUS_E_NONFINITE, "adjoint: non-finite coefficients");
+      }
     }
   } catch (...) {
-    x.tape = nullptr;
-    x.timing.enabled = timing_was;
-    x.decision_hash = hash_was;
-    x.decisions = decisions_was;
+    // a failed HIP call may leave the stream unusable: the restore is then
+    // best effort and the first error is the one reported
+    try {
+      restore();
+      (void)hipStreamSynchronize(st);
+    } catch (...) {
+    }
     throw;
   }
-  x.timing.enabled = timing_was;
-  x.decision_hash = hash_was;
-  x.decisions = decisions_was;
   // d/drho = -y^2 d/dy
   std::vector<double> yb(C), rho(C);
   HIPCHK(hipMemcpyAsync(yb.data(), a.ybar, sizeof(double) * C, hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(rho.data(), x.rho, sizeof(double) * C, hipMemcpyDeviceToHost, st));
-  // restore the forward state
-  HIPCHK(hipMemcpyAsync(x.X, a.Xsave, sizeof(double) * cr, hipMemcpyDeviceToDevice, st));
-  HIPCHK(hipMemcpyAsync(x.psi, a.psisave, sizeof(double) * cq, hipMemcpyDeviceToDevice, st));
-  HIPCHK(hipMemcpyAsync(x.dose, a.dosesave, sizeof(double) * C, hipMemcpyDeviceToDevice, st));
-  HIPCHK(hipMemcpyAsync(x.u0p, a.u0save, sizeof(double) * cu, hipMemcpyDeviceToDevice, st));
+  restore();
   HIPCHK(hipStreamSynchronize(st));
-  x.S = S_end;
-  x.W = W_end;
-  x.steps_done = done;
   for (int c = 0; c < C; ++c) {
     if (!std::isfinite(yb[c])) throw TitusError(TITUS_E_NONFINITE, "adjoint: non-finite gradient");
     grad[c] = -yb[c] / (rho[c] * rho[c]);
@@ -495,9 +512,7 @@ int titus_adjoint_batch(titus_ctx** ctxs, int32_t n, int32_t nthreads, const dou
   std::vector<int> codes(n, TITUS_OK);
   std::vector<std::string> msgs(n);
   std::atomic<int> next{0};
-  if (nt > 1)
-    for (int i = 0; i < n; ++i)
-      if (ctxs[i]) ctxs[i]->host_parallel = false;
+  HostParallelOff hp(ctxs, n, nt > 1);
   auto worker = [&] {
     for (;;) {
       const int i = next.fetch_add(1);

@@ -205,6 +205,7 @@ struct titus_ctx {
   int nx = 0, ny = 0, C = 0, nm = 0, nq = 0, ne = 0, nsig = 0, integrator = 0, precision = 0;
   double dx = 0, dy = 0, dE = 0, theta = 0, theta_index = 0, rank_tol = 1e-7;
   int rank_cap = 0, capx = 0;
+  double cfl = 0.0;           // dE / min(h) * max(1 / rho) (SolverCSD.jl:963)
   std::vector<double> stopping, sigma, bc_scale;
   std::vector<int> degree, cand;
   HMat Ax, Az, aAx, aAz, M;
@@ -1047,6 +1048,24 @@ struct titus_ctx {
 
 // ---------------------------------------------------------------------------
 namespace titus_host {
+
+// Turns the per-operator host threads of a batch's contexts off for the
+// batch's lifetime and restores each context's own setting afterwards (also
+// when a solve fails).
+struct HostParallelOff {
+  std::vector<std::pair<titus_ctx*, bool>> saved;
+  HostParallelOff(titus_ctx* const* ctxs, int n, bool on) {
+    if (!on) return;
+    for (int i = 0; i < n; ++i)
+      if (ctxs[i]) {
+        saved.emplace_back(ctxs[i], ctxs[i]->host_parallel);
+        ctxs[i]->host_parallel = false;
+      }
+  }
+  ~HostParallelOff() {
+    for (auto& s : saved) s.first->host_parallel = s.second;
+  }
+};
 
 template <class F>
 inline int guarded(F&& f) {

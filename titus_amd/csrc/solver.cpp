@@ -78,9 +78,12 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     if (nz > 1) require(p->dz > 0 && p->Ay && p->absAy, "3D grid needs dz, Ay and |Ay|");
     require(p->dx > 0 && p->dy > 0, "cell widths must be positive");
     require(p->nm >= 1 && p->nq >= 1 && p->ne >= 3 && p->nsigma >= 1, "empty dimension");
+    require(p->nb >= 0 && p->nrestore >= 0 && p->ncand >= 0, "negative cell count");
     require(p->density && p->degree && p->Ax && p->Az && p->absAx && p->absAz && p->omega &&
                 p->M && p->stopping && p->sigma && p->bcells && p->bc_base && p->bc_scale &&
                 p->cand_cells, "missing operand");
+    require(p->nrestore == 0 || p->restore_cells, "missing operand: restore_cells");
+    require(std::isfinite(p->dE) && p->dE > 0.0, "energy step must be positive and finite");
     require(p->integrator == TITUS_INTEGRATOR_FIXED || p->integrator == TITUS_INTEGRATOR_BUG ||
                 p->integrator == TITUS_INTEGRATOR_PARALLEL,
             "unknown integrator");
@@ -101,6 +104,9 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     for (int b = 0; b < p->nrestore; ++b)
       require(p->restore_cells[b] >= 0 && p->restore_cells[b] < C, "restore cell out of range");
     for (int b = 0; b < p->ncand; ++b) require(p->cand_cells[b] >= 0 && p->cand_cells[b] < C, "candidate out of range");
+    if (p->version >= 3 && p->psi0)
+      for (size_t e = 0; e < (size_t)C * p->nq; ++e)
+        require(std::isfinite(p->psi0[e]), "initial flux must be finite");
 
     std::unique_ptr<titus_ctx> ctx(new titus_ctx());
     titus_ctx& x = *ctx;
@@ -152,6 +158,12 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
               titus::SpMat::from_dense(x.Ax), titus::SpMat::from_dense(x.Az)};
     }
     x.nb = p->nb;
+    {
+      double hmin = std::min(p->dx, p->dy), irmax = 0.0;
+      if (nz > 1) hmin = std::min(hmin, p->dz);
+      for (int c = 0; c < C; ++c) irmax = std::max(irmax, 1.0 / p->density[c]);
+      x.cfl = p->dE / hmin * irmax;
+    }
 
     const size_t cx = (size_t)C * x.capx;
     x.rho = dalloc<double>(C);
@@ -230,8 +242,6 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
           for (int c = 0; c < C; ++c) u0[c] += p->psi0[(size_t)q * C + c] * m0[q];
         HIPCHK(hipMemcpy(x.psi, p->psi0, sizeof(double) * (size_t)C * p->nq, hipMemcpyHostToDevice));
       }
-      for (size_t e = 0; e < (size_t)C * p->nq; ++e)
-        require(std::isfinite(p->psi0[e]), "initial flux must be finite");
       HIPCHK(hipMemcpy(x.u0p, u0.data(), sizeof(double) * C, hipMemcpyHostToDevice));
     }
 
@@ -293,9 +303,9 @@ int titus_solve_batch(titus_ctx** ctxs, int32_t n, int32_t nthreads, int32_t* st
   std::vector<int> codes(n, TITUS_OK);
   std::vector<std::string> msgs(n);
   std::atomic<int> next{0};
-  if (nt > 1)
-    for (int i = 0; i < n; ++i)
-      if (ctxs[i]) ctxs[i]->host_parallel = false;
+  // the contexts run side by side: no per-operator host threads inside them
+  // during the batch; each context's own setting returns afterwards
+  HostParallelOff hp(ctxs, n, nt > 1);
   auto worker = [&] {
     for (;;) {
       const int i = next.fetch_add(1);
@@ -317,6 +327,13 @@ int titus_solve_batch(titus_ctx** ctxs, int32_t n, int32_t nthreads, int32_t* st
     }
   }
   return first;
+}
+
+int titus_get_cfl(const titus_ctx* ctx, double* cfl) {
+  return guarded([&] {
+    require(ctx != nullptr && cfl != nullptr, "null argument");
+    *cfl = ctx->cfl;
+  });
 }
 
 int titus_progress(const titus_ctx* ctx, int32_t* steps_done, int32_t* rank) {
