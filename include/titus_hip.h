@@ -287,6 +287,27 @@ int titus_fcs_kernel_stats(const titus_fcs* ctx, const char* name, double* total
 int titus_fcs_get_plan(const titus_fcs* ctx, int32_t* values, int32_t n);
 void titus_fcs_destroy(titus_fcs* ctx);
 
+/* ---- device-pointer primitives --------------------------------------------
+ * A structured grid whose density lives in device memory: cells
+ * c = (i*ny + j)*nz + k as above (nz <= 1: a 2D grid, c = i*ny + j). */
+typedef struct titus_grid_v1 {
+  int32_t nx, ny, nz;
+  double dx, dy, dz;
+  const double* inv_density;  /* device, C = nx*ny*nz: 1 / rho */
+} titus_grid_v1;
+
+/* Galerkin products of a spatial basis in device memory, the
+ * XL1xX = X'*obj.stencil.L1x*X, ... of SolverCSD.jl:1035-1038 (and of the
+ * CUDA.jl variant, :1583-1584): out (device) receives 2 nd blocks of r x r,
+ * column-major, in the order L1x, L1y, (L1z,) L2x, L2y, (L2z), each
+ * X^T L_o X with the stencils of stencils.jl:12-130 acting on X / rho.
+ * X: device, C x r with leading dimension ldx >= C, 1 <= r <= 96.  Runs on
+ * `stream` (a hipStream_t; NULL: the default stream).  The reduction's
+ * partial sums are allocated for the call and freed before it returns, so
+ * the call waits for its stream. */
+int titus_galerkin(const titus_grid_v1* grid, const double* X, int32_t ldx, int32_t r,
+                   double* out, void* stream);
+
 const char* titus_last_error(void);
 int32_t titus_abi_version(void);
 

@@ -170,3 +170,19 @@ def test_cfl_number_of_the_reference_grids():
     P = cases.phantom_bug(n=64, N=5, cap=8)
     cfl = P["dE"] / min(P["dx"], P["dy"]) * np.max(1.0 / P["density"])
     assert 0.1 < cfl < 1.0
+
+
+def test_galerkin_primitive_refuses_bad_arguments_before_any_device_call():
+    lib = _lib.load()
+    g = _lib.GridV1(8, 8, 1, 0.1, 0.1, 1.0, None)
+    buf = ctypes.create_string_buffer(8)
+    p = ctypes.cast(buf, ctypes.c_void_p)
+    assert lib.titus_galerkin(ctypes.byref(g), p, 64, 4, p, None) == -1     # no density
+    assert b"null" in lib.titus_last_error()
+    g.inv_density = p
+    assert lib.titus_galerkin(ctypes.byref(g), p, 64, 0, p, None) == -1     # rank 0
+    assert lib.titus_galerkin(ctypes.byref(g), p, 64, 97, p, None) == -1    # rank above 96
+    assert lib.titus_galerkin(ctypes.byref(g), p, 10, 4, p, None) == -1     # ldx < C
+    assert b"leading" in lib.titus_last_error()
+    g.nx = 2
+    assert lib.titus_galerkin(ctypes.byref(g), p, 64, 4, p, None) == -1

@@ -31,7 +31,8 @@ EXPORTED = ("titus_create", "titus_get_cfl", "titus_step", "titus_solve", "titus
             "titus_get_decisions", "titus_set_checkpoints", "titus_adjoint", "titus_adjoint_batch",
             "titus_destroy", "titus_fcs_create", "titus_fcs_step", "titus_fcs_solve",
             "titus_fcs_get_state", "titus_fcs_set_timing", "titus_fcs_kernel_stats",
-            "titus_fcs_get_plan", "titus_fcs_destroy", "titus_last_error", "titus_abi_version")
+            "titus_fcs_get_plan", "titus_fcs_destroy", "titus_galerkin", "titus_last_error",
+            "titus_abi_version")
 
 
 class TitusError(RuntimeError):
@@ -60,6 +61,13 @@ class ProblemV1(ctypes.Structure):
         ("nz", ctypes.c_int32), ("dz", ctypes.c_double), ("Ay", _P), ("absAy", _P),
         ("psi0", _P), ("flux_order", ctypes.c_int32),
     ]
+
+
+class GridV1(ctypes.Structure):
+    """``titus_grid_v1`` (inv_density is a device address)."""
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+                ("inv_density", ctypes.c_void_p)]
 
 
 class FcsProblemV1(ctypes.Structure):
@@ -119,6 +127,8 @@ def load():
     lib.titus_fcs_get_plan.argtypes = [vp, _PI, ctypes.c_int32]
     lib.titus_fcs_destroy.argtypes = [vp]
     lib.titus_fcs_destroy.restype = None
+    lib.titus_galerkin.argtypes = [ctypes.POINTER(GridV1), vp, ctypes.c_int32, ctypes.c_int32, vp, vp]
+    lib.titus_galerkin.restype = ctypes.c_int
     lib.titus_last_error.restype = ctypes.c_char_p
     lib.titus_abi_version.restype = ctypes.c_int32
     for name in ("titus_create", "titus_get_cfl", "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
