@@ -308,6 +308,18 @@ typedef struct titus_grid_v1 {
 int titus_galerkin(const titus_grid_v1* grid, const double* X, int32_t ldx, int32_t r,
                    double* out, void* stream);
 
+/* titus_galerkin without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as titus_galerkin, but the partial sums live
+ * in a library-owned device array of 12 MiB per GPU (1024 x 6 x 256 doubles,
+ * the bound of one reduction; every accepted size fits), kept until the
+ * process ends, and the call does not wait for its stream.  All calls on one
+ * GPU share that array: calls that overlap on one GPU (two streams, or two
+ * threads) race and must be serialised by the caller. */
+int titus_galerkin_noalloc(const titus_grid_v1* grid, const double* X, int32_t ldx, int32_t r,
+                           double* out, void* stream);
+
 const char* titus_last_error(void);
 int32_t titus_abi_version(void);
 

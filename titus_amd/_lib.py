@@ -31,7 +31,8 @@ EXPORTED = ("titus_create", "titus_get_cfl", "titus_step", "titus_solve", "titus
             "titus_get_decisions", "titus_set_checkpoints", "titus_adjoint", "titus_adjoint_batch",
             "titus_destroy", "titus_fcs_create", "titus_fcs_step", "titus_fcs_solve",
             "titus_fcs_get_state", "titus_fcs_set_timing", "titus_fcs_kernel_stats",
-            "titus_fcs_get_plan", "titus_fcs_destroy", "titus_galerkin", "titus_last_error",
+            "titus_fcs_get_plan", "titus_fcs_destroy", "titus_galerkin", "titus_galerkin_noalloc",
+            "titus_last_error",
             "titus_abi_version")
 
 
@@ -129,6 +130,8 @@ def load():
     lib.titus_fcs_destroy.restype = None
     lib.titus_galerkin.argtypes = [ctypes.POINTER(GridV1), vp, ctypes.c_int32, ctypes.c_int32, vp, vp]
     lib.titus_galerkin.restype = ctypes.c_int
+    lib.titus_galerkin_noalloc.argtypes = lib.titus_galerkin.argtypes
+    lib.titus_galerkin_noalloc.restype = ctypes.c_int
     lib.titus_last_error.restype = ctypes.c_char_p
     lib.titus_abi_version.restype = ctypes.c_int32
     for name in ("titus_create", "titus_get_cfl", "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",

@@ -3,9 +3,15 @@
 //
 // titus_galerkin allocates the reduction's per-workgroup partial sums for the
 // call and frees them when the call ends (it waits for its stream).
+// titus_galerkin_noalloc runs the same kernels on a file-scope device array,
+// `scratch`, so it allocates nothing and does not wait: for callers that
+// cannot call hipMalloc (graph capture, serving loops).  Each device holds its
+// own copy of the array; its address on a device is looked up once with
+// hipGetSymbolAddress while that device is current and kept per device.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "../../include/titus_hip.h"
@@ -13,6 +19,29 @@
 #include "kernels.h"
 
 namespace {
+
+// the partial sums of one reduction: at most tn_partial_size() doubles
+// (1024 workgroups x 6 operators x 256 values, 12 MiB)
+constexpr size_t kScratchDoubles = size_t(1024) * 6 * 256;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on the current device (looked up once per device)
+double* scratch_on_current_device() {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw TitusError(TITUS_E_ARG, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
 
 titus::dev::Grid check_grid(const titus_grid_v1* gr, const double* X, int32_t ldx, int32_t r,
                             double* out) {
@@ -50,6 +79,18 @@ int titus_galerkin(const titus_grid_v1* grid, const double* X, int32_t ldx, int3
       throw;
     }
     HIPCHK(hipFree(partial));
+  });
+}
+
+int titus_galerkin_noalloc(const titus_grid_v1* grid, const double* X, int32_t ldx, int32_t r,
+                           double* out, void* stream) {
+  return guarded([&] {
+    const titus::dev::Grid g = check_grid(grid, X, ldx, r, out);
+    static_assert(kScratchDoubles >= 1024 * 6 * 256, "scratch below the partial-sum bound");
+    require(titus::dev::tn_partial_size() <= kScratchDoubles, "partial sums exceed the scratch array");
+    double* partial = scratch_on_current_device();
+    titus::dev::launch_tn(static_cast<hipStream_t>(stream), g, X, ldx, r, X, ldx, r,
+                          titus::dev::OP_ALL, partial, out);
   });
 }
 
