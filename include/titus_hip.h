@@ -45,7 +45,8 @@ enum {
 };
 
 /* FIXED: SolveFirstCollisionSourceDLR (SolverCSD.jl:891-1120); BUG:
- * ...DLRBUGRejection with the rejection branches off (:2011-2374); PARALLEL:
+ * ...DLRBUGRejection (:2011-2374; rejection off unless titus_set_rejection
+ * turns it on); PARALLEL:
  * SolveFirstCollisionSourceDLRParallel (:1726-2008). */
 enum { TITUS_INTEGRATOR_FIXED = 0, TITUS_INTEGRATOR_BUG = 1, TITUS_INTEGRATOR_PARALLEL = 2 };
 /* FP64: every operation in FP64.
@@ -150,6 +151,22 @@ int titus_solve(struct titus_ctx* ctx, int32_t last_step);
  * No reference counterpart: the reference solves one beam per call
  * (code/SolverCSD.jl:891, 2011). */
 int titus_solve_batch(struct titus_ctx** ctxs, int32_t n, int32_t nthreads, int32_t* status);
+
+/* Rejection of the rank-adaptive step (SolverCSD.jl:2204-2233, 2282-2301),
+ * BUG integrator only, before the first step.  c_eta > 0 turns it on (the
+ * reference's settings.cη, 5 by default); 0 turns it off (the default).  A
+ * step is rejected when its truncation keeps all 2r columns or its eta
+ * exceeds c_eta * theta * max(1e-7, |S|^theta_index) / dE, and 2r is below
+ * the rank cap (the reference's `rmax` is undefined; settings.r is used).
+ * Unlike the reference, which repeats the energy index with the dose and
+ * the swept psi already applied, a rejected step restarts from its own start
+ * (X, S, W, psi, dose) with the rejected half's new directions added to X
+ * and W and zeros to S -- rank 2r after the streaming half, r0 + r after the
+ * scattering half -- so nothing is counted twice.  The history then holds
+ * the rank each accepted attempt started from. */
+int titus_set_rejection(struct titus_ctx* ctx, double c_eta);
+/* Number of rejected attempts so far. */
+int titus_get_rejections(const struct titus_ctx* ctx, int64_t* count);
 
 /* Number of steps taken so far and the current rank. */
 int titus_progress(const struct titus_ctx* ctx, int32_t* steps_done, int32_t* rank);

@@ -347,12 +347,27 @@ function CSolveFirstCollisionSourceDLR(obj; precision=TITUS_PRECISION_FP64)
     end
 end
 
+"""Turn the step rejection of the rank-adaptive integrator on (c_eta > 0) or
+off (SolverCSD.jl:2204-2233, 2282-2301), before the first step."""
+set_rejection!(ctx::TitusContext, c_eta::Real) =
+    check(ccall((:titus_set_rejection, libtitus), Cint, (Ptr{Cvoid}, Float64), ctx.ptr, Float64(c_eta)))
+
+function rejections(ctx::TitusContext)
+    n = Ref{Int64}(0)
+    check(ccall((:titus_get_rejections, libtitus), Cint, (Ptr{Cvoid}, Ptr{Int64}), ctx.ptr, n))
+    return Int(n[])
+end
+
 """SolveFirstCollisionSourceDLRBUGRejection(obj) on the GPU (SolverCSD.jl:2011-2374):
-the outputs of the fixed-rank form plus [E rank] and [E eta]."""
+the outputs of the fixed-rank form plus [E rank] and [E eta].  `rejection=true`
+turns on the step rejection with `settings.cη` (a rejected step restarts
+from its own start at the enlarged rank, so nothing is counted twice)."""
 function SolveFirstCollisionSourceDLRBUGRejection(obj, recordGif::Bool=false;
-                                                   precision=TITUS_PRECISION_FP64)
+                                                   precision=TITUS_PRECISION_FP64,
+                                                   rejection::Bool=false)
     ctx = create(obj, TITUS_INTEGRATOR_BUG; precision=precision)
     try
+        rejection && set_rejection!(ctx, obj.settings.cη)
         solve!(ctx)
         done, _ = progress(ctx)
         ranks, etas = history(ctx, done)

@@ -11,6 +11,9 @@ It restates, in numpy with T = Float64 throughout:
                       rejection branches off (``code/SolverCSD.jl:2011-2374``)
 * ``solve_parallel`` -- ``SolveFirstCollisionSourceDLRParallel``
                       (``code/SolverCSD.jl:1726-2008``)
+* ``solve_rk4``    -- ``CudaSolveFirstCollisionSourceDLR4thOrder``
+                      (``code/SolverCSD.jl:1422-1724``): fixed rank, central
+                      stencils only, RK4 K/L/S substeps, analytic uncollided flux
 * ``truncate``     -- ``truncate!``                       (``code/SolverCSD.jl:3369-3399``)
 * ``upwind_flux``  -- ``solveFluxUpwind!``                (``code/SolverCSD.jl:388-463``)
 * ``limited_flux`` -- ``solveFlux!`` with ``slopefit``    (``code/SolverCSD.jl:278-386``),
@@ -471,13 +474,30 @@ def solve_fixed(P, steps=None):
                 sigma=sv / np.sqrt(2.0), WV=W @ Vt.T, steps=nE - 1)
 
 
-def solve_bug(P, steps=None):
-    """Rank-adaptive augmented BUG (``SolverCSD.jl:2011-2374``), rejection off.
+def solve_bug(P, steps=None, on_step=None):
+    """Rank-adaptive augmented BUG (``SolverCSD.jl:2011-2374``).
 
-    The reference's streaming-rejection branch re-runs a step after the dose
-    and sweep have been applied (double counting) and its scattering branch
-    reads an undefined ``rmax``; both are disabled here.  The eta indicator of
-    the streaming step is returned as a diagnostic.
+    The eta indicator of the streaming step is returned as a diagnostic.
+    ``on_step(s, dose)``, when given, is called after each step s (the CPU
+    baseline times a window of steps with it).
+
+    Rejection (``P["rejection"]`` = c_eta > 0; off when absent or 0).  The
+    reference rejects a step (:2204-2233, :2282-2301) when its truncation kept
+    all 2r columns or when eta exceeds c_eta theta max(1e-7, |S|^thetaIndex)
+    / dE, and 2r is below the cap (its `rmax` is undefined; the cap here is
+    ``rank_cap`` = settings.r).  Eta is |Xt^T (sum_o L_o X) S (W^T A_o Wt)| in
+    the streaming half (|S| of the untruncated augmented S) and
+    |(Xt^T psi)(M^T D Wt)| in the scattering half (|S| of the truncated one).
+    The reference then repeats the energy index with the dose and the swept
+    psi already applied (and, in one branch, without leaving the step).  Here
+    a rejected step restarts from its own start -- X, S, W, psi and the dose
+    as they were -- with X and W extended by the rejected half's new
+    directions Xt and Wt and S by zeros, which leaves u = X S W^T unchanged:
+    rank 2r after the streaming half; r0 + r after the scattering half, whose
+    r new directions (r the rank after streaming, the reference's test is on
+    2r) are first made orthogonal to the step's starting bases.  The
+    scattering test also needs r0 + r <= cap.  Nothing is counted twice.
+    ``rejections`` counts the restarts.
     """
     ops = _Ops(P)
     r = P["r"]
@@ -497,12 +517,24 @@ def solve_bug(P, steps=None):
     icand = ops.interior
     mcand = range(ops.Nm)
     ranks, etas = [], []
+    c_eta = float(P.get("rejection", 0.0) or 0.0)
+    rejections = 0
     X[ops.bcells, :] = 0.0
-    for s in range(1, nE):
+
+    def pad(S_, extra):
+        n = S_.shape[0]
+        Sp = np.zeros((n + extra, n + extra))
+        Sp[:n, :n] = S_
+        return Sp
+
+    s = 0
+    while s + 1 < nE:
+        s += 1
         # the reference's while-loop breaks at n == nEnergies (:2104), so its
         # last grid point is never stepped; steps=None follows that rule
         if steps is None and s == P["nE"] - 1:
             break
+        start = (X.copy(), S.copy(), W.copy(), psi.copy(), dose.copy(), r, len(etas))
         ranks.append(r)
         sig = P["sigma"][s]
         psi = _begin_step(ops, P, psi, s)
@@ -539,6 +571,18 @@ def solve_bug(P, steps=None):
                 acc = acc + XLXt[t] @ S @ WAWt[t]
             eta = np.linalg.norm(acc)
             etas.append(eta)
+            r0 = start[5]
+            if c_eta > 0 and 2 * r0 < cap:
+                bound = c_eta * theta * max(1e-7, np.linalg.norm(Sup) ** thidx) / dE
+                if SUP.shape[0] == 2 * r0 or eta > bound:
+                    # restart the step at rank 2r in [X0, Xt] x [W0, Wt]
+                    X0, S0, W0, psi, dose, _, ne_ = start
+                    X, W, S, r = X1, W1, pad(S0, r0), 2 * r0
+                    ranks.pop()
+                    del etas[ne_:]
+                    rejections += 1
+                    s -= 1
+                    continue
             S, X, W = SUP, XUP, WUP
             r = S.shape[0]
         # out-scattering (:2242-2249)
@@ -564,12 +608,32 @@ def solve_bug(P, steps=None):
         S = Mu @ S @ Nu.T
         MtDW = M.T @ (Dvec[:, None] * W)
         Sup = S + dE * (X.T @ psi) @ MtDW
-        X, S, W = truncate(X, Sup, W, theta, thidx, cap)
+        XUP, SUP, WUP = truncate(X, Sup, W, theta, thidx, cap)
+        r0, rc = start[5], r          # rank at the step's start and after streaming
+        if c_eta > 0 and 2 * rc < cap and r0 + rc <= cap:
+            # eta of the scattering half (:2285-2290)
+            eta_s = np.linalg.norm((Xt.T @ psi) @ (M.T @ (Dvec[:, None] * Wt)))
+            bound = c_eta * theta * max(1e-7, np.linalg.norm(SUP) ** thidx) / dE
+            if SUP.shape[0] == 2 * rc or eta_s > bound:
+                # restart at rank r0 + rc: the rc new directions of this half,
+                # orthogonalised against the step's starting bases
+                X0, S0, W0, psi, dose, _, ne_ = start
+                Xt2 = range_basis(Xt, rc, icand, against=X0, scale=np.linalg.norm(Xt))
+                Wt2 = range_basis(Wt, rc, mcand, against=W0, scale=np.linalg.norm(Wt), greedy=True)
+                X, W, S, r = np.hstack([X0, Xt2]), np.hstack([W0, Wt2]), pad(S0, rc), r0 + rc
+                ranks.pop()
+                del etas[ne_:]
+                rejections += 1
+                s -= 1
+                continue
+        X, S, W = XUP, SUP, WUP
         r = S.shape[0]
         dose += 0.5 * dE * (X @ (S @ W[0, :]) + psi @ M[0, :]) * Sst[s] / rho
+        if on_step is not None and on_step(s, dose) is False:
+            break
     U, sv, Vt = np.linalg.svd(S)
     return dict(dose=dose, X=X, S=S, W=W, psi=psi, Xout=X @ U, sigma=sv / np.sqrt(2.0),
-                WV=W @ Vt.T, ranks=np.array(ranks), etas=np.array(etas))
+                WV=W @ Vt.T, ranks=np.array(ranks), etas=np.array(etas), rejections=rejections)
 
 
 def solve_parallel(P, steps=None):
@@ -656,6 +720,119 @@ def solve_parallel(P, steps=None):
     U, sv, Vt = np.linalg.svd(S)
     return dict(dose=dose, X=X, S=S, W=W, psi=psi, Xout=X @ U, sigma=sv / np.sqrt(2.0),
                 WV=W @ Vt.T, rvec=rvec)
+
+
+def normpdf(x, mu, sigma):
+    """``utils.jl:43-45``."""
+    return 1.0 / (sigma * np.sqrt(2.0 * np.pi)) * np.exp(-((x - mu) ** 2) / 2.0 / sigma ** 2)
+
+
+def analytic_psi(P, psi, s, int_sigma):
+    """The uncollided flux of ``CudaSolveFirstCollisionSourceDLR4thOrder`` at
+    step s (``SolverCSD.jl:1549-1557``): for each reduced ordinate q and cell
+    (i, j), 1e5 exp(-1e4 (Omega1 - w_x)^2) exp(-1e4 (Omega3 - w_z)^2)
+    exp(-int_sigma) normpdf(x_i - tau w_x, x0, 0.1) normpdf(y_j - tau w_z, y0,
+    0.1) with tau = eTrafo[s]; entries whose x or y factor is below 1e-6 keep
+    their previous value (the loop skips them).  Updates psi (C x Q) in place."""
+    nx, ny = P["nx"], P["ny"]
+    b = P["beam"]
+    tau = P["eTrafo"][s]
+    P3 = psi.reshape(nx, ny, -1)
+    for q, w in enumerate(np.asarray(P["omega"])):
+        amp = (1e5 * np.exp(-10000.0 * (b["Omega1"] - w[0]) ** 2)
+               * np.exp(-10000.0 * (b["Omega3"] - w[2]) ** 2) * np.exp(-int_sigma))
+        by = normpdf(P["yMid"] - tau * w[2], b["y0"], 0.1)
+        bx = normpdf(P["xMid"] - tau * w[0], b["x0"], 0.1)
+        keep = (bx[:, None] >= 1e-6) & (by[None, :] >= 1e-6)
+        P3[:, :, q] = np.where(keep, amp * bx[:, None] * by[None, :], P3[:, :, q])
+    return psi
+
+
+def solve_rk4(P, steps=None):
+    """``CudaSolveFirstCollisionSourceDLR4thOrder`` (``SolverCSD.jl:1422-1724``) in FP64.
+
+    Fixed rank; the streaming K-, L- and S-steps integrate the central terms
+    only (L2x with Ax, L2y with Az; no Roe terms) with the classical RK4
+    stages of the reference (k1, k2 = f(. + dE/2 k1), k3 = f(. + dE/2 k2),
+    k4 = f(. + dE k3), update dE (k1 + 2 k2 + 2 k3 + k4) / 6); the uncollided
+    flux is the analytic beam (``analytic_psi``) instead of a sweep, with the
+    running int sigma = dE sum_k sigma_0(E_k) from E_1 on; no ghost rows are
+    zeroed after the first step, and the loop runs over every grid point.
+    New bases are range_basis of K and L (svd!(K).U, svd!(L).U).  Returns
+    dict(dose, X, S, W, psi).
+    """
+    ops = _Ops(P)
+    r = P["r"]
+    X = np.array(P["X0"], dtype=np.float64)
+    W = np.array(P["W0"], dtype=np.float64)
+    S = np.array(P["S0"], dtype=np.float64)
+    psi = initial_psi(P)
+    C = P["C"]
+    dose = np.zeros(C)
+    rho, dE, Sst, M = ops.rho, P["dE"], P["Sst"], ops.M
+    nE = P["nE"] if steps is None else min(P["nE"], steps + 1)
+    icand, mcand = ops.interior, range(ops.Nm)
+    X[ops.bcells, :] = 0.0
+    int_sigma = dE * P["sigma"][0][0]
+    h = 0.5 * dE
+
+    def rk4(Y, f):
+        k = f(Y)
+        up = Y + dE * k / 6
+        k = f(Y + h * k)
+        up = up + 2 * dE * k / 6
+        k = f(Y + h * k)
+        up = up + 2 * dE * k / 6
+        k = f(Y + dE * k)
+        return up + dE * k / 6
+
+    for s in range(1, nE):
+        sig = P["sigma"][s]
+        dose += h * (X @ (S @ W[0, :]) + psi @ M[0, :]) * Sst[s - 1] / rho
+        int_sigma += dE * sig[0]
+        psi = analytic_psi(P, psi, s, int_sigma)
+        Dvec = sig[ops.deg]
+        if s > 1:
+            # K-step: f(K) = -L2x K (W^T Ax W) - L2y K (W^T Az W)
+            WAx, WAz = W.T @ ops.Ax @ W, W.T @ ops.Az @ W
+            def fK(K):
+                Ls = ops.L(K)
+                return -Ls[2] @ WAx - Ls[3] @ WAz
+            K = rk4(X @ S, fK)
+            Xn = range_basis(K, r, icand)
+            MUp = Xn.T @ X
+            # L-step with the old X: f(L) = -Ax L (X^T L2x X)^T - Az L (X^T L2y X)^T
+            XL = ops.galerkin_x(X, X)
+            XL2x, XL2y = XL[2], XL[3]
+            def fL(L):
+                return -ops.Ax @ L @ XL2x.T - ops.Az @ L @ XL2y.T
+            L = rk4(W @ S.T, fL)
+            Wn = range_basis(L, r, mcand, greedy=True)
+            NUp = Wn.T @ W
+            W, X = Wn, Xn
+            S = MUp @ S @ NUp.T
+            XL = ops.galerkin_x(X, X)
+            XL2x, XL2y = XL[2], XL[3]
+            WAx, WAz = W.T @ ops.Ax @ W, W.T @ ops.Az @ W
+            def fS(S_):
+                return -XL2x @ S_ @ WAx - XL2y @ S_ @ WAz
+            S = rk4(S, fS)
+        # out-scattering (:1637-1645)
+        L = (W @ S.T) / (1.0 + dE * (sig[0] - Dvec))[:, None]
+        W = range_basis(L, r, mcand, greedy=True)
+        S = L.T @ W
+        # in-scattering (:1649-1676), no ghost rows zeroed
+        K = X @ S + dE * psi @ (M.T @ (Dvec[:, None] * W))
+        Xn = range_basis(K, r, icand)
+        MUp = Xn.T @ X
+        L = W @ S.T + dE * Dvec[:, None] * (M @ (psi.T @ X))
+        Wn = range_basis(L, r, mcand, greedy=True)
+        NUp = Wn.T @ W
+        W, X = Wn, Xn
+        S = MUp @ S @ NUp.T
+        S = S + dE * (X.T @ psi) @ (M.T @ (Dvec[:, None] * W))
+        dose += h * (X @ (S @ W[0, :]) + psi @ M[0, :]) * Sst[s] / rho
+    return dict(dose=dose, X=X, S=S, W=W, psi=psi)
 
 
 def solve_dense(P, steps=None, stream=True):

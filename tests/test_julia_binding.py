@@ -22,7 +22,7 @@ JULIA_TO_C = {
     "Ptr{Float64}": "double*", "Ptr{Int32}": "int32_t*",
     "Ptr{Cvoid}": "titus_ctx*", "Ptr{Ptr{Cvoid}}": "titus_ctx**",
     "Ref{TitusProblemV1}": "titus_problem_v1*",
-    "Ptr{Ptr{Float64}}": "double**",
+    "Ptr{Ptr{Float64}}": "double**", "Ptr{Int64}": "int64_t*",
 }
 
 
@@ -100,7 +100,7 @@ def test_every_ccall_matches_its_prototype():
     for need in ("titus_create", "titus_solve", "titus_get_state", "titus_progress",
                  "titus_get_history", "titus_destroy", "titus_solve_batch",
                  "titus_set_checkpoints", "titus_adjoint_batch", "titus_last_error",
-                 "titus_get_cfl"):
+                 "titus_get_cfl", "titus_set_rejection", "titus_get_rejections"):
         assert need in seen, need
 
 

@@ -25,7 +25,8 @@ ERRORS = {
 }
 
 # every symbol include/titus_hip.h declares
-EXPORTED = ("titus_create", "titus_get_cfl", "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
+EXPORTED = ("titus_create", "titus_get_cfl", "titus_set_rejection", "titus_get_rejections",
+            "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
             "titus_get_state",
             "titus_get_history", "titus_set_timing", "titus_kernel_stats", "titus_get_tuning",
             "titus_get_decisions", "titus_set_checkpoints", "titus_adjoint", "titus_adjoint_batch",
@@ -101,6 +102,8 @@ def load():
     lib.titus_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(ProblemV1), ctypes.c_int32]
     lib.titus_step.argtypes = [vp, ctypes.c_int32]
     lib.titus_get_cfl.argtypes = [vp, _P]
+    lib.titus_set_rejection.argtypes = [vp, ctypes.c_double]
+    lib.titus_get_rejections.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     lib.titus_solve.argtypes = [vp, ctypes.c_int32]
     lib.titus_solve_batch.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32, _PI]
     lib.titus_progress.argtypes = [vp, _PI, _PI]
@@ -134,7 +137,8 @@ def load():
     lib.titus_galerkin_noalloc.restype = ctypes.c_int
     lib.titus_last_error.restype = ctypes.c_char_p
     lib.titus_abi_version.restype = ctypes.c_int32
-    for name in ("titus_create", "titus_get_cfl", "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
+    for name in ("titus_create", "titus_get_cfl", "titus_set_rejection", "titus_get_rejections",
+                 "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
                  "titus_get_state", "titus_get_history", "titus_set_timing",
                  "titus_kernel_stats", "titus_get_tuning", "titus_get_decisions",
                  "titus_set_checkpoints",
@@ -294,6 +298,15 @@ class Context:
         v = ctypes.c_double()
         check(self._lib.titus_get_cfl(self.h, ctypes.byref(v)))
         return v.value
+
+    def set_rejection(self, c_eta):
+        """Turn the rank-adaptive step's rejection on (c_eta > 0) or off (0)."""
+        check(self._lib.titus_set_rejection(self.h, float(c_eta)))
+
+    def rejections(self):
+        n = ctypes.c_int64()
+        check(self._lib.titus_get_rejections(self.h, ctypes.byref(n)))
+        return n.value
 
     def step(self, s):
         check(self._lib.titus_step(self.h, s))

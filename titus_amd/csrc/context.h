@@ -265,10 +265,19 @@ struct titus_ctx {
   // adjoint: checkpoints and recomputation (adjoint.cpp)
   std::shared_ptr<AdjointStore> adj;
   StepTape* tape = nullptr;
+  // rejection of the rank-adaptive step (SolverCSD.jl:2204-2233, 2282-2301):
+  // c_eta > 0 turns it on; the state at the start of each step is kept so
+  // that a rejected step restarts from it
+  double c_eta = 0.0;
+  int64_t rejections = 0;
+  double *X_start = nullptr, *dose_start = nullptr, *u0_start = nullptr;
+  void* psi_start = nullptr;
+  HMat S_start, W_start;
+  int r_start = 0;
 
   ~titus_ctx() {
     void* ptrs[] = {rho, irho, omega, bc_base, m0, psi, psi2, dose, u0p, X, Xn, K, Kc, Q1, Q2, partial, arena,
-                    upl, bcells, cand_d, bmask, restore};
+                    upl, bcells, cand_d, bmask, restore, X_start, dose_start, u0_start, psi_start};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     void* hptrs[] = {h_arena, h_upl};
     for (void* p : hptrs) if (p) (void)hipHostFree(p);
@@ -834,13 +843,21 @@ struct titus_ctx {
     dose_half(stopping[s]);
   }
 
-  // truncate! (SolverCSD.jl:3369-3399) of the 2r x 2r Sup over the augmented
-  // bases X1 (device, C x 2r) and W1; writes X (device), W, S, rank.
-  void truncate_to(const HMat& Sup, const double* X1, const HMat& W1) {
-    HostScope hs(timing, "host:truncate");
+  // truncate! (SolverCSD.jl:3369-3399) of the 2r x 2r Sup: the SVD and the
+  // kept rank (plan_truncation), then the new factors over the augmented
+  // bases X1 (device, C x 2r) and W1 (apply_truncation: X (device), W, S,
+  // rank).  Split so that a rejected step can look at the decision first.
+  struct TruncPlan {
     HMat U, V;
     std::vector<double> D;
-    titus::svd_completed(Sup, rank_tol, U, D, V);
+    int rmax = 0;
+    double kept_norm = 0.0;   // |S| of the truncated S (Frobenius)
+  };
+  TruncPlan plan_truncation(const HMat& Sup) {
+    HostScope hs(timing, "host:truncate");
+    TruncPlan t;
+    titus::svd_completed(Sup, rank_tol, t.U, t.D, t.V);
+    const std::vector<double>& D = t.D;
     const int n = (int)D.size();
     double nrm = 0.0;
     for (double d : D) nrm += d * d;
@@ -852,13 +869,78 @@ struct titus_ctx {
       for (int i = jj; i < n; ++i) tail += D[i];
       if (std::fabs(tail) < tol) { rmax = jj + 1; break; }
     }
-    rmax = std::max(std::min(rmax, rank_cap), 2);
+    t.rmax = std::max(std::min(rmax, rank_cap), 2);
+    for (int i = 0; i < t.rmax; ++i) t.kept_norm += D[i] * D[i];
+    t.kept_norm = std::sqrt(t.kept_norm);
+    return t;
+  }
+  void apply_truncation(const TruncPlan& t, const double* X1, const HMat& W1) {
+    HostScope hs(timing, "host:truncate");
+    const int n = (int)t.D.size(), rmax = t.rmax;
     fold(rmax);
-    gemm(X1, n, titus::cols(U, 0, rmax), 1.0, 0.0, X, nullptr);
-    W = titus::matmul(W1, titus::cols(V, 0, rmax));
+    gemm(X1, n, titus::cols(t.U, 0, rmax), 1.0, 0.0, X, nullptr);
+    W = titus::matmul(W1, titus::cols(t.V, 0, rmax));
     S = HMat(rmax, rmax);
-    for (int i = 0; i < rmax; ++i) S(i, i) = D[i];
+    for (int i = 0; i < rmax; ++i) S(i, i) = t.D[i];
     rank = rmax;
+  }
+  void truncate_to(const HMat& Sup, const double* X1, const HMat& W1) {
+    apply_truncation(plan_truncation(Sup), X1, W1);
+  }
+
+  // ---- rejection (SolverCSD.jl:2204-2233, 2282-2301) ---------------------------
+  // State at the start of the step (device copies of X, psi, dose and the
+  // moment partials; host S, W, rank).
+  void save_start() {
+    const size_t psz = mixed ? sizeof(float) : sizeof(double);
+    HIPCHK(hipMemcpyAsync(X_start, X, sizeof(double) * (size_t)C * rank, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(psi_start, psi, psz * (size_t)C * nq, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(dose_start, dose, sizeof(double) * C, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(u0_start, u0p, sizeof(double) * (size_t)u0_groups * C,
+                          hipMemcpyDeviceToDevice, st));
+    S_start = S;
+    W_start = W;
+    r_start = rank;
+  }
+  // psi, dose and the moment partials back to the step's start; S padded
+  // with zeros to the new rank (u = X S W^T is unchanged)
+  void restart_state(int newrank, size_t neta) {
+    const size_t psz = mixed ? sizeof(float) : sizeof(double);
+    HIPCHK(hipMemcpyAsync(psi, psi_start, psz * (size_t)C * nq, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(dose, dose_start, sizeof(double) * C, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(u0p, u0_start, sizeof(double) * (size_t)u0_groups * C,
+                          hipMemcpyDeviceToDevice, st));
+    S = HMat(newrank, newrank);
+    for (int j = 0; j < r_start; ++j)
+      for (int i = 0; i < r_start; ++i) S(i, j) = S_start(i, j);
+    rank = newrank;
+    ranks.pop_back();
+    etas.resize(neta);
+    ++rejections;
+    fold(-1 - newrank);
+  }
+  // streaming half rejected: X1 = [X_start | Xt] (device Xn), W1 = [W_start | Wt]
+  void reject_streaming(const HMat& W1, size_t neta) {
+    const int n = W1.n;
+    HIPCHK(hipMemcpyAsync(X, Xn, sizeof(double) * (size_t)C * n, hipMemcpyDeviceToDevice, st));
+    W = W1;
+    restart_state(n, neta);
+  }
+  // scattering half rejected: its rc new directions (Xn columns rc..2rc-1,
+  // W1 columns rc..) made orthogonal to the step's starting bases
+  void reject_scattering(int rc, const HMat& W1, size_t neta) {
+    const int r0 = r_start;
+    HIPCHK(hipMemcpyAsync(K, Xn + (size_t)rc * C, sizeof(double) * (size_t)C * rc,
+                          hipMemcpyDeviceToDevice, st));
+    areset();
+    RangeJob j = range_launch(K, rc, X_start, r0);
+    const HMat Wt = titus::cols(W1, rc, W1.n);
+    HMat Wt2 = range_w(Wt, rc, &W_start, titus::frob(Wt));
+    fetch();
+    HIPCHK(hipMemcpyAsync(X, X_start, sizeof(double) * (size_t)C * r0, hipMemcpyDeviceToDevice, st));
+    range_finish(j, rc, X + (size_t)r0 * C);
+    W = titus::hcat(W_start, Wt2);
+    restart_state(r0 + rc, neta);
   }
   HMat augment_w(const HMat& L) {
     return titus::hcat(W, range_w(L, rank, &W, titus::frob(L)));
@@ -868,7 +950,20 @@ struct titus_ctx {
   }
 
   void step_bug(int s) {
+    // a rejected attempt restores the step's start and enlarges the rank;
+    // every rejection needs 2r < cap and adds r >= 1 columns, so the loop
+    // ends
+    for (;;) {
+      if (c_eta > 0.0) save_start();
+      if (step_bug_attempt(s)) return;
+    }
+  }
+
+  // One attempt at a BUG step; false when the rejection test sent it back
+  // to its start (state already restored and enlarged).
+  bool step_bug_attempt(int s) {
     ranks.push_back(rank);
+    const size_t neta = etas.size();
     std::vector<double> Dvec, Dout;
     begin_step(s, Dvec, Dout);
     if (s > 1) {
@@ -911,8 +1006,18 @@ struct titus_ctx {
         HMat t = titus::matmul(titus::matmul(xl, S1), wa);
         for (size_t i = 0; i < acc.a.size(); ++i) acc.a[i] += t.a[i];
       }
-      etas.push_back(titus::frob(acc));
-      truncate_to(Sup, Xn, W1);
+      const double eta = titus::frob(acc);
+      etas.push_back(eta);
+      const TruncPlan tp = plan_truncation(Sup);
+      if (c_eta > 0.0 && 2 * r < rank_cap) {
+        // rejection test of the streaming half (:2204-2233)
+        const double bound = c_eta * theta * std::max(1e-7, std::pow(titus::frob(Sup), theta_index)) / dE;
+        if (tp.rmax == 2 * r || eta > bound) {
+          reject_streaming(W1, neta);
+          return false;
+        }
+      }
+      apply_truncation(tp, Xn, W1);
     }
     // X^T psi (old X) on the device during the host out-scattering
     const int r = rank;
@@ -944,10 +1049,28 @@ struct titus_ctx {
     fetch();
     HMat Mu = view(mu, 2 * r, r);
     HMat S1 = titus::matmul(titus::matmul(Mu, S), Nu, false, true);
-    HMat add = titus::matmul(view(x1p, 2 * r, nq), MtDW1);
+    HMat X1psi = view(x1p, 2 * r, nq);
+    HMat add = titus::matmul(X1psi, MtDW1);
     for (size_t i = 0; i < S1.a.size(); ++i) S1.a[i] += dE * add.a[i];
-    truncate_to(S1, Xn, W1);
+    const TruncPlan tp = plan_truncation(S1);
+    if (c_eta > 0.0 && 2 * r < rank_cap && r_start + r <= rank_cap) {
+      // rejection test of the scattering half (:2282-2301): eta =
+      // |(Xt^T psi)(M^T D Wt)| from rows r.. of X1^T psi, columns r.. of M^T D W1
+      HMat xt(r, nq), mt(nq, r);
+      for (int c = 0; c < nq; ++c)
+        for (int i = 0; i < r; ++i) xt(i, c) = X1psi(r + i, c);
+      for (int c = 0; c < r; ++c)
+        for (int i = 0; i < nq; ++i) mt(i, c) = MtDW1(i, r + c);
+      const double eta_s = titus::frob(titus::matmul(xt, mt));
+      const double bound = c_eta * theta * std::max(1e-7, std::pow(tp.kept_norm, theta_index)) / dE;
+      if (tp.rmax == 2 * r || eta_s > bound) {
+        reject_scattering(r, W1, neta);
+        return false;
+      }
+    }
+    apply_truncation(tp, Xn, W1);
     dose_half(stopping[s]);
+    return true;
   }
 
   // [[S, L^T Wt], [Xt^T K, 0]] (SolverCSD.jl:1882-1888, 1935-1940)

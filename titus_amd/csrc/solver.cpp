@@ -329,6 +329,32 @@ int titus_solve_batch(titus_ctx** ctxs, int32_t n, int32_t nthreads, int32_t* st
   return first;
 }
 
+int titus_set_rejection(titus_ctx* ctx, double c_eta) {
+  return guarded([&] {
+    require(ctx != nullptr, "null context");
+    require(std::isfinite(c_eta) && c_eta >= 0.0, "c_eta must be finite and >= 0");
+    if (ctx->integrator != TITUS_INTEGRATOR_BUG)
+      throw TitusError(TITUS_E_ARG, "rejection: rank-adaptive BUG integrator only");
+    if (ctx->steps_done != 0) throw TitusError(TITUS_E_STATE, "rejection must be set before the first step");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (c_eta > 0.0 && !ctx->X_start) {
+      const size_t psz = ctx->mixed ? sizeof(float) : sizeof(double);
+      ctx->X_start = dalloc<double>((size_t)ctx->C * ctx->capx);
+      ctx->psi_start = dalloc<unsigned char>((size_t)ctx->C * ctx->nq * psz);
+      ctx->dose_start = dalloc<double>(ctx->C);
+      ctx->u0_start = dalloc<double>((size_t)dev::kMaxSweepGroups * ctx->C);
+    }
+    ctx->c_eta = c_eta;
+  });
+}
+
+int titus_get_rejections(const titus_ctx* ctx, int64_t* count) {
+  return guarded([&] {
+    require(ctx != nullptr && count != nullptr, "null argument");
+    *count = ctx->rejections;
+  });
+}
+
 int titus_get_cfl(const titus_ctx* ctx, double* cfl) {
   return guarded([&] {
     require(ctx != nullptr && cfl != nullptr, "null argument");

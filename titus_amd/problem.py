@@ -109,6 +109,9 @@ def build_problem(s, moments="subset", integrator=INTEGRATOR_FIXED):
         integrator=integrator, r=r0, rank_cap=int(s.rank_cap),
         theta=float(s.theta), thetaIndex=float(s.thetaIndex), cEta=float(s.cEta),
         X0=X0, W0=W0, S0=S0, psi0=psi0,
+        # the beam of the analytic uncollided flux (CUDA.jl variant, SolverCSD.jl:1437-1440)
+        xMid=np.asarray(s.xMid, dtype=np.float64), yMid=np.asarray(s.yMid, dtype=np.float64),
+        beam=dict(x0=float(s.x0), y0=float(s.y0), Omega1=float(s.Omega1), Omega3=float(s.Omega3)),
     )
 
 

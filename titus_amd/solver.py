@@ -134,11 +134,16 @@ def CSolveFirstCollisionSourceDLR(obj):
     return _finish(obj, P, st)
 
 
-def SolveFirstCollisionSourceDLRBUGRejection(obj, recordGif=False):
-    """Rank-adaptive augmented-BUG solve on the GPU (rejection branches off)."""
+def SolveFirstCollisionSourceDLRBUGRejection(obj, recordGif=False, rejection=False):
+    """Rank-adaptive augmented-BUG solve on the GPU.  ``rejection=True`` turns
+    on the step rejection of ``SolverCSD.jl:2204-2233, 2282-2301`` with
+    ``settings.cEta`` (a rejected step restarts from its own start at the
+    enlarged rank; see ``titus_set_rejection``)."""
     P = obj.problem(INTEGRATOR_BUG)
     ctx = obj.context(INTEGRATOR_BUG)
     try:
+        if rejection:
+            ctx.set_rejection(obj.settings.cEta)
         ctx.solve(0)
         st = ctx.state()
         done, _ = ctx.progress()
