@@ -48,7 +48,11 @@ enum {
  * ...DLRBUGRejection (:2011-2374; rejection off unless titus_set_rejection
  * turns it on); PARALLEL:
  * SolveFirstCollisionSourceDLRParallel (:1726-2008). */
-enum { TITUS_INTEGRATOR_FIXED = 0, TITUS_INTEGRATOR_BUG = 1, TITUS_INTEGRATOR_PARALLEL = 2 };
+enum { TITUS_INTEGRATOR_FIXED = 0, TITUS_INTEGRATOR_BUG = 1, TITUS_INTEGRATOR_PARALLEL = 2,
+       /* CudaSolveFirstCollisionSourceDLR4thOrder (:1422-1724): fixed rank,
+        * central stencils only with RK4 K/L/S substeps, the analytic
+        * uncollided flux of titus_set_analytic_beam; 2D */
+       TITUS_INTEGRATOR_RK4 = 3 };
 /* FP64: every operation in FP64.
  * MIXED: the uncollided flux psi stored and swept in FP32 (half the HBM
  *   traffic of the dominant kernel); everything else in FP64.
@@ -151,6 +155,24 @@ int titus_solve(struct titus_ctx* ctx, int32_t last_step);
  * No reference counterpart: the reference solves one beam per call
  * (code/SolverCSD.jl:891, 2011). */
 int titus_solve_batch(struct titus_ctx** ctxs, int32_t n, int32_t nthreads, int32_t* status);
+
+/* The pencil beam of the RK4 integrator's analytic uncollided flux
+ * (SolverCSD.jl:1437-1440, 1549-1557):
+ *   psi[c, q] = 1e5 exp(-1e4 (Omega1 - w_x)^2) exp(-1e4 (Omega3 - w_z)^2)
+ *               exp(-int sigma) normpdf(x_i - tau w_x, x0, 0.1)
+ *               normpdf(y_j - tau w_z, y0, 0.1)
+ * at tau = eTrafo[n] with int sigma = dE sum_{k <= n} sigma_0(E_k), written
+ * where both normpdf factors are >= 1e-6 (other entries keep their value). */
+typedef struct titus_beam_v1 {
+  double x0, y0;              /* settings.x0, settings.y0 (pos_beam) */
+  double Omega1, Omega3;      /* settings.Omega1, settings.Omega3 */
+  const double* xmid;         /* nx: settings.xMid */
+  const double* ymid;         /* ny: settings.yMid */
+  const double* tau;          /* ne: csd.eTrafo */
+} titus_beam_v1;
+/* Set the analytic beam of a TITUS_INTEGRATOR_RK4 context (before its first
+ * step; required by it). */
+int titus_set_analytic_beam(struct titus_ctx* ctx, const titus_beam_v1* beam);
 
 /* Rejection of the rank-adaptive step (SolverCSD.jl:2204-2233, 2282-2301),
  * BUG integrator only, before the first step.  c_eta > 0 turns it on (the

@@ -30,6 +30,7 @@ const TITUS_ABI_VERSION = Int32(3)
 const TITUS_INTEGRATOR_FIXED = Int32(0)
 const TITUS_INTEGRATOR_BUG = Int32(1)
 const TITUS_INTEGRATOR_PARALLEL = Int32(2)
+const TITUS_INTEGRATOR_RK4 = Int32(3)
 const TITUS_PRECISION_FP64 = Int32(0)
 const TITUS_PRECISION_MIXED = Int32(1)
 const TITUS_PRECISION_MIXED_FULL_FP32 = Int32(2)
@@ -81,6 +82,17 @@ struct TitusProblemV1
     absAy::Ptr{Float64}
     psi0::Ptr{Float64}
     flux_order::Int32
+end
+
+# mirror of `titus_beam_v1`: the RK4 variant's analytic pencil beam
+struct TitusBeamV1
+    x0::Float64
+    y0::Float64
+    Omega1::Float64
+    Omega3::Float64
+    xmid::Ptr{Float64}
+    ymid::Ptr{Float64}
+    tau::Ptr{Float64}
 end
 
 struct TitusError <: Exception
@@ -356,6 +368,31 @@ function rejections(ctx::TitusContext)
     n = Ref{Int64}(0)
     check(ccall((:titus_get_rejections, libtitus), Cint, (Ptr{Cvoid}, Ptr{Int64}), ctx.ptr, n))
     return Int(n[])
+end
+
+"""CudaSolveFirstCollisionSourceDLR4thOrder(obj) on the MI355X (SolverCSD.jl:1422-1724):
+fixed rank, RK4 on the central streaming terms, the analytic uncollided flux
+of the pencil beam at (x0, y0) along (Omega1, Omega3).  Returns the outputs of
+the fixed-rank form with psi as C x nq, as the reference returns Matrix(psi)."""
+function CudaSolveFirstCollisionSourceDLR4thOrder(obj; precision=TITUS_PRECISION_FP64)
+    ctx = create(obj, TITUS_INTEGRATOR_RK4; precision=precision)
+    try
+        s = obj.settings
+        xmid, ymid = Float64.(s.xMid), Float64.(s.yMid)
+        tau = Float64.(obj.csd.eTrafo)
+        GC.@preserve xmid ymid tau begin
+            b = TitusBeamV1(Float64(s.x0), Float64(s.y0), Float64(s.Omega1), Float64(s.Omega3),
+                            pointer(xmid), pointer(ymid), pointer(tau))
+            check(ccall((:titus_set_analytic_beam, libtitus), Cint,
+                        (Ptr{Cvoid}, Ref{TitusBeamV1}), ctx.ptr, Ref(b)))
+        end
+        solve!(ctx)
+        out = reference_outputs(obj, ctx)
+        _, _, _, _, psi = get_state(ctx)
+        return out[1:5]..., psi
+    finally
+        destroy!(ctx)
+    end
 end
 
 """SolveFirstCollisionSourceDLRBUGRejection(obj) on the GPU (SolverCSD.jl:2011-2374):

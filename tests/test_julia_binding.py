@@ -21,7 +21,7 @@ JULIA_TO_C = {
     "Cstring": "char*",
     "Ptr{Float64}": "double*", "Ptr{Int32}": "int32_t*",
     "Ptr{Cvoid}": "titus_ctx*", "Ptr{Ptr{Cvoid}}": "titus_ctx**",
-    "Ref{TitusProblemV1}": "titus_problem_v1*",
+    "Ref{TitusProblemV1}": "titus_problem_v1*", "Ref{TitusBeamV1}": "titus_beam_v1*",
     "Ptr{Ptr{Float64}}": "double**", "Ptr{Int64}": "int64_t*",
 }
 
@@ -112,7 +112,8 @@ def test_header_parser_sees_every_declared_function():
 @pytest.mark.parametrize("name", ["titus_problem", "SolveFirstCollisionSourceDLR",
                                   "SolveFirstCollisionSourceDLRBUGRejection",
                                   "SolveFirstCollisionSourceDLRParallel",
-                                  "CSolveFirstCollisionSourceDLR", "plan_objective"])
+                                  "CSolveFirstCollisionSourceDLR", "plan_objective",
+                                  "CudaSolveFirstCollisionSourceDLR4thOrder"])
 def test_binding_defines_the_reference_entry_points(name):
     txt = open(JL).read()
     assert re.search(r"\nfunction " + name + r"\(", txt), name

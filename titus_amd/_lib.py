@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtitus_hip.so")
 
 ABI_VERSION = 3
-INTEGRATOR_FIXED, INTEGRATOR_BUG, INTEGRATOR_PARALLEL = 0, 1, 2
+INTEGRATOR_FIXED, INTEGRATOR_BUG, INTEGRATOR_PARALLEL, INTEGRATOR_RK4 = 0, 1, 2, 3
 PRECISION_FP64, PRECISION_MIXED, PRECISION_MIXED_FULL_FP32 = 0, 1, 2
 
 _P = ctypes.POINTER(ctypes.c_double)
@@ -25,7 +25,8 @@ ERRORS = {
 }
 
 # every symbol include/titus_hip.h declares
-EXPORTED = ("titus_create", "titus_get_cfl", "titus_set_rejection", "titus_get_rejections",
+EXPORTED = ("titus_create", "titus_get_cfl", "titus_set_analytic_beam", "titus_set_rejection",
+            "titus_get_rejections",
             "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
             "titus_get_state",
             "titus_get_history", "titus_set_timing", "titus_kernel_stats", "titus_get_tuning",
@@ -72,6 +73,12 @@ class GridV1(ctypes.Structure):
                 ("inv_density", ctypes.c_void_p)]
 
 
+class BeamV1(ctypes.Structure):
+    """``titus_beam_v1`` (the RK4 integrator's analytic beam)."""
+    _fields_ = [("x0", ctypes.c_double), ("y0", ctypes.c_double), ("Omega1", ctypes.c_double),
+                ("Omega3", ctypes.c_double), ("xmid", _P), ("ymid", _P), ("tau", _P)]
+
+
 class FcsProblemV1(ctypes.Structure):
     """``titus_fcs_problem_v1``."""
     _fields_ = [
@@ -103,6 +110,7 @@ def load():
     lib.titus_step.argtypes = [vp, ctypes.c_int32]
     lib.titus_get_cfl.argtypes = [vp, _P]
     lib.titus_set_rejection.argtypes = [vp, ctypes.c_double]
+    lib.titus_set_analytic_beam.argtypes = [vp, ctypes.POINTER(BeamV1)]
     lib.titus_get_rejections.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     lib.titus_solve.argtypes = [vp, ctypes.c_int32]
     lib.titus_solve_batch.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32, _PI]
@@ -137,7 +145,8 @@ def load():
     lib.titus_galerkin_noalloc.restype = ctypes.c_int
     lib.titus_last_error.restype = ctypes.c_char_p
     lib.titus_abi_version.restype = ctypes.c_int32
-    for name in ("titus_create", "titus_get_cfl", "titus_set_rejection", "titus_get_rejections",
+    for name in ("titus_create", "titus_get_cfl", "titus_set_analytic_beam", "titus_set_rejection",
+                 "titus_get_rejections",
                  "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
                  "titus_get_state", "titus_get_history", "titus_set_timing",
                  "titus_kernel_stats", "titus_get_tuning", "titus_get_decisions",
@@ -299,6 +308,15 @@ class Context:
         check(self._lib.titus_get_cfl(self.h, ctypes.byref(v)))
         return v.value
 
+    def set_analytic_beam(self, P):
+        """The analytic beam of the RK4 integrator from a problem dict
+        (``beam``, ``xMid``, ``yMid``, ``eTrafo``)."""
+        keep = [np.ascontiguousarray(P[k], dtype=np.float64) for k in ("xMid", "yMid", "eTrafo")]
+        b = P["beam"]
+        st = BeamV1(b["x0"], b["y0"], b["Omega1"], b["Omega3"], _dp(keep[0]), _dp(keep[1]),
+                    _dp(keep[2]))
+        check(self._lib.titus_set_analytic_beam(self.h, ctypes.byref(st)))
+
     def set_rejection(self, c_eta):
         """Turn the rank-adaptive step's rejection on (c_eta > 0) or off (0)."""
         check(self._lib.titus_set_rejection(self.h, float(c_eta)))
@@ -339,7 +357,7 @@ class Context:
         """Time every `period`-th step (True: every step; 0 / False: off)."""
         check(self._lib.titus_set_timing(self.h, int(period)))
 
-    KERNELS = ("bc", "dose", "sweep", "stream_K", "galerkin", "gram", "gemm", "chol")
+    KERNELS = ("bc", "dose", "sweep", "stream_K", "galerkin", "gram", "gemm", "chol", "psi")
     HOST_PHASES = ("host:step", "host:wait", "host:range_finish", "host:truncate", "host:range_w",
                    "host:galerkin_w", "host:stream_LS")
 

@@ -277,7 +277,8 @@ struct titus_ctx {
 
   ~titus_ctx() {
     void* ptrs[] = {rho, irho, omega, bc_base, m0, psi, psi2, dose, u0p, X, Xn, K, Kc, Q1, Q2, partial, arena,
-                    upl, bcells, cand_d, bmask, restore, X_start, dose_start, u0_start, psi_start};
+                    upl, bcells, cand_d, bmask, restore, X_start, dose_start, u0_start, psi_start,
+                    xmid_d, ymid_d};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     void* hptrs[] = {h_arena, h_upl};
     for (void* p : hptrs) if (p) (void)hipHostFree(p);
@@ -714,23 +715,26 @@ struct titus_ctx {
     return titus::matmul(M, DW, true, false);                  // nq x r
   }
   // K = X S + dE psi MtDW with ghost rows zeroed (SolverCSD.jl:1079-1083)
-  void scatter_K(const HMat& MtDW) {
+  // (mask: the ghost rows zeroed, as SolverCSD.jl:1083; NULL keeps them, as
+  // the CUDA.jl variant does, :1651)
+  void scatter_K(const HMat& MtDW, const unsigned char* mask) {
     gemm(X, rank, S, 1.0, 0.0, K, nullptr);
     HMat T = MtDW;
     for (double& v : T.a) v *= dE;
     const double ps = mixed ? 4.0 : 8.0;
     if (T.a.size() <= (size_t)dev::kArgMat) {
       timed("gemm", ps * C * nq + 16.0 * C * T.n, 2.0 * C * nq * T.n, [&] {
-        dev::launch_gemm_args(st, C, psi, tpsi, C, nq, T.a.data(), T.n, 1.0, 1.0, K, C, bmask,
+        dev::launch_gemm_args(st, C, psi, tpsi, C, nq, T.a.data(), T.n, 1.0, 1.0, K, C, mask,
                               full32);
       });
       return;
     }
     const double* Td = upload(T);
     timed("gemm", ps * C * nq + 16.0 * C * T.n, 2.0 * C * nq * T.n, [&] {
-      dev::launch_gemm(st, C, psi, tpsi, C, nq, Td, T.n, 1.0, 1.0, K, C, bmask, full32);
+      dev::launch_gemm(st, C, psi, tpsi, C, nq, Td, T.n, 1.0, 1.0, K, C, mask, full32);
     });
   }
+  void scatter_K(const HMat& MtDW) { scatter_K(MtDW, bmask); }
   // arena[off : off + m*nq] = Xa^T psi (the psi contractions; FP32 in MIXED_FULL_FP32)
   size_t tn_psi(const double* Xa, int m) {
     const size_t off = aalloc((size_t)m * nq);
@@ -807,9 +811,20 @@ struct titus_ctx {
       fetch_part();
     }
     out_scatter(Dout);
-    // in-scattering (:1079-1102)
+    in_scatter_fixed(Dvec, xtp0, bmask);
+    dose_half(stopping[s]);
+  }
+
+  // In-scattering of the fixed-rank step (SolverCSD.jl:1079-1102; the CUDA.jl
+  // variant :1649-1676 with mask = NULL): K = X S + dE psi M^T D W, new X from
+  // range(K), L = W S^T + dE D M (psi^T X), new W from range(L), then
+  // S = MUp S NUp^T + dE (X^T psi) M^T D W.  xtp0: arena offset of X^T psi
+  // (old X), fetched in part before the call.
+  void in_scatter_fixed(const std::vector<double>& Dvec, size_t xtp0, const unsigned char* mask) {
+    const int r = rank;
+    const size_t cr = sizeof(double) * (size_t)C * r;
     const HMat MtDW = mtdw(Dvec, W);
-    scatter_K(MtDW);
+    scatter_K(MtDW, mask);
     RangeJob j = range_launch(K, r, nullptr, 0);
     // the new W on the host while the device runs the K range job
     wait_part();
@@ -840,6 +855,136 @@ struct titus_ctx {
       tape->A3 = XtP; tape->S5 = S;
       HIPCHK(hipMemcpyAsync(tape->X3, X, cr, hipMemcpyDeviceToDevice, st));
     }
+  }
+
+  // ---- the CUDA.jl variant: CudaSolveFirstCollisionSourceDLR4thOrder ----------
+  // (SolverCSD.jl:1422-1724): fixed rank; the streaming K-, L- and S-steps
+  // integrate the central terms only (L2x with Ax, L2y with Az) with RK4; the
+  // uncollided flux is the analytic beam (k_analytic_psi) instead of a sweep.
+  bool beam_set = false;
+  double beam_x0 = 0, beam_y0 = 0, beam_om1 = 0, beam_om3 = 0, int_sigma = 0;
+  double *xmid_d = nullptr, *ymid_d = nullptr;
+  std::vector<double> tau;
+  // classical RK4 of Y' = f(Y) over dE on the host (the reference's stages)
+  template <class F>
+  HMat rk4_host(const HMat& Y, F&& f) const {
+    const double h = 0.5 * dE;
+    auto axpy = [](const HMat& a, double c, const HMat& b) {
+      HMat o = a;
+      for (size_t i = 0; i < o.a.size(); ++i) o.a[i] += c * b.a[i];
+      return o;
+    };
+    HMat k = f(Y);
+    HMat up = axpy(Y, dE / 6, k);
+    k = f(axpy(Y, h, k));
+    up = axpy(up, 2 * dE / 6, k);
+    k = f(axpy(Y, h, k));
+    up = axpy(up, 2 * dE / 6, k);
+    k = f(axpy(Y, dE, k));
+    return axpy(up, dE / 6, k);
+  }
+  // K <- RK4 of K' = -L2x K Bx - L2y K Bz on the device: each f is one
+  // stencil launch (centre and dissipative blocks zero, dE = 1), the stage
+  // sums are copies and axpys on C x r blocks (Q1: the running sum, Q2: the
+  // stage argument, Kc: the stage value)
+  void rk4_k_device(int r, const HMat& Bx, const HMat& Bz) {
+    HMat Z(r, 6 * r);
+    for (int b = 0; b < r; ++b)
+      for (int a = 0; a < r; ++a) {
+        Z(a, 2 * r + b) = Bx(a, b) / (2.0 * dx);
+        Z(a, 3 * r + b) = -Bx(a, b) / (2.0 * dx);
+        Z(a, 4 * r + b) = Bz(a, b) / (2.0 * dy);
+        Z(a, 5 * r + b) = -Bz(a, b) / (2.0 * dy);
+      }
+    const double* Zd = Z.a.size() <= (size_t)dev::kArgMat ? nullptr : upload(Z);
+    const size_t n = (size_t)C * r, bytes = sizeof(double) * n;
+    auto f = [&](const double* Y) {
+      timed("stream_K", 8.0 * C * (2 * r + 1), 8.0 * C * r * r, [&] {
+        if (Zd) dev::launch_stream_k(st, g, Y, C, r, Zd, r, 1.0, Kc, C, false);
+        else dev::launch_stream_k_args(st, g, Y, C, r, Z.a.data(), r, 1.0, Kc, C, false);
+      });
+    };
+    auto copy = [&](double* dst, const double* src) {
+      HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
+    };
+    const double h = 0.5 * dE;
+    f(K);
+    copy(Q1, K); dev::launch_axpy(st, n, dE / 6, Kc, Q1);
+    copy(Q2, K); dev::launch_axpy(st, n, h, Kc, Q2);
+    f(Q2);
+    dev::launch_axpy(st, n, 2 * dE / 6, Kc, Q1);
+    copy(Q2, K); dev::launch_axpy(st, n, h, Kc, Q2);
+    f(Q2);
+    dev::launch_axpy(st, n, 2 * dE / 6, Kc, Q1);
+    copy(Q2, K); dev::launch_axpy(st, n, dE, Kc, Q2);
+    f(Q2);
+    copy(K, Q1); dev::launch_axpy(st, n, dE / 6, Kc, K);
+  }
+  void step_rk4(int s) {
+    if (!beam_set) throw TitusError(TITUS_E_STATE, "RK4 variant: set the analytic beam before the first step");
+    const std::vector<double> sg = sig(s);
+    dose_half(stopping[s - 1]);
+    int_sigma += dE * sg[0];
+    const double ps = mixed ? 4.0 : 8.0;
+    timed("psi", ps * C * nq + 8.0 * C, 20.0 * C * nq, [&] {
+      dev::launch_analytic_psi(st, g, psi, tpsi, nq, omega, xmid_d, ymid_d, beam_x0, beam_y0,
+                               beam_om1, beam_om3, tau[s], int_sigma, m0, u0p);
+    });
+    std::vector<double> Dvec(nm), Dout(nm);
+    for (int i = 0; i < nm; ++i) {
+      Dvec[i] = sg[degree[i]];
+      Dout[i] = sg[0] - Dvec[i];
+    }
+    const int r = rank;
+    size_t xtp0 = 0;
+    if (s > 1) {
+      // K-step (:1580-1597) with X^T L X of the old X on the device meanwhile
+      areset();
+      const size_t gx = galerkin_x(X, r, X, r);
+      fetch_part();
+      const AWs aw = apply_a(W);                      // [|Ax| W, |Az| W, Ax W, Az W]
+      gemm(X, r, S, 1.0, 0.0, K, nullptr);
+      rk4_k_device(r, titus::matmul(W, aw[2], true, false), titus::matmul(W, aw[3], true, false));
+      RangeJob j = range_launch(K, r, nullptr, 0);
+      // L-step (:1600-1617) on the host with the old X
+      wait_part();
+      std::vector<HMat> XLX = galerkin_view(gx, r, r);
+      const HMat& Lx = XLX[2];
+      const HMat& Lz = XLX[3];
+      HMat L = rk4_host(titus::matmul(W, S, false, true), [&](const HMat& Y) {
+        HMat a = titus::matmul(titus::spmm(sA[2], Y), Lx, false, true);
+        HMat b = titus::matmul(titus::spmm(sA[3], Y), Lz, false, true);
+        for (size_t i = 0; i < a.a.size(); ++i) a.a[i] = -a.a[i] - b.a[i];
+        return a;
+      });
+      HMat WNew = range_w(L, r, nullptr, -1.0);
+      HMat NUp = titus::matmul(WNew, W, true, false);
+      fetch();
+      range_finish(j, r, Xn);
+      areset();
+      const size_t mu = tn(Xn, r, X, r, dev::OP_NONE);
+      const size_t gn = galerkin_x(Xn, r, Xn, r);
+      const AWs awn = apply_a(WNew);
+      fetch();
+      HMat MUp = view(mu, r, r);
+      XLX = galerkin_view(gn, r, r);
+      W = WNew;
+      std::swap(X, Xn);
+      // S-step (:1621-1634)
+      S = titus::matmul(titus::matmul(MUp, S), NUp, false, true);
+      const HMat Bx = titus::matmul(W, awn[2], true, false), Bz = titus::matmul(W, awn[3], true, false);
+      S = rk4_host(S, [&](const HMat& Y) {
+        HMat a = titus::matmul(titus::matmul(XLX[2], Y), Bx);
+        HMat b = titus::matmul(titus::matmul(XLX[3], Y), Bz);
+        for (size_t i = 0; i < a.a.size(); ++i) a.a[i] = -a.a[i] - b.a[i];
+        return a;
+      });
+    }
+    areset();
+    xtp0 = tn_psi(X, r);
+    fetch_part();
+    out_scatter(Dout);
+    in_scatter_fixed(Dvec, xtp0, nullptr);
     dose_half(stopping[s]);
   }
 
@@ -1160,6 +1305,7 @@ struct titus_ctx {
     HostScope hs(timing, "host:step");
     if (adj) adjoint_checkpoint(*this, s);
     if (integrator == TITUS_INTEGRATOR_FIXED) step_fixed(s);
+    else if (integrator == TITUS_INTEGRATOR_RK4) step_rk4(s);
     else if (integrator == TITUS_INTEGRATOR_PARALLEL) step_parallel(s);
     else step_bug(s);
     steps_done = s;

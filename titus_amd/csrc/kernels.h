@@ -60,6 +60,15 @@ int sweep_groups(const Grid& g, int nq, const SweepPlan& plan);
 void launch_bc(hipStream_t st, void* psi, DT tpsi, int C, int nq, const int* bcells, int nb,
                const double* bc_base, double scale, const double* m0, double* u0p, int G);
 
+// Analytic uncollided flux of the CUDA.jl variant (SolverCSD.jl:1549-1557) at
+// transformed energy tau with the running int sigma: psi[c, q] set where both
+// spatial normpdf factors of the beam are >= 1e-6, kept elsewhere; u0p (one
+// partial) receives the l = 0 moment of the result.  2D grids.
+void launch_analytic_psi(hipStream_t st, const Grid& g, void* psi, DT tpsi, int nq,
+                         const double* omega, const double* xmid, const double* ymid, double x0,
+                         double y0, double om1, double om3, double tau, double int_sigma,
+                         const double* m0, double* u0p);
+
 // dose[c] += coef * (X[c,:] . v + u0[c]) / rho[c]   (SolverCSD.jl:990-997), FP64
 void launch_dose(hipStream_t st, const Grid& g, double* dose, const double* X, int ldx, int r,
                  const double* v, const double* u0p, int G, double coef);

@@ -87,6 +87,46 @@ __global__ void k_dose(double* __restrict__ dose, const double* __restrict__ X, 
   dose[c] += coef * (acc + u0) / rho[c];
 }
 
+// Analytic uncollided flux of the CUDA.jl variant (SolverCSD.jl:1549-1557),
+// one thread per cell c = i*ny + j: for every ordinate q,
+//   psi[c, q] = amp_q * normpdf(xmid[i] - tau w_x, x0, 0.1)
+//                     * normpdf(ymid[j] - tau w_z, y0, 0.1),
+//   amp_q = 1e5 exp(-1e4 (Om1 - w_x)^2) exp(-1e4 (Om3 - w_z)^2) exp(-int_sigma),
+// written only where both normpdf factors are >= 1e-6 (other entries keep
+// their value, as the reference's loop skips them); then the l = 0 moment of
+// the new psi into u0p (one partial).
+__device__ inline double normpdf01(double x, double mu) {
+  const double sigma = 0.1;
+  return 1.0 / (sigma * sqrt(2.0 * M_PI)) * exp(-((x - mu) * (x - mu)) / 2.0 / (sigma * sigma));
+}
+
+template <class TP>
+__global__ void k_analytic_psi(TP* __restrict__ psi, int C, int ny, int nq,
+                               const double* __restrict__ omega, const double* __restrict__ xmid,
+                               const double* __restrict__ ymid, double x0, double y0, double om1,
+                               double om3, double tau, double int_sigma,
+                               const double* __restrict__ m0, double* __restrict__ u0p) {
+  const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  const int i = c / ny, j = c - (c / ny) * ny;
+  const double xi = xmid[i], yj = ymid[j];
+  const double att = exp(-int_sigma);
+  double u0 = 0.0;
+  for (int q = 0; q < nq; ++q) {
+    const double wx = omega[q], wz = omega[(size_t)2 * nq + q];
+    const double by = normpdf01(yj - tau * wz, y0);
+    const double bx = normpdf01(xi - tau * wx, x0);
+    TP* p = psi + (size_t)q * C + c;
+    if (by >= 1e-6 && bx >= 1e-6) {
+      const double amp = 1e5 * exp(-10000.0 * ((om1 - wx) * (om1 - wx))) *
+                         exp(-10000.0 * ((om3 - wz) * (om3 - wz))) * att;
+      *p = (TP)(amp * bx * by);
+    }
+    u0 += (double)(*p) * m0[q];
+  }
+  u0p[c] = u0;
+}
+
 // Uncollided upwind update.  Each wavefront owns a strip of H rows (index i)
 // by kSweepCols columns (index j, lanes 1..kSweepCols; lanes 0 and 63 carry the
 // neighbouring columns) and walks the ordinates of its group q = gq*qper ...
@@ -812,6 +852,21 @@ void launch_bc(hipStream_t st, void* psi, DT tpsi, int C, int nq, const int* bce
     hipLaunchKernelGGL(k_bc<float>, dim3(blocks_for(nb)), dim3(kBlock), 0, st,
                        static_cast<float*>(psi), C, nq, bcells, nb, bc_base, scale, m0, u0p, G);
   check_launch("bc");
+}
+
+void launch_analytic_psi(hipStream_t st, const Grid& g, void* psi, DT tpsi, int nq,
+                         const double* omega, const double* xmid, const double* ymid, double x0,
+                         double y0, double om1, double om3, double tau, double int_sigma,
+                         const double* m0, double* u0p) {
+  if (tpsi == DT::F64)
+    hipLaunchKernelGGL(k_analytic_psi<double>, dim3(blocks_for(g.C)), dim3(kBlock), 0, st,
+                       static_cast<double*>(psi), g.C, g.ny, nq, omega, xmid, ymid, x0, y0, om1,
+                       om3, tau, int_sigma, m0, u0p);
+  else
+    hipLaunchKernelGGL(k_analytic_psi<float>, dim3(blocks_for(g.C)), dim3(kBlock), 0, st,
+                       static_cast<float*>(psi), g.C, g.ny, nq, omega, xmid, ymid, x0, y0, om1,
+                       om3, tau, int_sigma, m0, u0p);
+  check_launch("analytic_psi");
 }
 
 void launch_dose(hipStream_t st, const Grid& g, double* dose, const double* X, int ldx, int r,

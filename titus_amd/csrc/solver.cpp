@@ -85,7 +85,7 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     require(p->nrestore == 0 || p->restore_cells, "missing operand: restore_cells");
     require(std::isfinite(p->dE) && p->dE > 0.0, "energy step must be positive and finite");
     require(p->integrator == TITUS_INTEGRATOR_FIXED || p->integrator == TITUS_INTEGRATOR_BUG ||
-                p->integrator == TITUS_INTEGRATOR_PARALLEL,
+                p->integrator == TITUS_INTEGRATOR_PARALLEL || p->integrator == TITUS_INTEGRATOR_RK4,
             "unknown integrator");
     require(p->precision == TITUS_PRECISION_FP64 || p->precision == TITUS_PRECISION_MIXED ||
                 p->precision == TITUS_PRECISION_MIXED_FULL_FP32,
@@ -95,6 +95,7 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     require(flux_order == 1 || flux_order == 2, "flux order must be 1 (upwind) or 2 (limited)");
     require(flux_order == 1 || nz == 1, "the limited flux is 2D only");
     require((long long)p->nx * p->ny * nz < (1LL << 31), "grid too large");
+    require(p->integrator != TITUS_INTEGRATOR_RK4 || nz == 1, "the RK4 variant is 2D only");
     const int C = p->nx * p->ny * nz;
     for (int c = 0; c < C; ++c)
       require(std::isfinite(p->density[c]) && p->density[c] > 0.0, "density must be positive and finite");
@@ -126,11 +127,11 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     x.dx = p->dx; x.dy = p->dy; x.dE = p->dE; x.theta = p->theta; x.theta_index = p->theta_index;
     x.rank_tol = p->rank_tol > 0.0 ? p->rank_tol : 1e-7;
     x.rank = p->r;
-    const bool adaptive = p->integrator != TITUS_INTEGRATOR_FIXED;
+    const bool adaptive = p->integrator == TITUS_INTEGRATOR_BUG || p->integrator == TITUS_INTEGRATOR_PARALLEL;
     x.rank_cap = adaptive ? p->rank_cap : p->r;
     require(x.rank_cap >= x.rank, "rank cap below the initial rank");
     x.capx = adaptive ? 2 * x.rank_cap : x.rank;
-    require(x.capx <= p->nm || p->integrator == TITUS_INTEGRATOR_FIXED, "2 * rank cap exceeds the moment count");
+    require(x.capx <= p->nm || !adaptive, "2 * rank cap exceeds the moment count");
     require(x.rank <= p->nm, "rank exceeds the moment count");
     require(x.capx <= 96, "rank cap too large for the device Cholesky (2 r <= 96)");
     require(x.capx + 8 <= p->ncand, "too few completion candidates for the rank cap");
@@ -179,7 +180,8 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     x.X = dalloc<double>(cx);
     x.Xn = dalloc<double>(cx);
     x.K = dalloc<double>(cx);
-    if (p->integrator == TITUS_INTEGRATOR_PARALLEL) x.Kc = dalloc<double>(cx);
+    if (p->integrator == TITUS_INTEGRATOR_PARALLEL || p->integrator == TITUS_INTEGRATOR_RK4)
+      x.Kc = dalloc<double>(cx);
     x.Q1 = dalloc<double>(cx);
     x.Q2 = dalloc<double>(cx);
     const int wmax = std::max(x.capx, p->nq);
@@ -222,8 +224,9 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     HIPCHK(hipMemset(x.dose, 0, sizeof(double) * C));
     // psi starts at zero, so does its moment
     HIPCHK(hipMemset(x.u0p, 0, sizeof(double) * dev::kMaxSweepGroups * C));
-    tune_sweep(x);
-    x.u0_groups = dev::sweep_groups(x.g, p->nq, x.sweep_plan);
+    // the RK4 variant has no sweep: its analytic flux writes one moment partial
+    if (p->integrator != TITUS_INTEGRATOR_RK4) tune_sweep(x);
+    x.u0_groups = p->integrator == TITUS_INTEGRATOR_RK4 ? 1 : dev::sweep_groups(x.g, p->nq, x.sweep_plan);
     HIPCHK(hipMemset(x.u0p, 0, sizeof(double) * dev::kMaxSweepGroups * C));
     HIPCHK(hipMemset(x.psi2, 0, psz * (size_t)C * p->nq));
     HIPCHK(hipMemset(x.X, 0, sizeof(double) * cx));
@@ -327,6 +330,31 @@ int titus_solve_batch(titus_ctx** ctxs, int32_t n, int32_t nthreads, int32_t* st
     }
   }
   return first;
+}
+
+int titus_set_analytic_beam(titus_ctx* ctx, const titus_beam_v1* b) {
+  return guarded([&] {
+    require(ctx != nullptr && b != nullptr, "null argument");
+    require(b->xmid && b->ymid && b->tau, "missing operand");
+    if (ctx->integrator != TITUS_INTEGRATOR_RK4)
+      throw TitusError(TITUS_E_ARG, "analytic beam: RK4 integrator only");
+    if (ctx->steps_done != 0) throw TitusError(TITUS_E_STATE, "the beam must be set before the first step");
+    for (double v : {b->x0, b->y0, b->Omega1, b->Omega3})
+      require(std::isfinite(v), "beam parameters must be finite");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (!ctx->xmid_d) {
+      ctx->xmid_d = dalloc<double>(ctx->nx);
+      ctx->ymid_d = dalloc<double>(ctx->ny);
+    }
+    HIPCHK(hipMemcpy(ctx->xmid_d, b->xmid, sizeof(double) * ctx->nx, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->ymid_d, b->ymid, sizeof(double) * ctx->ny, hipMemcpyHostToDevice));
+    ctx->tau.assign(b->tau, b->tau + ctx->ne);
+    ctx->beam_x0 = b->x0; ctx->beam_y0 = b->y0;
+    ctx->beam_om1 = b->Omega1; ctx->beam_om3 = b->Omega3;
+    // int sigma starts at dE sigma_0(E_1) (SolverCSD.jl:1532)
+    ctx->int_sigma = ctx->dE * ctx->sigma[0];
+    ctx->beam_set = true;
+  });
 }
 
 int titus_set_rejection(titus_ctx* ctx, double c_eta) {

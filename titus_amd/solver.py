@@ -134,6 +134,25 @@ def CSolveFirstCollisionSourceDLR(obj):
     return _finish(obj, P, st)
 
 
+def CudaSolveFirstCollisionSourceDLR4thOrder(obj, recordGif=False):
+    """The reference's CUDA.jl RK4 variant (``SolverCSD.jl:1422-1724``) on the
+    GPU: fixed rank, the central streaming terms only (L2x with Ax, L2y with
+    Az) integrated by RK4 in the K-, L- and S-steps, and the analytic
+    uncollided flux of its pencil beam instead of a sweep.  Returns
+    ``(X U, Sigma/sqrt2, O W V, W V, dose, psi)`` with psi C x Q, as there."""
+    P = obj.problem(INTEGRATOR_FIXED)
+    ctx = _lib.Context(P, _lib.INTEGRATOR_RK4, device=obj.device, precision=_PRECISIONS[obj.precision],
+                       rank_tol=obj.tolerance(INTEGRATOR_FIXED))
+    try:
+        ctx.set_analytic_beam(P)
+        ctx.solve(0)
+        st = ctx.state()
+    finally:
+        ctx.close()
+    out = _finish(obj, P, st)
+    return (*out[:5], st["psi"])
+
+
 def SolveFirstCollisionSourceDLRBUGRejection(obj, recordGif=False, rejection=False):
     """Rank-adaptive augmented-BUG solve on the GPU.  ``rejection=True`` turns
     on the step rejection of ``SolverCSD.jl:2204-2233, 2282-2301`` with
