@@ -357,7 +357,7 @@ class Context:
         """Time every `period`-th step (True: every step; 0 / False: off)."""
         check(self._lib.titus_set_timing(self.h, int(period)))
 
-    KERNELS = ("bc", "dose", "sweep", "stream_K", "galerkin", "gram", "gemm", "chol", "psi")
+    KERNELS = ("bc", "dose", "sweep", "stream_K", "galerkin", "gram", "gemm", "chol", "psi", "angular")
     HOST_PHASES = ("host:step", "host:wait", "host:range_finish", "host:truncate", "host:range_w",
                    "host:galerkin_w", "host:stream_LS")
 

@@ -45,8 +45,8 @@ def _run(cmd):
 
 def build(verbose=False):
     hipcc = _hipcc()
-    srcs = [os.path.join(CSRC, f) for f in ("kernels.hip", "solver.cpp", "adjoint.cpp", "fullpn.cpp",
-                                            "primitives.hip", "smallla.cpp")]
+    srcs = [os.path.join(CSRC, f) for f in ("kernels.hip", "angular.hip", "solver.cpp", "adjoint.cpp",
+                                            "fullpn.cpp", "primitives.hip", "smallla.cpp")]
     tmp = LIB + ".tmp"
     _run([hipcc, *HIP_FLAGS, "-shared", "-I", os.path.join(ROOT, "include"), *srcs, "-o", tmp])
     os.replace(tmp, LIB)

@@ -40,6 +40,7 @@
 #include <vector>
 
 #include "../../include/titus_hip.h"
+#include "angular.h"
 #include "kernels.h"
 #include "smallla.h"
 
@@ -265,6 +266,12 @@ struct titus_ctx {
   // adjoint: checkpoints and recomputation (adjoint.cpp)
   std::shared_ptr<AdjointStore> adj;
   StepTape* tape = nullptr;
+  // angular operators on the device (compressed rows of sA, in its order) and
+  // the blocks of the angular products: AW = A_o W (nops x nm x capx), Z of
+  // the K-step; the term order of the streaming sums
+  dev::AngOps angops{};
+  int *ang_rowptr = nullptr, *ang_col = nullptr, *ang_opnz = nullptr, *ang_order = nullptr;
+  double *ang_val = nullptr, *AWd = nullptr, *Zd = nullptr;
   // rejection of the rank-adaptive step (SolverCSD.jl:2204-2233, 2282-2301):
   // c_eta > 0 turns it on; the state at the start of each step is kept so
   // that a rejected step restarts from it
@@ -278,7 +285,7 @@ struct titus_ctx {
   ~titus_ctx() {
     void* ptrs[] = {rho, irho, omega, bc_base, m0, psi, psi2, dose, u0p, X, Xn, K, Kc, Q1, Q2, partial, arena,
                     upl, bcells, cand_d, bmask, restore, X_start, dose_start, u0_start, psi_start,
-                    xmid_d, ymid_d};
+                    xmid_d, ymid_d, ang_rowptr, ang_col, ang_opnz, ang_order, ang_val, AWd, Zd};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     void* hptrs[] = {h_arena, h_upl};
     for (void* p : hptrs) if (p) (void)hipHostFree(p);
@@ -517,6 +524,66 @@ struct titus_ctx {
     const double* Zd = upload(Z);
     timed("stream_K", 8.0 * C * (2 * r + 1), 2.0 * (2 * nd + 2) * C * r * r,
           [&] { dev::launch_stream_k(st, g, X, C, r, Zd, r, dE, K, C, full32); });
+  }
+
+  // ---- angular products on the device ---------------------------------------------
+  // The operators' compressed rows and the term order, copied once (titus_create).
+  void setup_angular() {
+    const int nops_ = (int)sA.size();
+    std::vector<int> rp, cl, nz;
+    std::vector<double> vl;
+    for (const titus::SpMat& A : sA) {
+      nz.push_back((int)cl.size());
+      rp.insert(rp.end(), A.rowptr.begin(), A.rowptr.end());
+      cl.insert(cl.end(), A.col.begin(), A.col.end());
+      vl.insert(vl.end(), A.val.begin(), A.val.end());
+    }
+    const std::vector<int> ord = term_order();
+    ang_rowptr = dalloc<int>(rp.size());
+    ang_col = dalloc<int>(cl.size());
+    ang_val = dalloc<double>(vl.size());
+    ang_opnz = dalloc<int>(nz.size());
+    ang_order = dalloc<int>(ord.size());
+    HIPCHK(hipMemcpy(ang_rowptr, rp.data(), sizeof(int) * rp.size(), hipMemcpyHostToDevice));
+    if (!cl.empty()) {
+      HIPCHK(hipMemcpy(ang_col, cl.data(), sizeof(int) * cl.size(), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(ang_val, vl.data(), sizeof(double) * vl.size(), hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(ang_opnz, nz.data(), sizeof(int) * nz.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ang_order, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice));
+    angops = dev::AngOps{nm, nops_, ang_rowptr, ang_col, ang_val, ang_opnz};
+    AWd = dalloc<double>((size_t)nops_ * nm * capx);
+    Zd = dalloc<double>((size_t)capx * (2 * nd + 2) * capx);
+  }
+  // arena offset of [W^T A_o W for every operator] for a device copy Wdev of W
+  // (nm x k); AWd receives A_o W
+  size_t galerkin_w_dev(const double* Wdev, int k) {
+    const size_t off = aalloc((size_t)nops() * k * k);
+    timed("angular", 8.0 * nm * k * (1 + nops()), 2.0 * nops() * k * (angops_nnz() + (double)nm * k),
+          [&] { dev::launch_ang_apply(st, angops, Wdev, k, Wdev, k, AWd, arena + off); });
+    return off;
+  }
+  double angops_nnz() const {
+    double n = 0.0;
+    for (const titus::SpMat& A : sA) n += (double)A.val.size();
+    return n / std::max<size_t>(sA.size(), 1);
+  }
+  // The streaming K-step with its coefficients formed on the device from the
+  // Galerkin blocks at arena offset gw (W^T A_o W of the current W)
+  void stream_k_dev(int r, const double* Sdev, size_t gw) {
+    timed("angular", 8.0 * (2.0 * nops() * r * r + (2 * nd + 2) * r * r), 2.0 * (2 * nd + 1) * r * r * r,
+          [&] { dev::launch_ang_stream_z(st, r, r, nd, dx, dy, dz, Sdev, arena + gw, Zd); });
+    timed("stream_K", 8.0 * C * (2 * r + 1), 2.0 * (2 * nd + 2) * C * r * r,
+          [&] { dev::launch_stream_k(st, g, X, C, r, Zd, r, dE, K, C, full32); });
+  }
+  // arena offset of the L-step L = W S^T - dE sum_o A_o W S^T (X^T L_o X)^T
+  // (nm x r) from AWd (of this W) and the Galerkin blocks at arena offset gx
+  size_t stream_l_dev(int r, const double* Wdev, const double* Sdev, size_t gx) {
+    const size_t off = aalloc((size_t)nm * r);
+    timed("angular", 8.0 * nm * r * (2 + nops()), 2.0 * nops() * nm * r * r,
+          [&] { dev::launch_ang_stream_l(st, nm, r, nops(), ang_order, dE, Wdev, Sdev, AWd,
+                                         arena + gx, arena + off); });
+    return off;
   }
 
   // ---- device range basis --------------------------------------------------------
@@ -762,29 +829,33 @@ struct titus_ctx {
     size_t xtp0 = 0;
     if (s > 1) {
       // K-step and old Galerkin products (SolverCSD.jl:1019-1038)
+      // the Galerkin products of X and W, the L-step (:1033-1046) and the
+      // K-step coefficients on the device; the host takes L as soon as it is
+      // there while the device runs the K-step and its range job
       areset();
       const size_t gx = galerkin_x(X, r, X, r);
+      const double* Wdev = upload(W);
+      const double* Sdev = upload(S);
+      const size_t gw = galerkin_w_dev(Wdev, r);
+      const size_t lo = stream_l_dev(r, Wdev, Sdev, gx);
       fetch_part();
-      const AWs aw = apply_a(W);
-      const std::vector<HMat> WAW = galerkin_from(W, aw);
-      stream_k(r, WAW);
+      stream_k_dev(r, Sdev, gw);
       RangeJob j = range_launch(K, r, nullptr, 0);
-      // the L-step (:1033-1046) on the host while the device runs the K-step
-      // and its range job
       wait_part();
+      const std::vector<HMat> WAW = galerkin_view(gw, r, r);
       std::vector<HMat> XLX = galerkin_view(gx, r, r);
-      HMat L = stream_L(aw, XLX);
+      HMat L = view(lo, nm, r);
       HMat WNew = range_w(L, r, nullptr, -1.0, tape ? &tape->Rw1 : nullptr);
       HMat NUp = titus::matmul(WNew, W, true, false);
       fetch();
       range_finish(j, r, Xn, tape ? &tape->Rx1 : nullptr);
-      // the device writes XNew and its products while the host forms the
-      // new W's Galerkin products
+      // the device writes XNew and the Galerkin products of XNew and WNew
       areset();
       const size_t mu = tn(Xn, r, X, r, dev::OP_NONE);
       const size_t gn = galerkin_x(Xn, r, Xn, r);
-      const std::vector<HMat> WAW1 = galerkin_w(WNew, WNew);
+      const size_t gw1 = galerkin_w_dev(upload(WNew), r);
       fetch();
+      const std::vector<HMat> WAW1 = galerkin_view(gw1, r, r);
       HMat MUp = view(mu, r, r);
       if (tape) {
         tape->WAW = WAW; tape->XLX = XLX; tape->L0 = titus::matmul(W, S, false, true);
@@ -1113,31 +1184,33 @@ struct titus_ctx {
     begin_step(s, Dvec, Dout);
     if (s > 1) {
       const int r = rank;
+      // Galerkin products of X and W, the L-step (old X) and the K-step
+      // coefficients on the device; the new angular basis on the host as soon
+      // as L is there, while the device runs the K-step and its range job
       areset();
       const size_t gx = galerkin_x(X, r, X, r);
+      const double* Wdev = upload(W);
+      const double* Sdev = upload(S);
+      const size_t gw = galerkin_w_dev(Wdev, r);
+      const size_t lo = stream_l_dev(r, Wdev, Sdev, gx);
       fetch_part();
-      const AWs aw = apply_a(W);
-      stream_k(r, galerkin_from(W, aw));
+      stream_k_dev(r, Sdev, gw);
       copy_x_to_aug(r);
       RangeJob j = range_launch(K, r, X, r);
-      // the L-step (old X) and the new angular basis on the host while the
-      // device runs the K-step and its range job
       wait_part();
-      std::vector<HMat> XLX = galerkin_view(gx, r, r);
-      HMat W1 = augment_w(stream_L(aw, XLX));
+      HMat W1 = augment_w(view(lo, nm, r));
       HMat Nu = titus::matmul(W1, W, true, false);
       fetch();
       range_finish(j, r, Xn + (size_t)r * C);               // Xn = X1 = [X | Xt]
-      // products of X1 on the device while the host forms W1^T A W1
+      // products of X1 and W1^T A W1 on the device
       areset();
       const size_t mu = tn(Xn, 2 * r, X, r, dev::OP_NONE);
       const size_t g1 = galerkin_x(Xn, 2 * r, Xn, 2 * r);
-      // A W1 = [A W, A Wt]
-      std::vector<HMat> WAW =
-          galerkin_from(W1, hcat_all(aw, apply_a(titus::cols(W1, r, W1.n))));
+      const size_t gw1 = galerkin_w_dev(upload(W1), 2 * r);
       fetch();
+      std::vector<HMat> WAW = galerkin_view(gw1, 2 * r, 2 * r);
       HMat Mu = view(mu, 2 * r, r);
-      XLX = galerkin_view(g1, 2 * r, 2 * r);
+      std::vector<HMat> XLX = galerkin_view(g1, 2 * r, 2 * r);
       HMat S1 = titus::matmul(titus::matmul(Mu, S), Nu, false, true);   // 2r x 2r
       HMat Sup = stream_S(S1, XLX, WAW);
       // eta diagnostic (:2210-2220): rows r.. of X1^T L X1, columns r.. of W1^T A W1

@@ -189,12 +189,14 @@ int titus_create(titus_ctx** out, const titus_problem_v1* p, int32_t device) {
     const size_t w2 = (size_t)wmax * wmax;
     const size_t ncand_used = std::min<size_t>(p->ncand, (size_t)2 * x.capx + 64);
     // largest phase: Galerkin of the augmented basis plus one range job
-    x.arena_n = 8 * w2 + 10 * (size_t)x.capx * x.capx + ncand_used * 2 * x.capx + 256;
+    x.arena_n = 8 * w2 + 10 * (size_t)x.capx * x.capx + ncand_used * 2 * x.capx + 256 +
+                (size_t)x.nops() * x.capx * x.capx + (size_t)p->nm * x.capx;
     x.arena = dalloc<double>(x.arena_n);
     x.h_arena = halloc<double>(x.arena_n);
     x.upl_n = 64 * w2 + 8192;
     x.upl = dalloc<double>(x.upl_n);
     x.h_upl = halloc<double>(x.upl_n);
+    x.setup_angular();
     x.bcells = dalloc<int>(p->nb);
     x.cand_d = dalloc<int>(p->ncand);
     x.bmask = dalloc<unsigned char>(C);
