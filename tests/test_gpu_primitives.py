@@ -99,3 +99,31 @@ def test_galerkin_products_match_the_oracle(shape, r):
     err = np.max(np.abs(got - ref) / scale)
     print(f"[galerkin {nx}x{ny}x{nz} r={r}] max error / sum|terms| {err:.2e}", flush=True)
     assert err < 1e-13
+
+
+@pytest.mark.parametrize("shape,r", [((200, 200, 1), 18), ((64, 64, 64), 12), ((9, 8, 1), 5),
+                                     ((6, 7, 8), 7)])
+def test_vector_and_per_cell_stencil_paths_agree_bitwise(shape, r):
+    """The stencil products read B/rho with vector loads when the leading
+    dimension and the unit-stride axis allow it, and cell by cell otherwise;
+    both form the same values in the same order, so a leading dimension of
+    C + 1 (per-cell loads) must give the same bits as C (vector loads)."""
+    nx, ny, nz = shape
+    print(f"\n[case] galerkin vector vs per-cell loads {nx}x{ny}x{nz} r={r}", flush=True)
+    C = nx * ny * nz
+    rng = np.random.default_rng(7 * C + r)
+    rho = rng.uniform(0.3, 1.6, C)
+    h = (0.1, 0.07, 0.05)
+    X = rng.standard_normal((C, r))
+    irho = device(1.0 / rho)
+    nops = 4 if nz <= 1 else 6
+    res = []
+    for ld in (C, C + 1 if (C + 1) % 4 else C + 2):
+        Xp = np.zeros((ld, r))
+        Xp[:C] = X
+        Xd = DeviceArray(np.asfortranarray(Xp).ravel(order="F"))
+        out = DeviceArray(n=nops * r * r)
+        g = _lib.GridV1(nx, ny, nz, h[0], h[1], h[2], irho.ptr.value)
+        _lib.check(_lib.load().titus_galerkin(ctypes.byref(g), Xd.ptr, ld, r, out.ptr, None))
+        res.append(out.get())
+    assert np.array_equal(res[0], res[1])

@@ -21,6 +21,11 @@ HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # matrix-core accumulators in ordinary VGPRs: without it the loop-carried
+    # accumulators are copied to and from AGPRs around every MFMA loop trip
+    # (and the copies wait for the MFMA pipeline to drain)
+    # (an AMDGPU backend option; the host pass registers it too and ignores it)
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wno-unused-command-line-argument",
     # host-side small algebra: AVX2 + FMA (every x86-64 server CPU of an
     # MI355X node has them)

@@ -529,19 +529,20 @@ template <> struct Mfma<float> {
 // groups of one chunk (shared A) run on the same XCD.  The four wavefronts'
 // sums are added in LDS in a fixed order.
 // Four consecutive values at p (32-byte aligned for double, 16 for float) in
-// one vector load, converted to TC; zeros when !ok.
+// one vector load, converted to TC; zeros when !ok.  p must be readable
+// whatever ok is: the load is unconditional and the zeros are selected after
+// it, so that no branch (and no wait for the load) splits the caller's
+// stream of loads.
 template <class TS, class TC>
 __device__ inline void load4(const TS* p, bool ok, TC* v) {
-  if (!ok) {
-    v[0] = v[1] = v[2] = v[3] = TC(0);
-    return;
-  }
   if constexpr (sizeof(TS) == 8) {
     const double4 q = *reinterpret_cast<const double4*>(p);
-    v[0] = (TC)q.x; v[1] = (TC)q.y; v[2] = (TC)q.z; v[3] = (TC)q.w;
+    v[0] = ok ? (TC)q.x : TC(0); v[1] = ok ? (TC)q.y : TC(0);
+    v[2] = ok ? (TC)q.z : TC(0); v[3] = ok ? (TC)q.w : TC(0);
   } else {
     const float4 q = *reinterpret_cast<const float4*>(p);
-    v[0] = (TC)q.x; v[1] = (TC)q.y; v[2] = (TC)q.z; v[3] = (TC)q.w;
+    v[0] = ok ? (TC)q.x : TC(0); v[1] = ok ? (TC)q.y : TC(0);
+    v[2] = ok ? (TC)q.z : TC(0); v[3] = ok ? (TC)q.w : TC(0);
   }
 }
 
@@ -583,9 +584,14 @@ __global__ __launch_bounds__(kBlock) void k_tn_mfma(
   for (int o = 0; o < NACC; ++o) acc[o] = acc_t{};
   // plain products over whole 16-cell blocks: each lane's four cells are one
   // aligned vector load when both leading dimensions keep columns aligned
-  const bool vec = NOPS == 1 && lda % 4 == 0 && ldb % 4 == 0 &&
-                   reinterpret_cast<uintptr_t>(A) % 32 == 0 &&
-                   reinterpret_cast<uintptr_t>(B) % (4 * sizeof(TB)) == 0;
+  bool vec = lda % 4 == 0 && ldb % 4 == 0 && reinterpret_cast<uintptr_t>(A) % 32 == 0 &&
+             reinterpret_cast<uintptr_t>(B) % (4 * sizeof(TB)) == 0;
+  // stencil products: the lane's four cells lie on one line of the unit-stride
+  // axis (the last one) when that axis has a multiple of four cells, and the
+  // other axes' neighbours of the four cells are again aligned vectors
+  constexpr int DI = ND - 1;                       // the unit-stride axis
+  if (NOPS > 1)
+    vec = vec && (ND == 2 ? ny : nz) % 4 == 0 && reinterpret_cast<uintptr_t>(irho) % 32 == 0;
   for (int c0 = cbeg; c0 < cend; c0 += 16) {
     const int cs = c0 + 4 * kg;
     TC av[4], bv[NACC][4];
@@ -593,6 +599,70 @@ __global__ __launch_bounds__(kBlock) void k_tn_mfma(
       load4(Acol + cs, va, av);
 #pragma unroll
       for (int t = 0; t < NBT; ++t) load4(Bcol[t] + cs, vb[t], bv[t]);
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int o = 0; o < NACC; ++o) acc[o] = MF::step(av[s], bv[o][s], acc[o]);
+      continue;
+    }
+    if (NOPS > 1 && vec && c0 + 16 <= cend) {
+      // the same stencil values as the per-cell path below, from vector loads:
+      // B/rho at the four cells, at their two outer neighbours on the
+      // unit-stride axis, and at the four cells shifted by +-1 on each other
+      // axis.  Every such load is inside the grid when the cells' coordinates
+      // on the other axes are interior, the only case in which any of the four
+      // cells is.
+      load4(Acol + cs, va, av);
+      const int ii = cs / syz, jj = (cs - ii * syz) / nz, kk = cs - (cs / nz) * nz;
+      const int crd[3] = {ii, jj, kk}, ext[3] = {nx, ny, nz};
+      const int ci = crd[DI];                       // first cell's unit-axis coordinate
+      bool outer = vb[0];
+#pragma unroll
+      for (int d = 0; d < DI; ++d) outer = outer && crd[d] >= 1 && crd[d] <= ext[d] - 2;
+      TC yc[4], ym[ND][4], yp[ND][4];
+      bool in[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) in[s] = outer && ci + s >= 1 && ci + s <= ext[DI] - 2;
+      // every load below is unconditional: off the interior its offsets
+      // collapse onto the lane's own cells (readable), and the cells' in[]
+      // flags select zeros afterwards
+      const TB* b = Bcol[0];
+      TC bq[4], rq[4];
+      load4(b + cs, true, bq);
+      load4(irho + cs, true, rq);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) yc[s] = bq[s] * rq[s];
+      // unit-stride axis: inner neighbours are the lane's own cells; the
+      // outer two are scalar loads (inside the grid when the cell is interior)
+      const int ilo = in[0] ? cs - 1 : cs, ihi = in[3] ? cs + 4 : cs + 3;
+      const TC lo = (TC)b[ilo] * (TC)irho[ilo];
+      const TC hi = (TC)b[ihi] * (TC)irho[ihi];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        ym[DI][s] = s == 0 ? lo : yc[s - 1];
+        yp[DI][s] = s == 3 ? hi : yc[s + 1];
+      }
+#pragma unroll
+      for (int d = 0; d < DI; ++d) {
+        const int off = outer ? stride[d] : 0;
+        TC bm[4], rm[4], bp[4], rp[4];
+        load4(b + cs - off, true, bm);
+        load4(irho + cs - off, true, rm);
+        load4(b + cs + off, true, bp);
+        load4(irho + cs + off, true, rp);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) { ym[d][s] = bm[s] * rm[s]; yp[d][s] = bp[s] * rp[s]; }
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+          const TC vc = in[s] ? yc[s] : TC(0), vm = in[s] ? ym[d][s] : TC(0),
+                   vp = in[s] ? yp[d][s] : TC(0);
+          bv[d % NACC][s] = (vc - h * vm - h * vp) * ih[d];
+          bv[(ND + d) % NACC][s] = (vp - vm) * (h * ih[d]);
+        }
+      }
 #pragma unroll
       for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -676,6 +746,155 @@ __global__ void k_tn_reduce(const double* __restrict__ partial, int nchunks, int
   for (int k = lane; k < nchunks; k += 64) s += partial[(size_t)k * mn + p];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   if (lane == 0) out[p] = s;
+}
+
+// The K-step of k_stream_k on the matrix cores, computed transposed: the
+// result tile K^T[b, c] of 16 columns b0.. by 16 cells c0.. is
+// sum_a Z_d[a, b] Y_d[c, a] over 2 ND + 2 coefficient blocks, where the cell
+// operands Y_d are X itself (Z0), X/rho (centre) and, per axis, the sum and
+// the difference of X/rho at the two neighbours (zero on ghost cells, as in
+// k_stream_k), paired with half the sum and half the difference of the
+// neighbour blocks: y+ Z+ + y- Z- = (y+ + y-)(Z+ + Z-)/2 + (y+ - y-)(Z+ - Z-)/2.
+// The central differences y+ - y- are so formed cell by cell before any
+// product, as the stencils are (the long sums would otherwise carry the large
+// and nearly cancelling neighbour terms apart).
+// A operand (lane l): block d at [a0 + l/16, b0 + l%16] from LDS, staged as
+// Zs[(d*rp + a)*16 + b] with rp = r rounded up to 4 (zero padding).
+// B operand (lane l): cell c0 + l%16 of column a0 + l/16 -- 16 consecutive
+// cells per column, so the loads are coalesced, and so are the stores: a
+// result lane holds one cell of four output columns.  The Z0 products and
+// the stencil sum accumulate apart and combine as a0 - dE * as, as in
+// k_stream_k.  Each wavefront walks `tpw` consecutive 16-cell tiles, two at
+// a time.
+template <class TC, int ND, class Src>
+__global__ __launch_bounds__(kBlock) void k_stream_k_mfma(
+    const double* __restrict__ X, int ldx, int r, int rp, int nx, int ny, int nz,
+    const double* __restrict__ rho, const Src Z, int n, double dE, double* __restrict__ K,
+    int ldk, int tpw) {
+  using MF = Mfma<TC>;
+  using acc_t = typename MF::acc_t;
+  constexpr int NZB = 2 * ND + 2;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  TC* Zs = reinterpret_cast<TC*>(lds_raw);
+  const int C = nx * ny * nz;
+  const int b0 = blockIdx.y * 16;
+  const int nbv = min(16, n - b0);
+  for (int e = threadIdx.x; e < NZB * rp * 16; e += blockDim.x) {
+    const int b = e & 15, a = (e >> 4) % rp, d = (e >> 4) / rp;
+    double v = 0.0;
+    if (b < nbv && a < r) {
+      if (d < 2) {
+        v = Z[((size_t)d * n + b0 + b) * r + a];
+      } else {
+        const int base = 2 + ((d - 2) & ~1);        // the axis' + block; - follows
+        const double zp = Z[((size_t)base * n + b0 + b) * r + a];
+        const double zm = Z[((size_t)(base + 1) * n + b0 + b) * r + a];
+        v = ((d - 2) & 1) ? 0.5 * (zp - zm) : 0.5 * (zp + zm);
+      }
+    }
+    Zs[e] = (TC)v;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cl = lane & 15, kg = lane >> 4;
+  const int L = xcd_block(blockIdx.x, gridDim.x);
+  const int syz = ny * nz;
+  const int stride[3] = {syz, nz, 1}, ext[3] = {nx, ny, nz};
+  const TC tdE = (TC)dE;
+  // two tiles per pass (independent accumulator chains), and the cell values
+  // of the next four columns loaded while the current four are multiplied
+  constexpr int TT = 2, NV = 1 + 2 * ND;
+  const int tile0 = (L * 4 + wave) * tpw;            // tpw is a multiple of TT
+  for (int t = 0; t < tpw; t += TT) {
+    if ((tile0 + t) * 16 >= C) break;
+    int cc[TT], nb[TT][2 * ND];
+    bool live[TT];
+    TC inv[TT][2 * ND], ic[TT];
+#pragma unroll
+    for (int u = 0; u < TT; ++u) {
+      const int c = (tile0 + t + u) * 16 + cl;
+      live[u] = c < C;
+      cc[u] = live[u] ? c : 0;                       // a valid cell for dead lanes
+      const int i = cc[u] / syz, rem = cc[u] - (cc[u] / syz) * syz;
+      const int j = rem / nz, kz = rem - (rem / nz) * nz;
+      const int coord[3] = {i, j, kz};
+      // (bitwise & throughout: no short-circuit branches around the loads)
+      bool in = live[u];
+#pragma unroll
+      for (int d = 0; d < ND; ++d) in = in & (coord[d] >= 1) & (coord[d] <= ext[d] - 2);
+      // densities loaded unconditionally (off the interior the neighbours
+      // collapse onto the cell itself); zeros by a 0/1 factor after the
+      // division (1/rho * 1 is exact), which keeps the divisions unbranched
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        nb[u][2 * d] = in ? cc[u] + stride[d] : cc[u];
+        nb[u][2 * d + 1] = in ? cc[u] - stride[d] : cc[u];
+      }
+      double rn[2 * ND];
+#pragma unroll
+      for (int d = 0; d < 2 * ND; ++d) rn[d] = rho[nb[u][d]];
+      const double rc = rho[cc[u]];
+      const TC m = in ? TC(1) : TC(0);
+#pragma unroll
+      for (int d = 0; d < 2 * ND; ++d) inv[u][d] = (TC)(1.0 / rn[d]) * m;
+      ic[u] = (TC)(1.0 / rc) * m;
+    }
+    // unconditional loads from readable addresses (column 0 for the padding
+    // columns), zeros selected at use: no branch between the loads
+    double cur[TT][NV], nxt[TT][NV];
+    auto fetch = [&](double (&v)[TT][NV], int ab) {
+      const int a = ab + kg;
+      const double* xa = X + (size_t)(a < r ? a : 0) * ldx;
+#pragma unroll
+      for (int u = 0; u < TT; ++u) {
+        v[u][0] = xa[cc[u]];
+#pragma unroll
+        for (int d = 0; d < 2 * ND; ++d) v[u][1 + d] = xa[nb[u][d]];
+      }
+    };
+    fetch(cur, 0);
+    acc_t a0[TT], as[TT];
+#pragma unroll
+    for (int u = 0; u < TT; ++u) { a0[u] = acc_t{}; as[u] = acc_t{}; }
+    for (int ab = 0; ab < rp; ab += 4) {
+      if (ab + 4 < rp) fetch(nxt, ab + 4);
+      const int a = ab + kg;
+      const TC* z = Zs + (size_t)a * 16 + cl;
+      TC za[NZB];
+#pragma unroll
+      for (int d = 0; d < NZB; ++d) za[d] = z[(size_t)d * rp * 16];
+#pragma unroll
+      for (int u = 0; u < TT; ++u) {
+        const bool ok = live[u] && a < r;
+        const TC x0 = ok ? (TC)cur[u][0] : TC(0);
+        const TC xc = x0 * ic[u];
+        TC xn[2 * ND];
+#pragma unroll
+        for (int d = 0; d < 2 * ND; ++d) xn[d] = ok ? (TC)cur[u][1 + d] * inv[u][d] : TC(0);
+        a0[u] = MF::step(za[0], x0, a0[u]);
+        as[u] = MF::step(za[1], xc, as[u]);
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+          as[u] = MF::step(za[2 + 2 * d], xn[2 * d] + xn[2 * d + 1], as[u]);
+          as[u] = MF::step(za[3 + 2 * d], xn[2 * d] - xn[2 * d + 1], as[u]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < TT; ++u)
+#pragma unroll
+        for (int e = 0; e < NV; ++e) cur[u][e] = nxt[u][e];
+    }
+#pragma unroll
+    for (int u = 0; u < TT; ++u) {
+      if (!live[u]) continue;
+      const int c = (tile0 + t + u) * 16 + cl;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int b = MF::row(kg, v);
+        if (b < nbv) K[(size_t)(b0 + b) * ldk + c] = (double)(a0[u][v] - tdE * as[u][v]);
+      }
+    }
+  }
 }
 
 // Cholesky G + s I = R^T R of a k x k SPD matrix (one workgroup, LDS) and the
@@ -995,10 +1214,37 @@ void stream_k_nb(hipStream_t st, const Grid& g, const double* X, int ldx, int r,
   if (g.ndim() == 3) stream_k_nd<TC, NB, 3, false, Src>(st, g, X, ldx, r, Z, n, dE, K, ldk);
   else stream_k_nd<TC, NB, 2, false, Src>(st, g, X, ldx, r, Z, n, dE, K, ldk);
 }
+// The matrix-core form when its coefficient tile fits 64 KiB of LDS; false
+// otherwise.  Each wavefront takes an even number (at least two) of 16-cell
+// tiles, about 4096 workgroups per column tile at most.
+template <class TC, int ND, class Src>
+bool stream_k_mfma_nd(hipStream_t st, const Grid& g, const double* X, int ldx, int r,
+                      const Src& Z, int n, double dE, double* K, int ldk) {
+  const int rp = (r + 3) / 4 * 4;
+  const size_t lds = (size_t)(2 * ND + 2) * rp * 16 * sizeof(TC);
+  if (lds > 65536) return false;
+  const int tiles = (g.C + 15) / 16;
+  const int tpw = std::max(2, (tiles + 16383) / 16384 + 1) / 2 * 2;   // even
+  const int blocks = (tiles + 4 * tpw - 1) / (4 * tpw);
+  hipLaunchKernelGGL((k_stream_k_mfma<TC, ND, Src>), dim3(blocks, (n + 15) / 16), dim3(kBlock),
+                     lds, st, X, ldx, r, rp, g.nx, g.ny, g.nz, g.rho, Z, n, dE, K, ldk, tpw);
+  return true;
+}
+
 template <class Src>
 void stream_k_any(hipStream_t st, const Grid& g, const double* X, int ldx, int r, const Src& Z,
                   int n, double dE, double* K, int ldk, bool fp32) {
-  // column blocks of NB = 16 (8 when r is wide, to bound the LDS footprint)
+  const bool three = g.ndim() == 3;
+  const bool mfma = fp32 ? (three ? stream_k_mfma_nd<float, 3>(st, g, X, ldx, r, Z, n, dE, K, ldk)
+                                  : stream_k_mfma_nd<float, 2>(st, g, X, ldx, r, Z, n, dE, K, ldk))
+                         : (three ? stream_k_mfma_nd<double, 3>(st, g, X, ldx, r, Z, n, dE, K, ldk)
+                                  : stream_k_mfma_nd<double, 2>(st, g, X, ldx, r, Z, n, dE, K, ldk));
+  if (mfma) {
+    check_launch("stream_K");
+    return;
+  }
+  // VALU form: column blocks of NB = 16 (8 when r is wide, to bound the LDS
+  // footprint)
   const bool narrow = r > 48;
   if (fp32) {
     if (narrow) stream_k_nb<float, 8>(st, g, X, ldx, r, Z, n, dE, K, ldk);
