@@ -516,6 +516,25 @@ template <> struct Mfma<float> {
   __device__ static int row(int kg, int v) { return 4 * kg + v; }
 };
 
+// Lane `src` (0..15, a constant after unrolling) of each 16-lane row of the
+// wavefront, to every lane of that row: DPP row_newbcast (dpp_ctrl 0x150 +
+// src), two 32-bit moves for a double.  Register moves only.
+__device__ __forceinline__ int row_bcast_i32(int v, int src) {
+#define TITUS_RB(n) case n: return __builtin_amdgcn_update_dpp(0, v, 0x150 + n, 0xF, 0xF, false);
+  switch (src) {
+    TITUS_RB(0) TITUS_RB(1) TITUS_RB(2) TITUS_RB(3) TITUS_RB(4) TITUS_RB(5) TITUS_RB(6)
+    TITUS_RB(7) TITUS_RB(8) TITUS_RB(9) TITUS_RB(10) TITUS_RB(11) TITUS_RB(12) TITUS_RB(13)
+    TITUS_RB(14) TITUS_RB(15)
+    default: return v;
+  }
+#undef TITUS_RB
+}
+__device__ __forceinline__ double row_bcast(double v, int src) {
+  const int lo = row_bcast_i32(__double2loint(v), src);
+  const int hi = row_bcast_i32(__double2hiint(v), src);
+  return __hiloint2double(hi, lo);
+}
+
 // Tall-skinny A^T op(B) on the matrix cores.
 // partial[chunk][o*mn + a + b*m] = sum over the cells of `chunk` of
 // A[c, a] * op_o(B)[c, b] (mn = m*n).  NOPS = 1: op(B) = B; NOPS = 4: the
@@ -592,6 +611,10 @@ __global__ __launch_bounds__(kBlock) void k_tn_mfma(
   constexpr int DI = ND - 1;                       // the unit-stride axis
   if (NOPS > 1)
     vec = vec && (ND == 2 ? ny : nz) % 4 == 0 && reinterpret_cast<uintptr_t>(irho) % 32 == 0;
+  // a diagonal tile of A^T op(A): the B values at the lane's cells are its A
+  // values (same column, same conversion to TC)
+  const bool same = sizeof(TB) == 8 && static_cast<const void*>(A) == static_cast<const void*>(B) &&
+                    lda == ldb && tm == tn0;
   for (int c0 = cbeg; c0 < cend; c0 += 16) {
     const int cs = c0 + 4 * kg;
     TC av[4], bv[NACC][4];
@@ -607,52 +630,89 @@ __global__ __launch_bounds__(kBlock) void k_tn_mfma(
     }
     if (NOPS > 1 && vec && c0 + 16 <= cend) {
       // the same stencil values as the per-cell path below, from vector loads:
-      // B/rho at the four cells, at their two outer neighbours on the
-      // unit-stride axis, and at the four cells shifted by +-1 on each other
-      // axis.  Every such load is inside the grid when the cells' coordinates
-      // on the other axes are interior, the only case in which any of the four
-      // cells is.
+      // B at the four cells, at their two outer neighbours on the unit-stride
+      // axis, and at the four cells shifted by +-1 on each other axis.  Every
+      // such load is inside the grid when the cells' coordinates on the other
+      // axes are interior, the only case in which any of the four cells is;
+      // otherwise the offsets collapse onto the lane's own cells (readable)
+      // and the cells' in[] flags select zeros afterwards.  All loads are
+      // unconditional.
       load4(Acol + cs, va, av);
       const int ii = cs / syz, jj = (cs - ii * syz) / nz, kk = cs - (cs / nz) * nz;
       const int crd[3] = {ii, jj, kk}, ext[3] = {nx, ny, nz};
       const int ci = crd[DI];                       // first cell's unit-axis coordinate
-      bool outer = vb[0];
+      // geometry only (the same in every lane of a row, whose 1/rho table
+      // lanes therefore agree on the offsets); in[] adds the column's validity
+      bool outer = true;
 #pragma unroll
-      for (int d = 0; d < DI; ++d) outer = outer && crd[d] >= 1 && crd[d] <= ext[d] - 2;
-      TC yc[4], ym[ND][4], yp[ND][4];
-      bool in[4];
+      for (int d = 0; d < DI; ++d) outer = outer & (crd[d] >= 1) & (crd[d] <= ext[d] - 2);
+      bool ing[4], in[4];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) in[s] = outer && ci + s >= 1 && ci + s <= ext[DI] - 2;
-      // every load below is unconditional: off the interior its offsets
-      // collapse onto the lane's own cells (readable), and the cells' in[]
-      // flags select zeros afterwards
+      for (int s = 0; s < 4; ++s) {
+        ing[s] = outer & (ci + s >= 1) & (ci + s <= ext[DI] - 2);
+        in[s] = ing[s] & vb[0];
+      }
+      const int ilo = ing[0] ? cs - 1 : cs, ihi = ing[3] ? cs + 4 : cs + 3;
+      int off[DI > 0 ? DI : 1];
+#pragma unroll
+      for (int d = 0; d < DI; ++d) off[d] = outer ? stride[d] : 0;
+      // 1/rho of the row's cells (a row of 16 lanes shares its four cells):
+      // value j of the table [cells 0..3, lo, hi, then per other axis the
+      // four cells shifted by -1 and by +1] is loaded by lane j % 16 of the
+      // row, in load j / 16, and broadcast across the row
+      constexpr int NRV = 6 + 8 * DI, NRL = (NRV + 15) / 16;
+      double rl[NRL];
+#pragma unroll
+      for (int k = 0; k < NRL; ++k) {
+        const int j = col + 16 * k;
+        int idx = cs;
+        if (j < 4) idx = cs + j;
+        else if (j == 4) idx = ilo;
+        else if (j == 5) idx = ihi;
+        else if (j < NRV) {
+          const int t = j - 6, d = t >> 3, w = t & 7;
+          idx = (w < 4 ? cs - off[d] : cs + off[d]) + (w & 3);
+        }
+        rl[k] = irho[idx];
+      }
+      auto rv = [&](int j) { return (TC)row_bcast(rl[j >> 4], j & 15); };
+      // every B load is issued before any value is used
       const TB* b = Bcol[0];
-      TC bq[4], rq[4];
-      load4(b + cs, true, bq);
-      load4(irho + cs, true, rq);
+      const TB blo = b[ilo], bhi = b[ihi];
+      TC bm[DI > 0 ? DI : 1][4], bp[DI > 0 ? DI : 1][4];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) yc[s] = bq[s] * rq[s];
-      // unit-stride axis: inner neighbours are the lane's own cells; the
-      // outer two are scalar loads (inside the grid when the cell is interior)
-      const int ilo = in[0] ? cs - 1 : cs, ihi = in[3] ? cs + 4 : cs + 3;
-      const TC lo = (TC)b[ilo] * (TC)irho[ilo];
-      const TC hi = (TC)b[ihi] * (TC)irho[ihi];
+      for (int d = 0; d < DI; ++d) {
+        load4(b + cs - off[d], true, bm[d]);
+        load4(b + cs + off[d], true, bp[d]);
+      }
+      // centre values: the A values themselves when A and B are one matrix
+      // and this is a diagonal tile
+      TC bq[4];
+      if (same) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) bq[s] = av[s];
+      } else {
+        load4(b + cs, true, bq);
+      }
+      TC yc[4], ym[ND][4], yp[ND][4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) yc[s] = bq[s] * rv(s);
+      // unit-stride axis: inner neighbours are the lane's own cells, the
+      // outer two the scalar loads
+      const TC lo = (TC)blo * rv(4);
+      const TC hi = (TC)bhi * rv(5);
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         ym[DI][s] = s == 0 ? lo : yc[s - 1];
         yp[DI][s] = s == 3 ? hi : yc[s + 1];
       }
 #pragma unroll
-      for (int d = 0; d < DI; ++d) {
-        const int off = outer ? stride[d] : 0;
-        TC bm[4], rm[4], bp[4], rp[4];
-        load4(b + cs - off, true, bm);
-        load4(irho + cs - off, true, rm);
-        load4(b + cs + off, true, bp);
-        load4(irho + cs + off, true, rp);
+      for (int d = 0; d < DI; ++d)
 #pragma unroll
-        for (int s = 0; s < 4; ++s) { ym[d][s] = bm[s] * rm[s]; yp[d][s] = bp[s] * rp[s]; }
-      }
+        for (int s = 0; s < 4; ++s) {
+          ym[d][s] = bm[d][s] * rv(6 + 8 * d + s);
+          yp[d][s] = bp[d][s] * rv(10 + 8 * d + s);
+        }
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
 #pragma unroll
