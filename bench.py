@@ -47,12 +47,17 @@ PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 
 # HBM bytes per call of the headline's dominant kernel over the timed window,
 # measured with two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) of the
-# headline command `bench.py --headline-only` with the same window, scaled by
+# mixed form's headline solve (`bench.py --profile-form mixed`, the same
+# steps and window as the timed run), scaled by
 # what the counters read for a 256 MiB device copy (x2 fetch, x1 write).  The
 # per-dispatch figures, the command and the window are in
 # profiles/dlra_step_phantom200_p21_pmc.md; the entry names them, and the
 # bench line repeats them beside the number.
-MEASURED_TRAFFIC = {}
+MEASURED_TRAFFIC = {"gram": {"bytes": 5.89e6, "kernels": "k_tn_mfma<1,...> + k_tn_reduce",
+                             "window": "steps 321..620, mixed form",
+                             "command": "rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE) -- "
+                                        "python3 bench.py --profile-form mixed",
+                             "profile": "profiles/dlra_step_phantom200_p21_pmc.md"}}
 # HBM bytes per launch of the full-rank solver's collided kernel on the
 # phantom (FETCH_SIZE x 2 + WRITE_SIZE x 1, profiles/fcs_collide_phantom200_p21.md)
 MEASURED_TRAFFIC_FULL_PN = {"collide": 2.698e8}
@@ -647,6 +652,9 @@ def main():
                     help="only the full-rank P_N solver measurement")
     ap.add_argument("--stencil-only", action="store_true",
                     help="only the large-grid stencil measurement (+ calibration copy), for PMC runs")
+    ap.add_argument("--profile-form", choices=("fp64", "mixed"), default=None,
+                    help="only this form's headline solve (steps 1..advance+warmup+steps, the "
+                         "window timed), for profiler passes; prints its rate")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the timed FP64 and mixed phantom windows (for profiler runs)")
     args = ap.parse_args()
@@ -681,6 +689,14 @@ def main():
     W = max(0, min(args.warmup, last - A - 1))
     K = max(1, min(args.steps, last - A - W))
     period = 0 if args.no_events else args.event_period
+    if args.profile_form:
+        prec = _lib.PRECISION_MIXED if args.profile_form == "mixed" else _lib.PRECISION_FP64
+        one = run_form(P, prec, A + W, K, 0, sync, MIXED_RANK_TOL)
+        print(json.dumps({"form": args.profile_form, "energy_steps_per_s": one["rate"],
+                          "steps_timed": f"{A + W + 1}..{A + W + K}",
+                          "rank_range": [int(one["ranks"][A + W:].min()),
+                                         int(one["ranks"][A + W:].max())]}), flush=True)
+        return
     # both timed forms at the mixed form's rank floor (FP32 noise must not
     # decide which near-null directions enter the basis)
     f64 = run_form(P, _lib.PRECISION_FP64, A + W, K, period, sync, MIXED_RANK_TOL)
