@@ -156,6 +156,26 @@ int titus_solve(struct titus_ctx* ctx, int32_t last_step);
  * (code/SolverCSD.jl:891, 2011). */
 int titus_solve_batch(struct titus_ctx** ctxs, int32_t n, int32_t nthreads, int32_t* status);
 
+/* The same batch with shared launches: up to `width` solves (<= 0: all n)
+ * advance side by side, one host thread each, in `groups` groups (<= 0: 1)
+ * of about width / groups solves with a stream each; in a group every kernel
+ * of the energy step runs as one launch for all solves that reach it
+ * together (solves of different ranks, ordinate counts and energy grids
+ * included), and the groups' host work overlaps the other groups' device
+ * work.  A finished solve's place goes to the next context.  Each solve's
+ * arithmetic is that of titus_solve on its own (bitwise equal results).
+ * Contexts must be 2D, on one device, with the fixed-rank, BUG or parallel
+ * integrator, the upwind flux and no checkpoints (TITUS_E_ARG otherwise,
+ * before any solve).  status[i] (optional) as titus_solve_batch; stats
+ * (optional, 6 values): flushes (host syncs of the groups), batched
+ * launches, the recorded launches they carried, nanoseconds spent in flushes
+ * and, of those, waiting for the streams, and the sum over flushes of the
+ * longest recorded list (the fewest launches those flushes could take).
+ * No reference counterpart: the reference solves one beam per call
+ * (code/SolverCSD.jl:891, 2011). */
+int titus_solve_batch_lockstep(struct titus_ctx** ctxs, int32_t n, int32_t width, int32_t groups,
+                               int32_t* status, int64_t* stats);
+
 /* The pencil beam of the RK4 integrator's analytic uncollided flux
  * (SolverCSD.jl:1437-1440, 1549-1557):
  *   psi[c, q] = 1e5 exp(-1e4 (Omega1 - w_x)^2) exp(-1e4 (Omega3 - w_z)^2)

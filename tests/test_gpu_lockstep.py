@@ -1,0 +1,123 @@
+"""Lockstep beamlet batches (titus_solve_batch_lockstep) on the GPU.
+
+Every solve of a lockstep batch computes what it computes alone: doses,
+factors and rank decisions equal independent titus_solve runs bit for bit.
+The batches mix beamlets of different ranks, ordinate counts and energy
+grids (solves end at different steps, so the group shrinks and, with a
+width below the batch size, takes the next context), FP64 and mixed forms,
+and the rank-adaptive, fixed-rank and parallel integrators side by side.
+"""
+import numpy as np
+import pytest
+
+import cases
+from oracle import dlra
+from titus_amd import _lib
+from titus_amd.beamlets import beamlet_plan
+from titus_amd.problem import INTEGRATOR_BUG, INTEGRATOR_FIXED, INTEGRATOR_PARALLEL
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def small_plan():
+    return beamlet_plan(n_beamlets=6, n=40, N=7, seed=3, rank_range=(4, 12), e_range=(3.0, 6.0))
+
+
+def solve_alone(P, integrator, precision):
+    ctx = _lib.Context(P, integrator, precision=precision)
+    try:
+        ctx.solve(0)
+        st = ctx.state()
+        done, _ = ctx.progress()
+        st["ranks"] = ctx.history(done)[0]
+        st["decisions"] = ctx.decisions()
+    finally:
+        ctx.close()
+    return st
+
+
+def solve_lockstep(problems, width, precision):
+    ctxs = [_lib.Context(P, integ, precision=precision) for P, integ in problems]
+    try:
+        status, stats = _lib.solve_batch_lockstep(ctxs, width)
+        out = []
+        for c in ctxs:
+            st = c.state()
+            done, _ = c.progress()
+            st["ranks"] = c.history(done)[0]
+            st["decisions"] = c.decisions()
+            out.append(st)
+    finally:
+        for c in ctxs:
+            c.close()
+    return status, stats, out
+
+
+def same(a, b):
+    return (np.array_equal(a["dose"], b["dose"]) and np.array_equal(a["X"], b["X"])
+            and np.array_equal(a["S"], b["S"]) and np.array_equal(a["W"], b["W"])
+            and np.array_equal(a["ranks"], b["ranks"]) and a["decisions"] == b["decisions"])
+
+
+@pytest.mark.parametrize("precision", [_lib.PRECISION_FP64, _lib.PRECISION_MIXED])
+@pytest.mark.parametrize("width", [6, 4])
+def test_lockstep_equals_alone_bitwise(small_plan, precision, width):
+    print(f"\n[case] lockstep batch of {len(small_plan)} beamlets 40x40 P_7, width {width}, "
+          f"precision {precision}", flush=True)
+    problems = [(P, INTEGRATOR_BUG) for _, P in small_plan]
+    status, stats, got = solve_lockstep(problems, width, precision)
+    assert np.all(status == 0)
+    steps = sum(P["nE"] - 2 for P, _ in problems)
+    print(f"[lockstep] {steps} beamlet steps, {stats['flushes']} flushes, {stats['launches']} launches "
+          f"carrying {stats['ops']} recorded launches", flush=True)
+    assert stats["launches"] < stats["ops"]
+    for b, (P, integ) in enumerate(problems):
+        ref = solve_alone(P, integ, precision)
+        assert same(got[b], ref), b
+
+
+def test_lockstep_mixed_integrators_bitwise():
+    """Fixed-rank, rank-adaptive and parallel solves in one group."""
+    problems = [(cases.small_fixed(n=24, N=5, r=6), INTEGRATOR_FIXED),
+                (cases.small_bug(n=30, N=7, cap=12, seed=3), INTEGRATOR_BUG),
+                (cases.small_parallel(n=30, N=7, cap=12, seed=3), INTEGRATOR_PARALLEL),
+                (cases.small_fixed(n=32, N=3, r=4), INTEGRATOR_FIXED)]
+    print("\n[case] lockstep group of fixed, BUG and parallel solves", flush=True)
+    status, stats, got = solve_lockstep(problems, 0, _lib.PRECISION_FP64)
+    assert np.all(status == 0)
+    for b, (P, integ) in enumerate(problems):
+        assert same(got[b], solve_alone(P, integ, _lib.PRECISION_FP64)), b
+
+
+def test_lockstep_matches_oracle(small_plan):
+    problems = [(P, INTEGRATOR_BUG) for _, P in small_plan[:3]]
+    _, _, got = solve_lockstep(problems, 0, _lib.PRECISION_FP64)
+    for b, (P, _) in enumerate(problems):
+        e = rel(got[b]["dose"], dlra.solve_bug(P)["dose"])
+        print(f"[lockstep oracle] beamlet {b}: dose rel {e:.3e}", flush=True)
+        assert e <= 1e-10
+
+
+def test_lockstep_refuses_unbatched_contexts(small_plan):
+    """3D grids, checkpointed contexts and the RK4 integrator are refused
+    before any solve runs."""
+    from titus_amd.problem3d import build_problem3d, proton_cube
+    P3 = build_problem3d(proton_cube(n=8, N=3, r=4))
+    P = small_plan[0][1]
+    a = _lib.Context(P, INTEGRATOR_BUG)
+    b = _lib.Context(P3, INTEGRATOR_FIXED)
+    c = _lib.Context(cases.small_fixed(n=24, N=5, r=6), INTEGRATOR_FIXED)
+    try:
+        c.set_checkpoints(4)
+        for bad in (b, c):
+            with pytest.raises(RuntimeError):
+                _lib.solve_batch_lockstep([a, bad], 0)
+            assert a.progress()[0] == 0 and bad.progress()[0] == 0
+    finally:
+        for x in (a, b, c):
+            x.close()

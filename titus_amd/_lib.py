@@ -27,8 +27,8 @@ ERRORS = {
 # every symbol include/titus_hip.h declares
 EXPORTED = ("titus_create", "titus_get_cfl", "titus_set_analytic_beam", "titus_set_rejection",
             "titus_get_rejections",
-            "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
-            "titus_get_state",
+            "titus_step", "titus_solve", "titus_solve_batch", "titus_solve_batch_lockstep",
+            "titus_progress", "titus_get_state",
             "titus_get_history", "titus_set_timing", "titus_kernel_stats", "titus_get_tuning",
             "titus_get_decisions", "titus_set_checkpoints", "titus_adjoint", "titus_adjoint_batch",
             "titus_destroy", "titus_fcs_create", "titus_fcs_step", "titus_fcs_solve",
@@ -114,6 +114,8 @@ def load():
     lib.titus_get_rejections.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     lib.titus_solve.argtypes = [vp, ctypes.c_int32]
     lib.titus_solve_batch.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32, _PI]
+    lib.titus_solve_batch_lockstep.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32,
+                                               ctypes.c_int32, _PI, ctypes.POINTER(ctypes.c_int64)]
     lib.titus_progress.argtypes = [vp, _PI, _PI]
     lib.titus_get_state.argtypes = [vp, _P, _P, _P, _P, _P]
     lib.titus_get_history.argtypes = [vp, _PI, _P, ctypes.c_int32]
@@ -147,8 +149,8 @@ def load():
     lib.titus_abi_version.restype = ctypes.c_int32
     for name in ("titus_create", "titus_get_cfl", "titus_set_analytic_beam", "titus_set_rejection",
                  "titus_get_rejections",
-                 "titus_step", "titus_solve", "titus_solve_batch", "titus_progress",
-                 "titus_get_state", "titus_get_history", "titus_set_timing",
+                 "titus_step", "titus_solve", "titus_solve_batch", "titus_solve_batch_lockstep",
+                 "titus_progress", "titus_get_state", "titus_get_history", "titus_set_timing",
                  "titus_kernel_stats", "titus_get_tuning", "titus_get_decisions",
                  "titus_set_checkpoints",
                  "titus_adjoint", "titus_adjoint_batch", "titus_fcs_create", "titus_fcs_step",
@@ -265,6 +267,27 @@ def solve_batch(contexts, nthreads=0):
     code = lib.titus_solve_batch(arr, n, nthreads, _ip(status))
     check(code)
     return status
+
+
+def solve_batch_lockstep(contexts, width=0, groups=1):
+    """Run every context to the end of its grid with shared launches
+    (titus_solve_batch_lockstep): up to `width` solves (0: all) advance
+    together in `groups` groups, and in a group each kernel runs once for all
+    of its solves.
+
+    Returns (per-context status codes, {"flushes", "launches", "ops", "flush_s",
+    "stream_wait_s", "launches_lower_bound"}); raises on the first failure.
+    """
+    lib = load()
+    n = len(contexts)
+    arr = (ctypes.c_void_p * n)(*[c.h for c in contexts])
+    status = np.zeros(n, dtype=np.int32)
+    stats = (ctypes.c_int64 * 6)()
+    code = lib.titus_solve_batch_lockstep(arr, n, width, groups, _ip(status), stats)
+    check(code)
+    return status, {"flushes": int(stats[0]), "launches": int(stats[1]), "ops": int(stats[2]),
+                    "flush_s": stats[3] * 1e-9, "stream_wait_s": stats[4] * 1e-9,
+                    "launches_lower_bound": int(stats[5])}
 
 
 def adjoint_batch(contexts, dose_bars, nthreads=0):

@@ -14,6 +14,7 @@
 #include <string>
 
 #include "angular.h"
+#include "batch.h"
 
 namespace titus {
 namespace dev {
@@ -35,13 +36,13 @@ __device__ inline double wave_sum(double s) {
 // LDS and global memory (rows summed over the row's nonzeros in order), then
 // G_o[:, j] = Wa^T AW_o[:, j] with one wavefront per output row i: lanes
 // stride the moments, a shuffle tree adds the 64 partial sums.
-__global__ __launch_bounds__(kAngBlock) void k_ang_apply(
+__device__ __forceinline__ void d_ang_apply(Blk blk, 
     int nm, int k, int ka, const int* __restrict__ rowptr, const int* __restrict__ col,
     const double* __restrict__ val, const int* __restrict__ op_nz, const double* __restrict__ W,
     const double* __restrict__ Wa, double* __restrict__ AW, double* __restrict__ G) {
   __shared__ double wl[kMaxAngRows];      // column j of W
   __shared__ double colv[kMaxAngRows];    // column j of A_o W
-  const int j = blockIdx.x, o = blockIdx.y;
+  const int j = blk.x, o = blk.y;
   const int* rp = rowptr + (size_t)o * (nm + 1);
   const int* cl = col + op_nz[o];
   const double* vl = val + op_nz[o];
@@ -94,12 +95,12 @@ __device__ inline double comb_entry(const double* G, int nn, int nd, const doubl
 
 // One workgroup per (column jj, block d): column jj of the combination C_d
 // into LDS, then column jj of S C_d (block 0 also copies column jj of S).
-__global__ __launch_bounds__(kAngBlock) void k_ang_stream_z(
+__device__ __forceinline__ void d_ang_stream_z(Blk blk, 
     int r, int n, int nd, double hx, double hy, double hz, const double* __restrict__ S,
     const double* __restrict__ G, double* __restrict__ Z) {
   __shared__ double cc[96];
   const double h[3] = {hx, hy, hz};
-  const int jj = blockIdx.x, d = blockIdx.y;
+  const int jj = blk.x, d = blk.y;
   const int nn = n * n, rn = r * n;
   for (int l = threadIdx.x; l < n; l += kAngBlock) cc[l] = comb_entry(G, nn, nd, h, d, jj * n + l);
   if (d == 0)
@@ -115,13 +116,13 @@ __global__ __launch_bounds__(kAngBlock) void k_ang_stream_z(
 // L = W S^T - dE sum_{o in order} AW_o (S^T XLX_o^T) (nm x r): one workgroup
 // per column c; v_o = S^T XLX_o[c, :]^T (r values per operator) in LDS, then
 // one thread per moment.
-__global__ __launch_bounds__(kAngBlock) void k_ang_stream_l(
+__device__ __forceinline__ void d_ang_stream_l(Blk blk, 
     int nm, int r, int nops, const int* __restrict__ order, double dE,
     const double* __restrict__ W, const double* __restrict__ S, const double* __restrict__ AW,
     const double* __restrict__ XLX, double* __restrict__ L) {
   __shared__ double v[kMaxAngOps * 96];
   __shared__ double s_c[96];
-  const int c = blockIdx.x;
+  const int c = blk.x;
   for (int e = threadIdx.x; e < nops * r; e += kAngBlock) {
     const int t = e / r, b = e - t * r;
     const int o = order[t];
@@ -147,22 +148,52 @@ __global__ __launch_bounds__(kAngBlock) void k_ang_stream_l(
   }
 }
 
+struct AngApplyBody {
+  struct P {
+    int nm, k, ka; const int* rowptr; const int* col; const double* val; const int* op_nz;
+    const double* W; const double* Wa; double* AW; double* G;
+  };
+  static constexpr int kThreads = kAngBlock;
+  static constexpr bool kBatch = true;
+  __device__ static void run(const P& p, Blk b) {
+    d_ang_apply(b, p.nm, p.k, p.ka, p.rowptr, p.col, p.val, p.op_nz, p.W, p.Wa, p.AW, p.G);
+  }
+};
+struct AngStreamZBody {
+  struct P { int r, n, nd; double hx, hy, hz; const double* S; const double* G; double* Z; };
+  static constexpr int kThreads = kAngBlock;
+  static constexpr bool kBatch = true;
+  __device__ static void run(const P& p, Blk b) {
+    d_ang_stream_z(b, p.r, p.n, p.nd, p.hx, p.hy, p.hz, p.S, p.G, p.Z);
+  }
+};
+struct AngStreamLBody {
+  struct P {
+    int nm, r, nops; const int* order; double dE; const double* W; const double* S;
+    const double* AW; const double* XLX; double* L;
+  };
+  static constexpr int kThreads = kAngBlock;
+  static constexpr bool kBatch = true;
+  __device__ static void run(const P& p, Blk b) {
+    d_ang_stream_l(b, p.nm, p.r, p.nops, p.order, p.dE, p.W, p.S, p.AW, p.XLX, p.L);
+  }
+};
+
 }  // namespace
 
 void launch_ang_apply(hipStream_t st, const AngOps& A, const double* W, int k, const double* Wa,
                       int ka, double* AW, double* G) {
   if (A.nm > kMaxAngRows) throw std::runtime_error("angular: more moments than the kernel holds");
   if (k < 1) return;
-  hipLaunchKernelGGL(k_ang_apply, dim3(k, A.nops), dim3(kAngBlock), 0, st, A.nm, k, ka, A.rowptr,
-                     A.col, A.val, A.op_nz, W, Wa, AW, G);
+  launch_body<AngApplyBody>(st, dim3(k, A.nops), 0,
+                            {A.nm, k, ka, A.rowptr, A.col, A.val, A.op_nz, W, Wa, AW, G});
   check("ang_apply");
 }
 
 void launch_ang_stream_z(hipStream_t st, int r, int n, int nd, double hx, double hy, double hz,
                          const double* S, const double* G, double* Z) {
   if (n > 96 || r > 96) throw std::runtime_error("angular: Z blocks beyond 96 columns");
-  hipLaunchKernelGGL(k_ang_stream_z, dim3(n, 1 + 2 * nd), dim3(kAngBlock), 0, st, r, n, nd, hx, hy,
-                     hz, S, G, Z);
+  launch_body<AngStreamZBody>(st, dim3(n, 1 + 2 * nd), 0, {r, n, nd, hx, hy, hz, S, G, Z});
   check("ang_stream_z");
 }
 
@@ -170,8 +201,7 @@ void launch_ang_stream_l(hipStream_t st, int nm, int r, int nops, const int* ord
                          const double* W, const double* S, const double* AW, const double* XLX,
                          double* L) {
   if (r > 96 || nops > kMaxAngOps) throw std::runtime_error("angular: L-step beyond its limits");
-  hipLaunchKernelGGL(k_ang_stream_l, dim3(r), dim3(kAngBlock), 0, st, nm, r, nops, order, dE, W, S,
-                     AW, XLX, L);
+  launch_body<AngStreamLBody>(st, dim3(r), 0, {nm, r, nops, order, dE, W, S, AW, XLX, L});
   check("ang_stream_l");
 }
 

@@ -41,6 +41,7 @@
 
 #include "../../include/titus_hip.h"
 #include "angular.h"
+#include "batch.h"
 #include "kernels.h"
 #include "smallla.h"
 
@@ -196,6 +197,10 @@ struct StepTape {
 
 struct AdjointStore;
 struct titus_ctx;
+struct LockstepGroup;
+// solver.cpp: the calling context's sync point in its lockstep group (flushes
+// the recorded launches of every context of the group once all have arrived)
+void lockstep_sync(titus_ctx& x);
 // adjoint.cpp: checkpoint of the state before step s (when enabled)
 void adjoint_checkpoint(titus_ctx& x, int s);
 
@@ -238,6 +243,14 @@ struct titus_ctx {
   bool sweep_tuned = false;
   double *X = nullptr, *Xn = nullptr, *K = nullptr, *Q1 = nullptr, *Q2 = nullptr;
   double* Kc = nullptr;                     // copy of K before its projection (parallel BUG)
+  // lockstep batch (titus_solve_batch_lockstep): the group this context
+  // advances with, or null when it runs alone.  In a group the context's
+  // launches are recorded in `rec` and issued by the group at the sync
+  // points; the pinned mirrors are read and written by copy kernels through
+  // their device addresses.
+  LockstepGroup* group = nullptr;
+  dev::Recorder rec;
+  double *h_arena_dev = nullptr, *h_upl_dev = nullptr;
   double *partial = nullptr, *arena = nullptr, *upl = nullptr;
   int *bcells = nullptr, *cand_d = nullptr;
   unsigned char *bmask = nullptr, *restore = nullptr;
@@ -318,6 +331,12 @@ struct titus_ctx {
   // one transfer of what the arena holds beyond the prefix already copied,
   // then wait for the stream
   void fetch() {
+    if (group) {
+      copy_arena_out();
+      lockstep_sync(*this);
+      upl_top = 0;
+      return;
+    }
     if (arena_top > fetched_top)
       HIPCHK(hipMemcpyAsync(h_arena + fetched_top, arena + fetched_top,
                             (arena_top - fetched_top) * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -326,10 +345,28 @@ struct titus_ctx {
     HIPCHK(hipStreamSynchronize(st));
     upl_top = 0;          // every queued upload has been consumed
   }
+  // in a group: the arena beyond the prefix already copied goes to the pinned
+  // mirror by a recorded copy kernel
+  void copy_arena_out() {
+    if (arena_top > fetched_top)
+      dev::launch_copy(st, h_arena_dev + fetched_top, arena + fetched_top,
+                       (arena_top - fetched_top) * sizeof(double));
+    fetched_top = arena_top;
+  }
+  // device-to-device copy on the context's stream (a recorded copy kernel in
+  // a group)
+  void dcopy(void* dst, const void* src, size_t bytes) {
+    if (group) dev::launch_copy(st, dst, src, bytes);
+    else HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
+  }
   // the arena so far goes to the host behind the work queued until now;
   // wait_part() waits for that copy only, so the host can use those results
   // while later work runs on the device
   void fetch_part() {
+    if (group) {
+      copy_arena_out();
+      return;
+    }
     if (arena_top > fetched_top)
       HIPCHK(hipMemcpyAsync(h_arena + fetched_top, arena + fetched_top,
                             (arena_top - fetched_top) * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -337,6 +374,10 @@ struct titus_ctx {
     HIPCHK(hipEventRecord(ev_part, st));
   }
   void wait_part() {
+    if (group) {
+      lockstep_sync(*this);
+      return;
+    }
     HostScope hs(timing, "host:wait");
     HIPCHK(hipEventSynchronize(ev_part));
   }
@@ -346,6 +387,16 @@ struct titus_ctx {
     return A;
   }
   double* upload_raw(const double* src, size_t n) {
+    if (group) {
+      // the ring is reset only at the group's full syncs: it must hold a phase
+      if (upl_top + n > upl_n) throw TitusError(TITUS_E_ARG, "upload ring too small for a batched phase");
+      double* h = h_upl + upl_top;
+      double* d = upl + upl_top;
+      std::memcpy(h, src, n * sizeof(double));
+      dev::launch_copy(st, d, h_upl_dev + upl_top, n * sizeof(double));
+      upl_top += (n + 31) & ~size_t(31);
+      return d;
+    }
     if (upl_top + n > upl_n) {
       HIPCHK(hipStreamSynchronize(st));
       upl_top = 0;
@@ -394,7 +445,7 @@ struct titus_ctx {
   // through the upload ring
   void gemm(const double* A, int k, const HMat& T, double alpha, double beta, double* out,
             const unsigned char* mask) {
-    if (T.a.size() > (size_t)dev::kArgMat) {
+    if (group || T.a.size() > (size_t)dev::kArgMat) {
       gemm_dev(A, k, upload(T), T.n, alpha, beta, out, mask);
       return;
     }
@@ -516,7 +567,7 @@ struct titus_ctx {
   }
   void stream_k(int r, const std::vector<HMat>& WAW) {
     const HMat Z = stream_Z(S, WAW);
-    if (Z.a.size() <= (size_t)dev::kArgMat) {
+    if (!group && Z.a.size() <= (size_t)dev::kArgMat) {
       timed("stream_K", 8.0 * C * (2 * r + 1), 2.0 * (2 * nd + 2) * C * r * r,
             [&] { dev::launch_stream_k_args(st, g, X, C, r, Z.a.data(), r, dE, K, C, full32); });
       return;
@@ -665,7 +716,7 @@ struct titus_ctx {
       T = titus::matmul(R3inv, titus::cols(P, 0, p));                    // k x p
       if (ot) {
         ot->sv = sv; ot->P = P; ot->V = Zr; ot->R3inv = R3inv;
-        HIPCHK(hipMemcpyAsync(ot->Q2, Q2, sizeof(double) * (size_t)C * k, hipMemcpyDeviceToDevice, st));
+        dcopy(ot->Q2, Q2, sizeof(double) * (size_t)C * k);
       }
       HMat Uc = titus::matmul(view(j.rows_q, j.nc, k), T);               // nc x p
       Bc = j.Xa ? titus::hcat(view(j.rows_x, j.nc, q), Uc) : Uc;
@@ -748,7 +799,7 @@ struct titus_ctx {
                      u0_groups);
     });
     if (tape)
-      HIPCHK(hipMemcpyAsync(tape->psi_a, psi, sizeof(double) * (size_t)C * nq, hipMemcpyDeviceToDevice, st));
+      dcopy(tape->psi_a, psi, sizeof(double) * (size_t)C * nq);
     dose_half(stopping[s - 1]);
     const double denom = 1.0 + dE * sg[0];
     // psi read and written once, density, restore flags and moment partials
@@ -757,7 +808,7 @@ struct titus_ctx {
     });
     std::swap(psi, psi2);
     if (tape)
-      HIPCHK(hipMemcpyAsync(tape->psi_b, psi, sizeof(double) * (size_t)C * nq, hipMemcpyDeviceToDevice, st));
+      dcopy(tape->psi_b, psi, sizeof(double) * (size_t)C * nq);
     Dvec.resize(nm);
     Dout.resize(nm);
     for (int i = 0; i < nm; ++i) {
@@ -789,7 +840,7 @@ struct titus_ctx {
     HMat T = MtDW;
     for (double& v : T.a) v *= dE;
     const double ps = mixed ? 4.0 : 8.0;
-    if (T.a.size() <= (size_t)dev::kArgMat) {
+    if (!group && T.a.size() <= (size_t)dev::kArgMat) {
       timed("gemm", ps * C * nq + 16.0 * C * T.n, 2.0 * C * nq * T.n, [&] {
         dev::launch_gemm_args(st, C, psi, tpsi, C, nq, T.a.data(), T.n, 1.0, 1.0, K, C, mask,
                               full32);
@@ -821,7 +872,7 @@ struct titus_ctx {
     const size_t cr = sizeof(double) * (size_t)C * rank;
     if (tape) {
       tape->s = s; tape->S = S; tape->W = W;
-      HIPCHK(hipMemcpyAsync(tape->X, X, cr, hipMemcpyDeviceToDevice, st));
+      dcopy(tape->X, X, cr);
     }
     begin_step(s, Dvec, Dout);
     if (tape) { tape->Dvec = Dvec; tape->Dout = Dout; tape->sig0 = sig(s)[0]; }
@@ -873,7 +924,7 @@ struct titus_ctx {
       S = titus::matmul(titus::matmul(MUp, S), NUp, false, true);
       if (tape) {
         tape->S1 = S; tape->XLX1 = XLX; tape->WAW1 = WAW1;
-        HIPCHK(hipMemcpyAsync(tape->X1, X, cr, hipMemcpyDeviceToDevice, st));
+        dcopy(tape->X1, X, cr);
       }
       S = stream_S(S, XLX, WAW1);
     } else {
@@ -924,7 +975,7 @@ struct titus_ctx {
     if (tape) {
       tape->MtDW = MtDW; tape->MUp3 = MUp; tape->W3 = W; tape->NUp3 = NUp; tape->B3 = B3;
       tape->A3 = XtP; tape->S5 = S;
-      HIPCHK(hipMemcpyAsync(tape->X3, X, cr, hipMemcpyDeviceToDevice, st));
+      dcopy(tape->X3, X, cr);
     }
   }
 
@@ -976,7 +1027,7 @@ struct titus_ctx {
       });
     };
     auto copy = [&](double* dst, const double* src) {
-      HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
+      dcopy(dst, src, bytes);
     };
     const double h = 0.5 * dE;
     f(K);
@@ -1109,11 +1160,10 @@ struct titus_ctx {
   // moment partials; host S, W, rank).
   void save_start() {
     const size_t psz = mixed ? sizeof(float) : sizeof(double);
-    HIPCHK(hipMemcpyAsync(X_start, X, sizeof(double) * (size_t)C * rank, hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipMemcpyAsync(psi_start, psi, psz * (size_t)C * nq, hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipMemcpyAsync(dose_start, dose, sizeof(double) * C, hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipMemcpyAsync(u0_start, u0p, sizeof(double) * (size_t)u0_groups * C,
-                          hipMemcpyDeviceToDevice, st));
+    dcopy(X_start, X, sizeof(double) * (size_t)C * rank);
+    dcopy(psi_start, psi, psz * (size_t)C * nq);
+    dcopy(dose_start, dose, sizeof(double) * C);
+    dcopy(u0_start, u0p, sizeof(double) * (size_t)u0_groups * C);
     S_start = S;
     W_start = W;
     r_start = rank;
@@ -1122,10 +1172,9 @@ struct titus_ctx {
   // with zeros to the new rank (u = X S W^T is unchanged)
   void restart_state(int newrank, size_t neta) {
     const size_t psz = mixed ? sizeof(float) : sizeof(double);
-    HIPCHK(hipMemcpyAsync(psi, psi_start, psz * (size_t)C * nq, hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipMemcpyAsync(dose, dose_start, sizeof(double) * C, hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipMemcpyAsync(u0p, u0_start, sizeof(double) * (size_t)u0_groups * C,
-                          hipMemcpyDeviceToDevice, st));
+    dcopy(psi, psi_start, psz * (size_t)C * nq);
+    dcopy(dose, dose_start, sizeof(double) * C);
+    dcopy(u0p, u0_start, sizeof(double) * (size_t)u0_groups * C);
     S = HMat(newrank, newrank);
     for (int j = 0; j < r_start; ++j)
       for (int i = 0; i < r_start; ++i) S(i, j) = S_start(i, j);
@@ -1138,7 +1187,7 @@ struct titus_ctx {
   // streaming half rejected: X1 = [X_start | Xt] (device Xn), W1 = [W_start | Wt]
   void reject_streaming(const HMat& W1, size_t neta) {
     const int n = W1.n;
-    HIPCHK(hipMemcpyAsync(X, Xn, sizeof(double) * (size_t)C * n, hipMemcpyDeviceToDevice, st));
+    dcopy(X, Xn, sizeof(double) * (size_t)C * n);
     W = W1;
     restart_state(n, neta);
   }
@@ -1146,14 +1195,13 @@ struct titus_ctx {
   // W1 columns rc..) made orthogonal to the step's starting bases
   void reject_scattering(int rc, const HMat& W1, size_t neta) {
     const int r0 = r_start;
-    HIPCHK(hipMemcpyAsync(K, Xn + (size_t)rc * C, sizeof(double) * (size_t)C * rc,
-                          hipMemcpyDeviceToDevice, st));
+    dcopy(K, Xn + (size_t)rc * C, sizeof(double) * (size_t)C * rc);
     areset();
     RangeJob j = range_launch(K, rc, X_start, r0);
     const HMat Wt = titus::cols(W1, rc, W1.n);
     HMat Wt2 = range_w(Wt, rc, &W_start, titus::frob(Wt));
     fetch();
-    HIPCHK(hipMemcpyAsync(X, X_start, sizeof(double) * (size_t)C * r0, hipMemcpyDeviceToDevice, st));
+    dcopy(X, X_start, sizeof(double) * (size_t)C * r0);
     range_finish(j, rc, X + (size_t)r0 * C);
     W = titus::hcat(W_start, Wt2);
     restart_state(r0 + rc, neta);
@@ -1162,7 +1210,7 @@ struct titus_ctx {
     return titus::hcat(W, range_w(L, rank, &W, titus::frob(L)));
   }
   void copy_x_to_aug(int r) {
-    HIPCHK(hipMemcpyAsync(Xn, X, (size_t)C * r * sizeof(double), hipMemcpyDeviceToDevice, st));
+    dcopy(Xn, X, (size_t)C * r * sizeof(double));
   }
 
   void step_bug(int s) {
@@ -1322,7 +1370,7 @@ struct titus_ctx {
       const std::vector<HMat> WAW = galerkin_from(W, aw);
       stream_k(r, WAW);                                   // K (:1843-1846)
       copy_x_to_aug(r);
-      HIPCHK(hipMemcpyAsync(Kc, K, sizeof(double) * (size_t)C * r, hipMemcpyDeviceToDevice, st));
+      dcopy(Kc, K, sizeof(double) * (size_t)C * r);
       RangeJob j = range_launch(K, r, X, r);              // K is left projected off X
       // L- and S-steps from the old factors on the host while the device runs
       // the K range job
@@ -1348,7 +1396,7 @@ struct titus_ctx {
     const HMat MtDW = mtdw(Dvec, W);
     scatter_K(MtDW);
     copy_x_to_aug(r);
-    HIPCHK(hipMemcpyAsync(Kc, K, sizeof(double) * (size_t)C * r, hipMemcpyDeviceToDevice, st));
+    dcopy(Kc, K, sizeof(double) * (size_t)C * r);
     RangeJob j = range_launch(K, r, X, r);
     // L- and S-steps on the host while the device runs the K range job
     wait_part();
@@ -1374,7 +1422,7 @@ struct titus_ctx {
   void step(int s) {
     if (s != steps_done + 1) throw TitusError(TITUS_E_STATE, "steps must run in order");
     if (s >= ne) throw TitusError(TITUS_E_STATE, "step beyond the energy grid");
-    timing.enabled = timing.period > 0 && s % timing.period == 0;
+    timing.enabled = !group && timing.period > 0 && s % timing.period == 0;
     HostScope hs(timing, "host:step");
     if (adj) adjoint_checkpoint(*this, s);
     if (integrator == TITUS_INTEGRATOR_FIXED) step_fixed(s);

@@ -1,10 +1,12 @@
 // Device kernels of the DLRA energy step for gfx950 (see kernels.h).
 #include "kernels.h"
+#include "batch.h"
 
 #include <algorithm>
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace titus {
@@ -58,10 +60,10 @@ struct ArgSrc {
 // psi at the boundary cells for every ordinate, and the l = 0 moment of those
 // cells: u0p[0][cell] = sum_q psi[cell, q] m0[q] (the other partials are 0).
 template <class TP>
-__global__ void k_bc(TP* __restrict__ psi, int C, int nq, const int* __restrict__ bcells, int nb,
+__device__ __forceinline__ void d_bc(Blk blk, TP* __restrict__ psi, int C, int nq, const int* __restrict__ bcells, int nb,
                      const double* __restrict__ base, double scale,
                      const double* __restrict__ m0, double* __restrict__ u0p, int G) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blk.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   const int cell = bcells[b];
   double u0 = 0.0;
@@ -75,10 +77,10 @@ __global__ void k_bc(TP* __restrict__ psi, int C, int nq, const int* __restrict_
 }
 
 // dose[c] += coef * (X[c,:] v + sum_g u0p[g][c]) / rho[c]
-__global__ void k_dose(double* __restrict__ dose, const double* __restrict__ X, int ldx, int r,
+__device__ __forceinline__ void d_dose(Blk blk, double* __restrict__ dose, const double* __restrict__ X, int ldx, int r,
                        const double* __restrict__ v, const double* __restrict__ u0p, int G, int C,
                        const double* __restrict__ rho, double coef) {
-  const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int c = xcd_block(blk.x, blk.nx) * blockDim.x + threadIdx.x;
   if (c >= C) return;
   double acc = 0.0;
   for (int a = 0; a < r; ++a) acc += X[(size_t)a * ldx + c] * v[a];
@@ -135,13 +137,13 @@ __global__ void k_analytic_psi(TP* __restrict__ psi, int C, int ny, int nq,
 // The new psi's l = 0 moment accumulates per cell and is written to
 // u0p[gq][c] at the end (the dose needs it after the sweep).
 template <class T, int H, bool NT>
-__global__ __launch_bounds__(kBlock) void k_sweep(
+__device__ __forceinline__ void d_sweep(Blk blk, 
     const T* __restrict__ pin, T* __restrict__ pout, int nx, int ny, int nq, int qper,
     const double* __restrict__ omega, T dx, T dy, const double* __restrict__ rho, T dE, T denom,
     const unsigned char* __restrict__ restore, const double* __restrict__ m0,
     double* __restrict__ u0p, int ntj, int nsb) {
   const int C = nx * ny;
-  const int L = xcd_block(blockIdx.x, gridDim.x);
+  const int L = xcd_block(blk.x, blk.nx);
   const int jt = L % ntj, sb = (L / ntj) % nsb, gq = L / (ntj * nsb);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i0 = (sb * (kBlock / 64) + wave) * H;
@@ -566,19 +568,21 @@ __device__ inline void load4(const TS* p, bool ok, TC* v) {
 }
 
 template <int NOPS, int NBT, class TB, class TC>
-__global__ __launch_bounds__(kBlock) void k_tn_mfma(
+__device__ __forceinline__ void d_tn_mfma(Blk blk, 
     const double* __restrict__ A, int lda, int m, const TB* __restrict__ B, int ldb, int n,
     int C, int nx, int ny, int nz, double idx_, double idy_, double idz_,
-    const double* __restrict__ irho, int cpw, int ngroups, int groups_n,
+    const double* __restrict__ irho, int cpw, int ngroups, int groups_n, int nbt,
     double* __restrict__ partial) {
   using MF = Mfma<TC>;
   using acc_t = typename MF::acc_t;
   constexpr int NACC = NOPS * NBT;
   __shared__ acc_t red[3][NACC][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int L = xcd_block(blockIdx.x, gridDim.x);
+  const int L = xcd_block(blk.x, blk.nx);
   const int grp = L % ngroups, chunk = L / ngroups;
-  const int tm = grp / groups_n, tn0 = (grp % groups_n) * NBT;
+  // nbt <= NBT B tiles of this launch (a batched launch runs the widest
+  // template for every item and each item's own tile count)
+  const int tm = grp / groups_n, tn0 = (grp % groups_n) * nbt;
   const int col = lane & 15, kg = lane >> 4;
   const int ca = tm * 16 + col;
   const bool va = ca < m;
@@ -588,7 +592,7 @@ __global__ __launch_bounds__(kBlock) void k_tn_mfma(
 #pragma unroll
   for (int t = 0; t < NBT; ++t) {
     const int cb = (tn0 + t) * 16 + col;
-    vb[t] = cb < n;
+    vb[t] = t < nbt && cb < n;
     Bcol[t] = B + (size_t)(vb[t] ? cb : 0) * ldb;
   }
   const int cbeg = (chunk * 4 + wave) * cpw;
@@ -797,10 +801,10 @@ __global__ __launch_bounds__(kBlock) void k_tn_mfma(
 
 // One wavefront per output: lane l sums chunks l, l+64, ...; then a fixed
 // shuffle tree (deterministic order).
-__global__ void k_tn_reduce(const double* __restrict__ partial, int nchunks, int mn,
+__device__ __forceinline__ void d_tn_reduce(Blk blk, const double* __restrict__ partial, int nchunks, int mn,
                             double* __restrict__ out) {
   const int lane = threadIdx.x & 63;
-  const int p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int p = blk.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (p >= mn) return;
   double s = 0.0;
   for (int k = lane; k < nchunks; k += 64) s += partial[(size_t)k * mn + p];
@@ -827,9 +831,9 @@ __global__ void k_tn_reduce(const double* __restrict__ partial, int nchunks, int
 // k_stream_k.  Each wavefront walks `tpw` consecutive 16-cell tiles, two at
 // a time.
 template <class TC, int ND, class Src>
-__global__ __launch_bounds__(kBlock) void k_stream_k_mfma(
+__device__ __forceinline__ void d_stream_k_mfma(Blk blk, 
     const double* __restrict__ X, int ldx, int r, int rp, int nx, int ny, int nz,
-    const double* __restrict__ rho, const Src Z, int n, double dE, double* __restrict__ K,
+    const double* __restrict__ rho, const Src& Z, int n, double dE, double* __restrict__ K,
     int ldk, int tpw) {
   using MF = Mfma<TC>;
   using acc_t = typename MF::acc_t;
@@ -837,7 +841,7 @@ __global__ __launch_bounds__(kBlock) void k_stream_k_mfma(
   extern __shared__ __align__(16) unsigned char lds_raw[];
   TC* Zs = reinterpret_cast<TC*>(lds_raw);
   const int C = nx * ny * nz;
-  const int b0 = blockIdx.y * 16;
+  const int b0 = blk.y * 16;
   const int nbv = min(16, n - b0);
   for (int e = threadIdx.x; e < NZB * rp * 16; e += blockDim.x) {
     const int b = e & 15, a = (e >> 4) % rp, d = (e >> 4) / rp;
@@ -857,7 +861,7 @@ __global__ __launch_bounds__(kBlock) void k_stream_k_mfma(
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int cl = lane & 15, kg = lane >> 4;
-  const int L = xcd_block(blockIdx.x, gridDim.x);
+  const int L = xcd_block(blk.x, blk.nx);
   const int syz = ny * nz;
   const int stride[3] = {syz, nz, 1}, ext[3] = {nx, ny, nz};
   const TC tdE = (TC)dE;
@@ -961,7 +965,7 @@ __global__ __launch_bounds__(kBlock) void k_stream_k_mfma(
 // inverse of R.  Shift schedule: (s = 0 first when allow_plain), then
 // s = 1e-14 tr(G) growing tenfold per failed attempt, at most 12 attempts.
 // status[0] = number of attempts used, or -1 when all failed.
-__global__ void k_chol_inv(const double* __restrict__ G, int k, int allow_plain,
+__device__ __forceinline__ void d_chol_inv(Blk blk, const double* __restrict__ G, int k, int allow_plain,
                            double* __restrict__ R, double* __restrict__ Rinv,
                            double* __restrict__ status) {
   // A (k x k, column-major, LDS): the factor in the upper triangle; its
@@ -1035,13 +1039,13 @@ __global__ void k_chol_inv(const double* __restrict__ G, int k, int allow_plain,
 // are staged in LDS as Ts[a*NB + k] (broadcast reads).  Sums run over a in
 // ascending order.
 template <class TA, class TC, int NB, class Src>
-__global__ __launch_bounds__(kBlock) void k_gemm(int C, const TA* __restrict__ A, int lda, int k,
-                                                 const Src T, int n, double alpha,
+__device__ __forceinline__ void d_gemm(Blk blk, int C, const TA* __restrict__ A, int lda, int k,
+                                                 const Src& T, int n, double alpha,
                                                  double beta, double* __restrict__ out, int ldo,
                                                  const unsigned char* __restrict__ mask) {
   __shared__ TC Ts[kRowsKT * NB];
-  const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const int b0 = blockIdx.y * NB;
+  const int c = xcd_block(blk.x, blk.nx) * blockDim.x + threadIdx.x;
+  const int b0 = blk.y * NB;
   const int nbv = min(NB, n - b0);
   TC acc[NB];
 #pragma unroll
@@ -1077,44 +1081,163 @@ __global__ __launch_bounds__(kBlock) void k_gemm(int C, const TA* __restrict__ A
   }
 }
 
-__global__ void k_gather_rows(const double* __restrict__ A, int lda, int ncols,
+__device__ __forceinline__ void d_gather_rows(Blk blk, const double* __restrict__ A, int lda, int ncols,
                               const int* __restrict__ rows, int nrows, double* __restrict__ out) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t t = (size_t)blk.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)nrows * ncols) return;
   const int r = (int)(t % nrows), k = (int)(t / nrows);
   out[t] = A[(size_t)k * lda + rows[r]];
 }
 
 // packed = [rows (n), cols (n), vals (n)], indices stored as doubles
-__global__ void k_scatter_add(double* __restrict__ A, int lda, const double* __restrict__ packed,
+__device__ __forceinline__ void d_scatter_add(Blk blk, double* __restrict__ A, int lda, const double* __restrict__ packed,
                               int n) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blk.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int row = (int)packed[t], col = (int)packed[n + t];
   A[(size_t)col * lda + row] += packed[2 * n + t];
 }
 
+// ---- launch bodies (batch.h): parameter structs of the kernels above ----------
+template <class TP>
+struct BcBody {
+  struct P {
+    TP* psi; int C, nq; const int* bcells; int nb; const double* base; double scale;
+    const double* m0; double* u0p; int G;
+  };
+  static constexpr int kThreads = kBlock;
+  static constexpr bool kBatch = true;
+  __device__ static void run(const P& p, Blk b) {
+    d_bc<TP>(b, p.psi, p.C, p.nq, p.bcells, p.nb, p.base, p.scale, p.m0, p.u0p, p.G);
+  }
+};
+struct DoseBody {
+  struct P {
+    double* dose; const double* X; int ldx, r; const double* v; const double* u0p; int G, C;
+    const double* rho; double coef;
+  };
+  static constexpr int kThreads = kBlock;
+  static constexpr bool kBatch = true;
+  __device__ static void run(const P& p, Blk b) {
+    d_dose(b, p.dose, p.X, p.ldx, p.r, p.v, p.u0p, p.G, p.C, p.rho, p.coef);
+  }
+};
+template <class T, int H, bool NT>
+struct SweepBody {
+  struct P {
+    const T* pin; T* pout; int nx, ny, nq, qper; const double* omega; T dx, dy;
+    const double* rho; T dE, denom; const unsigned char* restore; const double* m0;
+    double* u0p; int ntj, nsb;
+  };
+  static constexpr int kThreads = kBlock;
+  static constexpr bool kBatch = true;
+  __device__ static void run(const P& p, Blk b) {
+    d_sweep<T, H, NT>(b, p.pin, p.pout, p.nx, p.ny, p.nq, p.qper, p.omega, p.dx, p.dy, p.rho,
+                      p.dE, p.denom, p.restore, p.m0, p.u0p, p.ntj, p.nsb);
+  }
+};
+template <int NOPS, int NBT, class TB, class TC>
+struct TnBody {
+  struct P {
+    const double* A; int lda, m; const TB* B; int ldb, n, C, nx, ny, nz; double idx, idy, idz;
+    const double* irho; int cpw, ngroups, groups_n, nbt; double* partial;
+  };
+  static constexpr int kThreads = kBlock;
+  static constexpr bool kBatch = true;
+  __device__ static void run(const P& p, Blk b) {
+    d_tn_mfma<NOPS, NBT, TB, TC>(b, p.A, p.lda, p.m, p.B, p.ldb, p.n, p.C, p.nx, p.ny, p.nz,
+                                 p.idx, p.idy, p.idz, p.irho, p.cpw, p.ngroups, p.groups_n,
+                                 p.nbt, p.partial);
+  }
+};
+struct TnReduceBody {
+  struct P { const double* partial; int nchunks, mn; double* out; };
+  static constexpr int kThreads = kBlock;
+  static constexpr bool kBatch = true;
+  __device__ static void run(const P& p, Blk b) { d_tn_reduce(b, p.partial, p.nchunks, p.mn, p.out); }
+};
+template <class TC, int ND, class Src>
+struct StreamKMfmaBody {
+  struct P {
+    const double* X; int ldx, r, rp, nx, ny, nz; const double* rho; Src Z; int n; double dE;
+    double* K; int ldk, tpw;
+  };
+  static constexpr int kThreads = kBlock;
+  static constexpr bool kBatch = !std::is_same<Src, ArgSrc>::value;
+  __device__ static void run(const P& p, Blk b) {
+    d_stream_k_mfma<TC, ND, Src>(b, p.X, p.ldx, p.r, p.rp, p.nx, p.ny, p.nz, p.rho, p.Z, p.n,
+                                 p.dE, p.K, p.ldk, p.tpw);
+  }
+};
+struct CholBody {
+  struct P { const double* G; int k, allow_plain; double* R; double* Rinv; double* status; };
+  static constexpr int kThreads = kBlock;
+  static constexpr bool kBatch = true;
+  __device__ static void run(const P& p, Blk b) {
+    d_chol_inv(b, p.G, p.k, p.allow_plain, p.R, p.Rinv, p.status);
+  }
+};
+template <class TA, class TC, int NB, class Src>
+struct GemmBody {
+  struct P {
+    int C; const TA* A; int lda, k; Src T; int n; double alpha, beta; double* out; int ldo;
+    const unsigned char* mask;
+  };
+  static constexpr int kThreads = kBlock;
+  static constexpr bool kBatch = !std::is_same<Src, ArgSrc>::value;
+  __device__ static void run(const P& p, Blk b) {
+    d_gemm<TA, TC, NB, Src>(b, p.C, p.A, p.lda, p.k, p.T, p.n, p.alpha, p.beta, p.out, p.ldo,
+                            p.mask);
+  }
+};
+struct GatherBody {
+  struct P { const double* A; int lda, ncols; const int* rows; int nrows; double* out; };
+  static constexpr int kThreads = kBlock;
+  static constexpr bool kBatch = true;
+  __device__ static void run(const P& p, Blk b) {
+    d_gather_rows(b, p.A, p.lda, p.ncols, p.rows, p.nrows, p.out);
+  }
+};
+struct ScatterAddBody {
+  struct P { double* A; int lda; const double* packed; int n; };
+  static constexpr int kThreads = kBlock;
+  static constexpr bool kBatch = true;
+  __device__ static void run(const P& p, Blk b) { d_scatter_add(b, p.A, p.lda, p.packed, p.n); }
+};
+// element-wise copy of n elements of E, grid-stride
+template <class E>
+struct CopyBody {
+  struct P { const E* src; E* dst; size_t n; };
+  static constexpr int kThreads = kBlock;
+  static constexpr bool kBatch = true;
+  __device__ static void run(const P& p, Blk b) {
+    const size_t stride = (size_t)b.nx * kBlock;
+    for (size_t i = (size_t)b.x * kBlock + threadIdx.x; i < p.n; i += stride) p.dst[i] = p.src[i];
+  }
+};
+
 template <int NOPS, int NBT, class TB, class TC>
 void tn_mfma_launch(hipStream_t st, int blocks, const void* A, int lda, int m, const void* B,
                     int ldb, int n, const Grid& g, int cpw, int ngroups, int groups_n,
-                    double* partial) {
-  hipLaunchKernelGGL((k_tn_mfma<NOPS, NBT, TB, TC>), dim3(blocks), dim3(kBlock), 0, st,
-                     static_cast<const double*>(A), lda, m, static_cast<const TB*>(B), ldb, n, g.C,
-                     g.nx, g.ny, g.nz, 1.0 / g.dx, 1.0 / g.dy, 1.0 / g.dz, g.irho, cpw, ngroups,
-                     groups_n, partial);
+                    int nbt, double* partial) {
+  using Body = TnBody<NOPS, NBT, TB, TC>;
+  launch_body<Body>(st, dim3(blocks), 0,
+                    typename Body::P{static_cast<const double*>(A), lda, m, static_cast<const TB*>(B),
+                                     ldb, n, g.C, g.nx, g.ny, g.nz, 1.0 / g.dx, 1.0 / g.dy,
+                                     1.0 / g.dz, g.irho, cpw, ngroups, groups_n, nbt, partial});
 }
 
 template <int NOPS, int NBT>
 void tn_dispatch(hipStream_t st, int blocks, const void* A, DT ta, int lda, int m, const void* B,
                  DT tb, int ldb, int n, const Grid& g, bool fp32, int cpw, int ngroups,
-                 int groups_n, double* partial) {
+                 int groups_n, double* partial, int nbt = NBT) {
   if (ta != DT::F64) throw std::runtime_error("tn: the left block is always FP64");
   if (tb == DT::F64) {
-    if (fp32) tn_mfma_launch<NOPS, NBT, double, float>(st, blocks, A, lda, m, B, ldb, n, g, cpw, ngroups, groups_n, partial);
-    else tn_mfma_launch<NOPS, NBT, double, double>(st, blocks, A, lda, m, B, ldb, n, g, cpw, ngroups, groups_n, partial);
+    if (fp32) tn_mfma_launch<NOPS, NBT, double, float>(st, blocks, A, lda, m, B, ldb, n, g, cpw, ngroups, groups_n, nbt, partial);
+    else tn_mfma_launch<NOPS, NBT, double, double>(st, blocks, A, lda, m, B, ldb, n, g, cpw, ngroups, groups_n, nbt, partial);
   } else {
-    if (fp32) tn_mfma_launch<NOPS, NBT, float, float>(st, blocks, A, lda, m, B, ldb, n, g, cpw, ngroups, groups_n, partial);
-    else tn_mfma_launch<NOPS, NBT, float, double>(st, blocks, A, lda, m, B, ldb, n, g, cpw, ngroups, groups_n, partial);
+    if (fp32) tn_mfma_launch<NOPS, NBT, float, float>(st, blocks, A, lda, m, B, ldb, n, g, cpw, ngroups, groups_n, nbt, partial);
+    else tn_mfma_launch<NOPS, NBT, float, double>(st, blocks, A, lda, m, B, ldb, n, g, cpw, ngroups, groups_n, nbt, partial);
   }
 }
 
@@ -1125,11 +1248,11 @@ void launch_bc(hipStream_t st, void* psi, DT tpsi, int C, int nq, const int* bce
                const double* bc_base, double scale, const double* m0, double* u0p, int G) {
   if (nb == 0) return;
   if (tpsi == DT::F64)
-    hipLaunchKernelGGL(k_bc<double>, dim3(blocks_for(nb)), dim3(kBlock), 0, st,
-                       static_cast<double*>(psi), C, nq, bcells, nb, bc_base, scale, m0, u0p, G);
+    launch_body<BcBody<double>>(st, dim3(blocks_for(nb)), 0,
+                                {static_cast<double*>(psi), C, nq, bcells, nb, bc_base, scale, m0, u0p, G});
   else
-    hipLaunchKernelGGL(k_bc<float>, dim3(blocks_for(nb)), dim3(kBlock), 0, st,
-                       static_cast<float*>(psi), C, nq, bcells, nb, bc_base, scale, m0, u0p, G);
+    launch_body<BcBody<float>>(st, dim3(blocks_for(nb)), 0,
+                               {static_cast<float*>(psi), C, nq, bcells, nb, bc_base, scale, m0, u0p, G});
   check_launch("bc");
 }
 
@@ -1137,6 +1260,7 @@ void launch_analytic_psi(hipStream_t st, const Grid& g, void* psi, DT tpsi, int 
                          const double* omega, const double* xmid, const double* ymid, double x0,
                          double y0, double om1, double om3, double tau, double int_sigma,
                          const double* m0, double* u0p) {
+  no_batch("analytic_psi");
   if (tpsi == DT::F64)
     hipLaunchKernelGGL(k_analytic_psi<double>, dim3(blocks_for(g.C)), dim3(kBlock), 0, st,
                        static_cast<double*>(psi), g.C, g.ny, nq, omega, xmid, ymid, x0, y0, om1,
@@ -1150,8 +1274,7 @@ void launch_analytic_psi(hipStream_t st, const Grid& g, void* psi, DT tpsi, int 
 
 void launch_dose(hipStream_t st, const Grid& g, double* dose, const double* X, int ldx, int r,
                  const double* v, const double* u0p, int G, double coef) {
-  hipLaunchKernelGGL(k_dose, dim3(blocks_for(g.C)), dim3(kBlock), 0, st, dose, X, ldx, r, v, u0p,
-                     G, g.C, g.rho, coef);
+  launch_body<DoseBody>(st, dim3(blocks_for(g.C)), 0, {dose, X, ldx, r, v, u0p, G, g.C, g.rho, coef});
   check_launch("dose");
 }
 
@@ -1182,6 +1305,7 @@ void sweep_launch(hipStream_t st, const Grid& g, const void* pin, void* pout, in
                   const double* omega, double dE, double denom, const unsigned char* restore,
                   const double* m0, double* u0p, const SweepShape& s, bool nt, bool lim) {
   if (g.ndim() == 3) {
+    no_batch("sweep (3D)");
     hipLaunchKernelGGL((k_sweep3<T, H>), dim3((unsigned)(s.ntj * s.njb * s.nsb * s.G)),
                        dim3(kBlock), 0, st, static_cast<const T*>(pin), static_cast<T*>(pout),
                        g.nx, g.ny, g.nz, nq, s.qper, omega, (T)g.dx, (T)g.dy, (T)g.dz, g.rho,
@@ -1189,17 +1313,24 @@ void sweep_launch(hipStream_t st, const Grid& g, const void* pin, void* pout, in
     return;
   }
   if (lim) {
+    no_batch("sweep (limited flux)");
     hipLaunchKernelGGL((k_sweep_lim<T, H>), dim3((unsigned)(s.ntj * s.nsb * s.G)), dim3(kBlock), 0,
                        st, static_cast<const T*>(pin), static_cast<T*>(pout), g.nx, g.ny, nq,
                        s.qper, omega, (T)g.dx, (T)g.dy, g.rho, (T)dE, (T)denom, restore, m0, u0p,
                        s.ntj, s.nsb);
     return;
   }
-  auto kern = nt ? k_sweep<T, H, true> : k_sweep<T, H, false>;
-  hipLaunchKernelGGL(kern, dim3((unsigned)(s.ntj * s.nsb * s.G)), dim3(kBlock), 0, st,
-                     static_cast<const T*>(pin), static_cast<T*>(pout), g.nx, g.ny, nq, s.qper,
-                     omega, (T)g.dx, (T)g.dy, g.rho, (T)dE, (T)denom, restore, m0, u0p, s.ntj,
-                     s.nsb);
+  const dim3 grid((unsigned)(s.ntj * s.nsb * s.G));
+  const T* in = static_cast<const T*>(pin);
+  T* out = static_cast<T*>(pout);
+  if (nt)
+    launch_body<SweepBody<T, H, true>>(st, grid, 0,
+                                       {in, out, g.nx, g.ny, nq, s.qper, omega, (T)g.dx, (T)g.dy, g.rho,
+                                        (T)dE, (T)denom, restore, m0, u0p, s.ntj, s.nsb});
+  else
+    launch_body<SweepBody<T, H, false>>(st, grid, 0,
+                                        {in, out, g.nx, g.ny, nq, s.qper, omega, (T)g.dx, (T)g.dy, g.rho,
+                                         (T)dE, (T)denom, restore, m0, u0p, s.ntj, s.nsb});
 }
 }  // namespace
 
@@ -1263,6 +1394,7 @@ namespace {
 template <class TC, int NB, int ND, bool TR = false, class Src = GlobalSrc>
 void stream_k_nd(hipStream_t st, const Grid& g, const double* X, int ldx, int r, const Src& Z,
                  int n, double dE, double* K, int ldk) {
+  no_batch("stream_K (VALU form)");
   const dim3 grid(blocks_for(g.C), (n + NB - 1) / NB);
   const size_t lds = (size_t)r * (2 * ND + 2) * NB * sizeof(TC);
   hipLaunchKernelGGL((k_stream_k<TC, NB, ND, TR, Src>), grid, dim3(kBlock), lds, st, X, ldx, r,
@@ -1286,8 +1418,9 @@ bool stream_k_mfma_nd(hipStream_t st, const Grid& g, const double* X, int ldx, i
   const int tiles = (g.C + 15) / 16;
   const int tpw = std::max(2, (tiles + 16383) / 16384 + 1) / 2 * 2;   // even
   const int blocks = (tiles + 4 * tpw - 1) / (4 * tpw);
-  hipLaunchKernelGGL((k_stream_k_mfma<TC, ND, Src>), dim3(blocks, (n + 15) / 16), dim3(kBlock),
-                     lds, st, X, ldx, r, rp, g.nx, g.ny, g.nz, g.rho, Z, n, dE, K, ldk, tpw);
+  using Body = StreamKMfmaBody<TC, ND, Src>;
+  launch_body<Body>(st, dim3(blocks, (n + 15) / 16), lds,
+                    typename Body::P{X, ldx, r, rp, g.nx, g.ny, g.nz, g.rho, Z, n, dE, K, ldk, tpw});
   return true;
 }
 
@@ -1361,6 +1494,8 @@ void launch_tn(hipStream_t st, const Grid& g, const void* A, DT ta, int lda, int
     tn_dispatch<6, 1>(st, blocks, A, ta, lda, m, B, tb, ldb, n, g, fp32, cpw, ngroups, groups_n, partial);
   } else if (nops == 4) {
     tn_dispatch<4, 1>(st, blocks, A, ta, lda, m, B, tb, ldb, n, g, fp32, cpw, ngroups, groups_n, partial);
+  } else if (t_recorder) {
+    tn_dispatch<1, kTnBTiles>(st, blocks, A, ta, lda, m, B, tb, ldb, n, g, fp32, cpw, ngroups, groups_n, partial, nbt);
   } else {
     switch (nbt) {
       case 1: tn_dispatch<1, 1>(st, blocks, A, ta, lda, m, B, tb, ldb, n, g, fp32, cpw, ngroups, groups_n, partial); break;
@@ -1373,8 +1508,8 @@ void launch_tn(hipStream_t st, const Grid& g, const void* A, DT ta, int lda, int
   const int mn = m * n;
   const int per_block = kBlock / 64;
   const int outs = nops * mn;
-  hipLaunchKernelGGL(k_tn_reduce, dim3((outs + per_block - 1) / per_block), dim3(kBlock), 0, st,
-                     partial, nchunks, outs, out);
+  launch_body<TnReduceBody>(st, dim3((outs + per_block - 1) / per_block), 0,
+                            {partial, nchunks, outs, out});
   check_launch("tn_reduce");
 }
 
@@ -1382,8 +1517,8 @@ void launch_chol_inv(hipStream_t st, const double* G, int k, bool allow_plain, d
                      double* Rinv, double* status) {
   if (k <= 0) return;
   if (k > 96) throw std::runtime_error("chol_inv: block wider than 96 columns");
-  hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(kBlock), (size_t)k * k * sizeof(double), st, G, k,
-                     allow_plain ? 1 : 0, R, Rinv, status);
+  launch_body<CholBody>(st, dim3(1), (size_t)k * k * sizeof(double),
+                        {G, k, allow_plain ? 1 : 0, R, Rinv, status});
   check_launch("chol_inv");
 }
 
@@ -1393,15 +1528,20 @@ template <class TA, class TC, class Src>
 void gemm_nb(hipStream_t st, int C, const void* A, int lda, int k, const Src& T, int n,
              double alpha, double beta, double* out, int ldo, const unsigned char* mask) {
   const TA* a = static_cast<const TA*>(A);
-  if (n <= 4)
-    hipLaunchKernelGGL((k_gemm<TA, TC, 4, Src>), dim3(blocks_for(C), 1), dim3(kBlock), 0, st, C, a,
-                       lda, k, T, n, alpha, beta, out, ldo, mask);
+  // every width sums each output column over a in the same order, so a
+  // batched launch takes one width for all items (one kernel, not three)
+  if (t_recorder)
+    launch_body<GemmBody<TA, TC, 16, Src>>(st, dim3(blocks_for(C), (n + 15) / 16), 0,
+                                           {C, a, lda, k, T, n, alpha, beta, out, ldo, mask});
+  else if (n <= 4)
+    launch_body<GemmBody<TA, TC, 4, Src>>(st, dim3(blocks_for(C), 1), 0,
+                                          {C, a, lda, k, T, n, alpha, beta, out, ldo, mask});
   else if (n <= 8)
-    hipLaunchKernelGGL((k_gemm<TA, TC, 8, Src>), dim3(blocks_for(C), 1), dim3(kBlock), 0, st, C, a,
-                       lda, k, T, n, alpha, beta, out, ldo, mask);
+    launch_body<GemmBody<TA, TC, 8, Src>>(st, dim3(blocks_for(C), 1), 0,
+                                          {C, a, lda, k, T, n, alpha, beta, out, ldo, mask});
   else
-    hipLaunchKernelGGL((k_gemm<TA, TC, 16, Src>), dim3(blocks_for(C), (n + 15) / 16), dim3(kBlock),
-                       0, st, C, a, lda, k, T, n, alpha, beta, out, ldo, mask);
+    launch_body<GemmBody<TA, TC, 16, Src>>(st, dim3(blocks_for(C), (n + 15) / 16), 0,
+                                           {C, a, lda, k, T, n, alpha, beta, out, ldo, mask});
 }
 
 template <class Src>
@@ -1440,15 +1580,31 @@ void launch_gather_rows(hipStream_t st, const double* A, int lda, int ncols, con
                         int nrows, double* out) {
   const size_t n = (size_t)nrows * ncols;
   if (n == 0) return;
-  hipLaunchKernelGGL(k_gather_rows, dim3(blocks_for(n)), dim3(kBlock), 0, st, A, lda, ncols, rows,
-                     nrows, out);
+  launch_body<GatherBody>(st, dim3(blocks_for(n)), 0, {A, lda, ncols, rows, nrows, out});
   check_launch("gather_rows");
 }
 
 void launch_scatter_add(hipStream_t st, double* A, int lda, const double* packed, int n) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_scatter_add, dim3(blocks_for(n)), dim3(kBlock), 0, st, A, lda, packed, n);
+  launch_body<ScatterAddBody>(st, dim3(blocks_for(n)), 0, {A, lda, packed, n});
   check_launch("scatter_add");
+}
+
+void launch_copy(hipStream_t st, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return;
+  const uintptr_t al = reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src) | bytes;
+  auto go = [&](auto tag) {
+    using E = decltype(tag);
+    const size_t n = bytes / sizeof(E);
+    const unsigned blocks = (unsigned)std::min<size_t>((n + kBlock - 1) / kBlock, 1024);
+    launch_body<CopyBody<E>>(st, dim3(blocks), 0,
+                             {static_cast<const E*>(src), static_cast<E*>(dst), n});
+  };
+  if (al % 16 == 0 && !t_recorder) go(uint4{});   // batched: one element width
+  else if (al % 8 == 0) go(uint2{});
+  else if (al % 4 == 0) go(0u);
+  else go((unsigned char)0);
+  check_launch("copy");
 }
 
 
@@ -1573,6 +1729,7 @@ __global__ void k_sweep_adj(int nx, int ny, int nq, const double* __restrict__ G
 void launch_adj_combine(hipStream_t st, int C, int n, double* out, int ldo, const double* T, int ldt,
                         const double* V, int ldv, const double* rho, double alpha, double* ybar) {
   if (C == 0) return;
+  no_batch("adj_combine");
   hipLaunchKernelGGL(k_adj_combine, dim3(blocks_for(C)), dim3(kBlock), 0, st, C, n, out, ldo, T,
                      ldt, V, ldv, rho, alpha, ybar);
   check_launch("adj_combine");
@@ -1580,6 +1737,7 @@ void launch_adj_combine(hipStream_t st, int C, int n, double* out, int ldo, cons
 
 void launch_axpy(hipStream_t st, size_t n, double alpha, const double* x, double* y) {
   if (n == 0) return;
+  no_batch("axpy");
   hipLaunchKernelGGL(k_axpy, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, alpha, x, y);
   check_launch("axpy");
 }
@@ -1587,6 +1745,7 @@ void launch_axpy(hipStream_t st, size_t n, double alpha, const double* x, double
 void launch_zero_rows(hipStream_t st, double* A, int lda, int ncols, const int* rows, int nrows) {
   const size_t n = (size_t)nrows * ncols;
   if (n == 0) return;
+  no_batch("zero_rows");
   hipLaunchKernelGGL(k_zero_rows, dim3(blocks_for(n)), dim3(kBlock), 0, st, A, lda, ncols, rows,
                      nrows);
   check_launch("zero_rows");
@@ -1596,6 +1755,7 @@ void launch_dose_adj(hipStream_t st, const Grid& g, const double* Db, double a, 
                      int ldx, int r, const double* v, const double* psi, int nq, const double* m0,
                      double* Xb, double* psib, double* ybar, double* gout) {
   if (g.C == 0) return;
+  no_batch("dose_adj");
   hipLaunchKernelGGL(k_dose_adj, dim3(blocks_for(g.C)), dim3(kBlock), 0, st, g.C, Db, a, g.rho, X,
                      ldx, r, v, psi, nq, m0, Xb, psib, ybar, gout);
   check_launch("dose_adj");
@@ -1606,6 +1766,7 @@ void launch_sweep_adj(hipStream_t st, const Grid& g, const double* G, const doub
                       double* out, double* ybar) {
   if (g.ndim() != 2) throw std::runtime_error("sweep adjoint: 2D grids only");
   if (g.C == 0) return;
+  no_batch("sweep_adj");
   hipLaunchKernelGGL(k_sweep_adj, dim3(blocks_for(g.C)), dim3(kBlock), 0, st, g.nx, g.ny, nq, G,
                      psi_a, omega, g.dx, g.dy, g.rho, dE, denom, restore, out, ybar);
   check_launch("sweep_adj");
@@ -1774,6 +1935,7 @@ void pn_collide_launch(hipStream_t st, const Grid& g, const PnBlocks& b, const d
                        const double* sig, double dE) {
   const int cells = 64 * CT;
   const unsigned blocks = (unsigned)(((g.C + cells - 1) / cells) * b.ngroups);
+  no_batch("pn_collide");
   hipLaunchKernelGGL((k_pn_collide<NBW, CT, KS>), dim3(blocks), dim3(kBlock), 0, st, u, un, g.C,
                      g.nx, g.ny, 1.0 / g.dx, 1.0 / g.dy, g.irho, b.nm, b.nkb, b.ngroups, b.mask,
                      b.start, b.pk, psi, nq, M, degree, sig, dE);

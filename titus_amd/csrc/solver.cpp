@@ -3,6 +3,8 @@
 // live in context.h; the adjoint in adjoint.cpp.
 #include "context.h"
 
+#include <condition_variable>
+
 // ---------------------------------------------------------------------------
 // Launch-plan autotuner of the uncollided sweep (the kernel that moves the
 // most bytes).  Each candidate plan (rows per strip x ordinate groups) is
@@ -60,6 +62,160 @@ static void tune_sweep(titus_ctx& x) {
   hipEventDestroy(b);
   g_tuned[key] = x.sweep_plan;
 }
+
+
+// ---------------------------------------------------------------------------
+// Lockstep batch.  The contexts of a batch advance side by side, one host
+// thread each, on the group's one stream.  A context's launches (kernels and
+// copies) are recorded, not issued; when every live context of the group
+// waits at a sync point (a fetch of device results, or leaving the group), the
+// last to arrive issues the recorded launches of all of them: repeatedly the
+// kernel that most contexts launch next, once for all those contexts
+// (batch.h: k_many), so the launches of each context keep their order.  One
+// copy of the launch items to the device precedes them; a stream sync ends
+// the flush.  The host work of the contexts (ranks, completions, small
+// algebra) runs in their threads between the syncs.
+struct LockstepGroup {
+  std::mutex m;
+  std::condition_variable cv;
+  int live = 0, arrived = 0;
+  uint64_t gen = 0;
+  bool failed = false;
+  std::string error;
+  std::vector<titus_ctx*> members;
+  hipStream_t st = nullptr;
+  unsigned char* h_items = nullptr;     // pinned staging of the launch items
+  unsigned char* d_items = nullptr;
+  size_t cap = 0;
+  std::vector<unsigned char> staging;
+  int64_t flushes = 0, launches = 0, ops = 0;
+  int64_t longest = 0;                    // sum over flushes of the longest recorded list
+  double flush_s = 0.0, sync_s = 0.0;     // wall time in flushes, of it waiting for the stream
+
+  ~LockstepGroup() {
+    if (h_items) (void)hipHostFree(h_items);
+    if (d_items) (void)hipFree(d_items);
+    if (st) (void)hipStreamDestroy(st);
+  }
+  void join(titus_ctx* c) {
+    std::lock_guard<std::mutex> lk(m);
+    ++live;
+    if (std::find(members.begin(), members.end(), c) == members.end()) members.push_back(c);
+  }
+  void sync() {
+    std::unique_lock<std::mutex> lk(m);
+    if (failed) throw TitusError(TITUS_E_HIP, "lockstep batch: " + error);
+    if (++arrived == live) {
+      flush_locked();
+    } else {
+      const uint64_t g = gen;
+      cv.wait(lk, [&] { return gen != g; });
+    }
+    if (failed) throw TitusError(TITUS_E_HIP, "lockstep batch: " + error);
+  }
+  // a context leaves (its solve ended or failed); its last launches stay
+  // recorded until the next flush, which this call makes when everyone else
+  // is already waiting (or nobody is left)
+  void leave() {
+    std::unique_lock<std::mutex> lk(m);
+    --live;
+    if (arrived == live) flush_locked();
+  }
+  void flush_locked() {
+    try {
+      flush_ops();
+    } catch (const std::exception& e) {
+      failed = true;
+      error = e.what();
+      (void)hipStreamSynchronize(st);
+    }
+    arrived = 0;
+    ++gen;
+    cv.notify_all();
+  }
+  void flush_ops() {
+    const auto t0 = std::chrono::steady_clock::now();
+    struct Head { dev::Recorder* r; size_t i; };
+    std::vector<Head> heads;
+    size_t lmax = 0;
+    for (titus_ctx* c : members)
+      if (!c->rec.ops.empty()) {
+        heads.push_back({&c->rec, 0});
+        lmax = std::max(lmax, c->rec.ops.size());
+      }
+    longest += (int64_t)lmax;
+    struct Launch { dev::ManyFn fn; size_t off, isz; int count; unsigned blocks; size_t lds; };
+    std::vector<Launch> plan;
+    std::vector<std::pair<dev::ManyFn, size_t>> next;
+    staging.clear();
+    for (;;) {
+      next.clear();
+      // the next kernel: the one whose contexts have the most launches left
+      // (the weighted majority merge of the recorded lists)
+      for (const Head& h : heads) {
+        if (h.i >= h.r->ops.size()) continue;
+        const dev::ManyFn f = h.r->ops[h.i].fn;
+        const size_t left = h.r->ops.size() - h.i;
+        auto it = std::find_if(next.begin(), next.end(), [&](const auto& e) { return e.first == f; });
+        if (it == next.end()) next.emplace_back(f, left);
+        else it->second += left;
+      }
+      if (next.empty()) break;
+      const auto best = *std::max_element(next.begin(), next.end(),
+                                          [](const auto& a, const auto& b) { return a.second < b.second; });
+      Launch L{best.first, (staging.size() + 15) & ~size_t(15), 0, 0, 0, 0};
+      staging.resize(L.off);
+      for (Head& h : heads) {
+        if (h.i >= h.r->ops.size() || h.r->ops[h.i].fn != best.first) continue;
+        const dev::RecOp& o = h.r->ops[h.i];
+        const size_t at = staging.size();
+        staging.resize(at + o.size);
+        std::memcpy(staging.data() + at, h.r->bytes.data() + o.off, o.size);
+        ++L.count;
+        L.isz = o.size;
+        L.blocks = std::max(L.blocks, o.blocks);
+        L.lds = std::max(L.lds, o.lds);
+        ++h.i;
+        ++ops;
+      }
+      plan.push_back(L);
+    }
+    if (!plan.empty()) {
+      if (staging.size() > cap) {
+        // the stream is idle here: every flush ends with a sync
+        if (h_items) HIPCHK(hipHostFree(h_items));
+        if (d_items) HIPCHK(hipFree(d_items));
+        h_items = nullptr;
+        d_items = nullptr;
+        cap = std::max(staging.size(), 2 * cap);
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&h_items), cap, hipHostMallocDefault));
+        HIPCHK(hipMalloc(reinterpret_cast<void**>(&d_items), cap));
+      }
+      std::memcpy(h_items, staging.data(), staging.size());
+      HIPCHK(hipMemcpyAsync(d_items, h_items, staging.size(), hipMemcpyHostToDevice, st));
+      for (const Launch& L : plan) {
+        // at most 65535 items per launch (the grid's y extent)
+        for (int first = 0; first < L.count; first += 65535) {
+          const int cnt = std::min(65535, L.count - first);
+          L.fn(st, d_items + L.off + (size_t)first * L.isz, cnt, L.blocks, L.lds);
+          const hipError_t e = hipGetLastError();
+          if (e != hipSuccess)
+            throw TitusError(TITUS_E_HIP, std::string("batched launch: ") + hipGetErrorString(e));
+          ++launches;
+        }
+      }
+    }
+    const auto t1 = std::chrono::steady_clock::now();
+    HIPCHK(hipStreamSynchronize(st));
+    const auto t2 = std::chrono::steady_clock::now();
+    for (Head& h : heads) h.r->clear();
+    ++flushes;
+    flush_s += std::chrono::duration<double>(t2 - t0).count();
+    sync_s += std::chrono::duration<double>(t2 - t1).count();
+  }
+};
+
+void lockstep_sync(titus_ctx& x) { x.group->sync(); }
 
 
 extern "C" {
@@ -325,6 +481,120 @@ int titus_solve_batch(titus_ctx** ctxs, int32_t n, int32_t nthreads, int32_t* st
   for (auto& th : pool) th.join();
   int first = TITUS_OK;
   for (int i = 0; i < n; ++i) {
+    if (status) status[i] = codes[i];
+    if (first == TITUS_OK && codes[i] != TITUS_OK) {
+      first = codes[i];
+      g_last_error = "solve " + std::to_string(i) + ": " + msgs[i];
+    }
+  }
+  return first;
+}
+
+int titus_solve_batch_lockstep(titus_ctx** ctxs, int32_t n, int32_t width, int32_t groups,
+                               int32_t* status, int64_t* stats) {
+  if (!ctxs || n < 0) {
+    g_last_error = "bad argument";
+    return TITUS_E_ARG;
+  }
+  if (batch_has_duplicates(ctxs, n)) {
+    g_last_error = "a context appears twice in the batch";
+    return TITUS_E_ARG;
+  }
+  for (int i = 0; i < n; ++i) {
+    titus_ctx* c = ctxs[i];
+    std::string why;
+    if (!c) why = "null context";
+    else if (c->device != ctxs[0]->device) why = "contexts on different devices";
+    else if (c->integrator != TITUS_INTEGRATOR_FIXED && c->integrator != TITUS_INTEGRATOR_BUG &&
+             c->integrator != TITUS_INTEGRATOR_PARALLEL)
+      why = "integrator without a batched form (RK4)";
+    else if (c->nd != 2) why = "3D grid (no batched sweep)";
+    else if (c->flux_order != 1) why = "limited flux (no batched sweep)";
+    else if (c->adj) why = "checkpoints set (adjoint recording runs alone)";
+    if (!why.empty()) {
+      g_last_error = "lockstep batch, context " + std::to_string(i) + ": " + why;
+      return TITUS_E_ARG;
+    }
+  }
+  if (n == 0) return TITUS_OK;
+  const int w = std::max(1, std::min<int>(width > 0 ? width : n, n));
+  std::vector<int> codes(n, TITUS_OK);
+  std::vector<std::string> msgs(n);
+  // `groups` groups of about w / groups contexts, each on its own stream: the
+  // host work of one group overlaps the device work of the others
+  const int G = std::max(1, std::min(groups > 0 ? groups : 1, w));
+  std::vector<std::unique_ptr<LockstepGroup>> grps;
+  for (int k = 0; k < G; ++k) grps.emplace_back(new LockstepGroup());
+  std::vector<int> gof(n, 0);
+  const int dev_id = ctxs[0]->device;
+  int rc = guarded([&] {
+    HIPCHK(hipSetDevice(dev_id));
+    for (auto& gp : grps) HIPCHK(hipStreamCreateWithFlags(&gp->st, hipStreamNonBlocking));
+    for (int i = 0; i < n; ++i) {
+      titus_ctx* c = ctxs[i];
+      // the context's own queued work (creation) is done before the group's
+      HIPCHK(hipStreamSynchronize(c->st));
+      HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->h_arena_dev), c->h_arena, 0));
+      HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->h_upl_dev), c->h_upl, 0));
+    }
+  });
+  if (rc != TITUS_OK) return rc;
+  HostParallelOff hp(ctxs, n, true);
+  // the first w contexts join before any thread starts, so that they start
+  // their first steps together and reach every sync point together (a
+  // context that joined after the others had flushed would stay some phases
+  // apart, and its launches would not share the others')
+  for (int i = 0; i < w; ++i) {
+    gof[i] = i % G;
+    grps[i % G]->join(ctxs[i]);
+  }
+  std::atomic<int> next{w};
+  auto worker = [&](int t) {
+    (void)hipSetDevice(dev_id);
+    LockstepGroup& grp = *grps[t % G];
+    for (int i = t;;) {
+      if (i >= n) return;
+      titus_ctx* c = ctxs[i];
+      if (i >= w) {
+        gof[i] = t % G;
+        grp.join(c);
+      }
+      c->group = &grp;
+      dev::t_recorder = &c->rec;
+      codes[i] = guarded([&] {
+        const int last = c->integrator == TITUS_INTEGRATOR_BUG ? c->ne - 2 : c->ne - 1;
+        for (int s = c->steps_done + 1; s <= last; ++s) c->step(s);
+      });
+      if (codes[i] != TITUS_OK) msgs[i] = g_last_error;
+      dev::t_recorder = nullptr;
+      c->group = nullptr;
+      grp.leave();
+      i = next.fetch_add(1);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < w; ++t) pool.emplace_back(worker, t);
+  worker(0);
+  for (auto& th : pool) th.join();
+  // every context has left; the last leave of each group flushed what remained
+  if (stats) {
+    std::fill(stats, stats + 6, int64_t(0));
+    for (auto& gp : grps) {
+      stats[0] += gp->flushes;
+      stats[1] += gp->launches;
+      stats[2] += gp->ops;
+      stats[3] += (int64_t)(gp->flush_s * 1e9);
+      stats[4] += (int64_t)(gp->sync_s * 1e9);
+      stats[5] += gp->longest;
+    }
+  }
+  int first = TITUS_OK;
+  for (int i = 0; i < n; ++i) {
+    const LockstepGroup& grp = *grps[gof[i]];
+    if (codes[i] == TITUS_OK && grp.failed) {
+      codes[i] = TITUS_E_HIP;
+      msgs[i] = "lockstep batch: " + grp.error;
+    }
     if (status) status[i] = codes[i];
     if (first == TITUS_OK && codes[i] != TITUS_OK) {
       first = codes[i];
