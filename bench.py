@@ -179,52 +179,73 @@ def run_form(P, precision, W, K, period, sync, rank_tol):
 
 
 def beamlet_batch(n_beamlets, nthreads, sync):
-    """Config-4 plan: FP64 solves one after another vs the mixed batch.
+    """Config-4 plan: the plain FP64 GPU path (one beamlet after another)
+    against the batched forms.
 
-    Both sets of contexts are created (operands in HBM) before their timed
-    region.  Returns throughput of both, their ratio and per-beamlet dose
-    differences (mixed vs FP64) of the same plan.
+    Batched forms: shared launches (titus_solve_batch_lockstep: all beamlets
+    in LOCKSTEP_GROUPS groups, every kernel one launch per group), mixed and
+    FP64; and one stream per beamlet over `nthreads` host workers
+    (titus_solve_batch), mixed.  Every set of contexts is created (operands
+    in HBM) before its timed region.  Returns throughput of each, the ratios
+    and per-beamlet dose differences (mixed vs FP64) of the same plan.
     """
     from titus_amd import _lib
-    from titus_amd.beamlets import beamlet_plan
+    from titus_amd.beamlets import LOCKSTEP_GROUPS, beamlet_plan
     from titus_amd.solver import MIXED_RANK_TOL
     plan = beamlet_plan(n_beamlets=n_beamlets)
     total_steps = sum(P["nE"] - 2 for _, P in plan)
 
-    def run(precision, threads):
+    def run(precision, mode):
         ctxs = [_lib.Context(P, _lib.INTEGRATOR_BUG, precision=precision, rank_tol=MIXED_RANK_TOL)
                 for _, P in plan]
+        stats = None
         try:
             sync()
             t0 = time.perf_counter()
-            _lib.solve_batch(ctxs, threads)
+            if mode == "lockstep":
+                _, stats = _lib.solve_batch_lockstep(ctxs, 0, LOCKSTEP_GROUPS)
+            elif mode == "streams":
+                _lib.solve_batch(ctxs, nthreads)
+            else:
+                for c in ctxs:
+                    c.solve(0)
             sync()
             dt = time.perf_counter() - t0
             doses = np.stack([c.state()["dose"] for c in ctxs], axis=1)
         finally:
             for c in ctxs:
                 c.close()
-        return dt, doses
+        return dt, doses, stats
 
-    t64, d64 = run(_lib.PRECISION_FP64, 1)
-    t64b, d64b = run(_lib.PRECISION_FP64, nthreads)
-    tmx, dmx = run(_lib.PRECISION_MIXED, nthreads)
-    errs = np.linalg.norm(dmx - d64, axis=0) / np.linalg.norm(d64, axis=0)
+    t64, d64, _ = run(_lib.PRECISION_FP64, "one_at_a_time")
+    tl64, dl64, st64 = run(_lib.PRECISION_FP64, "lockstep")
+    tlmx, dlmx, stmx = run(_lib.PRECISION_MIXED, "lockstep")
+    tsmx, dsmx, _ = run(_lib.PRECISION_MIXED, "streams")
+    errs = np.linalg.norm(dlmx - d64, axis=0) / np.linalg.norm(d64, axis=0)
     w = np.ones(len(plan))
-    plan_err = float(np.linalg.norm((dmx - d64) @ w) / np.linalg.norm(d64 @ w))
+    plan_err = float(np.linalg.norm((dlmx - d64) @ w) / np.linalg.norm(d64 @ w))
     rate = lambda t: {"beamlets_per_s": n_beamlets / t, "energy_steps_per_s": total_steps / t,
                       "seconds": t}
+
+    def shared(st):
+        return {"groups": LOCKSTEP_GROUPS, "host_syncs": st["flushes"], "launches": st["launches"],
+                "recorded_launches": st["ops"],
+                "beamlets_per_launch": st["ops"] / max(st["launches"], 1),
+                "launches_per_beamlet_step": st["launches"] / total_steps}
     return {
         "workload": f"{n_beamlets} pencil beamlets on a 128x128 ellipse phantom (10x10 cm), "
                     "P_15 (136 moments), 6-20 MeV, +-15 deg, rank caps 5-40, seed 0",
         "beamlet_steps": int(total_steps),
-        "mixed_batched": dict(rate(tmx), host_threads=nthreads),
-        "fp64_batched": dict(rate(t64b), host_threads=nthreads),
+        "mixed_batched": dict(rate(tlmx), form="shared launches", **shared(stmx)),
+        "fp64_batched": dict(rate(tl64), form="shared launches", **shared(st64)),
+        "mixed_one_stream_per_beamlet": dict(rate(tsmx), host_threads=nthreads),
         "fp64_one_at_a_time": rate(t64),
-        "speedup": t64 / tmx,
-        "speedup_from_batching_fp64": t64 / t64b,
-        "speedup_from_precision_batched": t64b / tmx,
-        "fp64_batched_equals_one_at_a_time": bool(np.array_equal(d64, d64b)),
+        "speedup_mixed_batched_vs_fp64_one_at_a_time": t64 / tlmx,
+        "speedup_from_batching_fp64": t64 / tl64,
+        "speedup_from_precision_batched": tl64 / tlmx,
+        "speedup_shared_launches_vs_streams_mixed": tsmx / tlmx,
+        "batched_equals_one_at_a_time_fp64": bool(np.array_equal(d64, dl64)),
+        "shared_launches_equal_streams_mixed": bool(np.array_equal(dlmx, dsmx)),
         "dose_rel_l2_per_beamlet": {"max": float(errs.max()), "median": float(np.median(errs)),
                                     "above_1e-4": int((errs > 1e-4).sum()),
                                     "worst": [[int(b), float(errs[b])] for b in np.argsort(errs)[::-1][:3]]},
@@ -655,6 +676,8 @@ def main():
     ap.add_argument("--profile-form", choices=("fp64", "mixed"), default=None,
                     help="only this form's headline solve (steps 1..advance+warmup+steps, the "
                          "window timed), for profiler passes; prints its rate")
+    ap.add_argument("--beamlets-only", action="store_true",
+                    help="only the batched beamlet plan")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the timed FP64 and mixed phantom windows (for profiler runs)")
     args = ap.parse_args()
@@ -679,6 +702,9 @@ def main():
         return
     if args.full_pn_only:
         print(json.dumps({"full_pn": full_pn_bench(sync, 0 if args.no_cpu else 2)}), flush=True)
+        return
+    if args.beamlets_only:
+        print(json.dumps({"beamlets": beamlet_batch(args.beamlets, args.threads, sync)}), flush=True)
         return
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_bench(sync, args.threads, fd_dirs=args.fd_dirs)}), flush=True)

@@ -1,4 +1,5 @@
-"""Batched beamlet solves (titus_solve_batch) on the GPU.
+"""Batched beamlet solves on the GPU: one stream per beamlet (titus_solve_batch)
+and, for the mixed-form check, shared launches (titus_solve_batch_lockstep).
 
 Pins: the concurrent batch equals independent sequential solves bit for bit
 in FP64 (each solve's arithmetic does not depend on what runs beside it);
@@ -27,7 +28,7 @@ def small_plan():
 
 def test_batch_equals_sequential_bitwise(small_plan):
     print(f"\n[case] batch of {len(small_plan)} beamlets, 40x40, P_7", flush=True)
-    batched, steps = solve_plan(small_plan, precision=_lib.PRECISION_FP64, nthreads=6)
+    batched, steps = solve_plan(small_plan, precision=_lib.PRECISION_FP64, nthreads=6, lockstep=False)
     seq = []
     for _, P in small_plan:
         ctx = _lib.Context(P, _lib.INTEGRATOR_BUG)
@@ -42,7 +43,7 @@ def test_batch_equals_sequential_bitwise(small_plan):
 
 
 def test_batch_matches_oracle(small_plan):
-    batched, _ = solve_plan(small_plan, precision=_lib.PRECISION_FP64, nthreads=3)
+    batched, _ = solve_plan(small_plan, precision=_lib.PRECISION_FP64, nthreads=3, lockstep=False)
     for b in (0, 3):
         ref = dlra.solve_bug(small_plan[b][1])["dose"]
         e = rel(batched[:, b], ref)

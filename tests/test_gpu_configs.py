@@ -153,9 +153,9 @@ def plan():
 def test_beamlet_plan_batch_equals_independent_solves(plan):
     announce("64-beamlet plan 128x128 P_15: batch vs independent solves, FP64")
     assert len(plan) == 64 and plan[0][1]["Nm"] == 136 and plan[0][1]["nx"] == 128
-    batched, steps = solve_plan(plan, precision=_lib.PRECISION_FP64, nthreads=16)
-    # each solve's arithmetic does not depend on what runs beside it: compare
-    # eight of them, solved alone, bit for bit
+    batched, steps = solve_plan(plan, precision=_lib.PRECISION_FP64)
+    # each solve's arithmetic does not depend on what runs beside it or on
+    # its launches being shared: compare eight of them, solved alone, bit for bit
     for b in range(0, 64, 8):
         g = run(plan[b][1], INTEGRATOR_BUG)
         assert np.array_equal(g["dose"], batched[:, b]), b
@@ -185,8 +185,8 @@ def test_beamlet_plan_mixed_error_per_beamlet(plan):
     (step 156); once the kept rank differs for a step the doses part by
     1e-4 to 5e-4."""
     announce("64-beamlet plan: mixed batch vs FP64 batch, per beamlet")
-    f64, _ = solve_plan(plan, precision=_lib.PRECISION_FP64, nthreads=16, rank_tol=MIXED_RANK_TOL)
-    mix, _ = solve_plan(plan, precision=_lib.PRECISION_MIXED, nthreads=16, rank_tol=MIXED_RANK_TOL)
+    f64, _ = solve_plan(plan, precision=_lib.PRECISION_FP64, rank_tol=MIXED_RANK_TOL)
+    mix, _ = solve_plan(plan, precision=_lib.PRECISION_MIXED, rank_tol=MIXED_RANK_TOL)
     errs = np.array([rel(mix[:, b], f64[:, b]) for b in range(64)])
     worst = np.argsort(errs)[::-1][:4]
     w = np.random.default_rng(0).uniform(0.5, 1.5, 64)

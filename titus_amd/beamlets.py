@@ -5,7 +5,9 @@ treatment plan is a weighted sum of beamlet doses, D(w) = sum_b w_b D_b; each
 beamlet is an independent rank-adaptive DLRA solve with its own entry point,
 direction, energy and rank cap.  ``beamlet_plan`` draws such a plan from a
 seed; ``solve_plan`` runs every beamlet on the GPU concurrently through
-``titus_solve_batch`` and returns the per-beamlet doses.
+``titus_solve_batch_lockstep`` (the kernels of all beamlets that reach the
+same point of the energy step run as one launch) or ``titus_solve_batch``
+(one stream per beamlet) and returns the per-beamlet doses.
 """
 import math
 
@@ -51,9 +53,17 @@ def beamlet_plan(n_beamlets=64, n=128, N=15, seed=0, rank_range=(5, 40), e_range
     return plan
 
 
-def solve_plan(plan, precision=_lib.PRECISION_MIXED, nthreads=0, device=0, rank_tol=None):
+LOCKSTEP_GROUPS = 4   # groups of a lockstep batch (host work of one overlaps the others' kernels)
+
+
+def solve_plan(plan, precision=_lib.PRECISION_MIXED, nthreads=0, device=0, rank_tol=None,
+               lockstep=True, groups=LOCKSTEP_GROUPS):
     """Solve every beamlet concurrently; returns (doses C x B, steps per beamlet).
 
+    ``lockstep``: shared launches (``titus_solve_batch_lockstep``, all beamlets
+    at once in ``groups`` groups; ``nthreads`` > 0 limits how many run at
+    once); otherwise one stream per beamlet over ``nthreads`` host workers
+    (``titus_solve_batch``).  Both give the same doses bit for bit.
     ``rank_tol`` None: the default cut of the precision (``solver.RANK_TOL`` or
     ``solver.MIXED_RANK_TOL``).
     """
@@ -62,7 +72,10 @@ def solve_plan(plan, precision=_lib.PRECISION_MIXED, nthreads=0, device=0, rank_
     ctxs = [_lib.Context(P, INTEGRATOR_BUG, precision=precision, device=device, rank_tol=rank_tol)
             for _, P in plan]
     try:
-        _lib.solve_batch(ctxs, nthreads)
+        if lockstep:
+            _lib.solve_batch_lockstep(ctxs, nthreads, groups)
+        else:
+            _lib.solve_batch(ctxs, nthreads)
         doses = np.stack([c.state()["dose"] for c in ctxs], axis=1)
         steps = np.array([c.progress()[0] for c in ctxs])
     finally:

@@ -126,6 +126,8 @@ inline void no_batch(const char* what) {
 // dst[0 : bytes) = src[0 : bytes) on the device (either side may be pinned
 // host memory mapped for the device); recorded like any kernel
 void launch_copy(hipStream_t st, void* dst, const void* src, size_t bytes);
+// the same copy recorded into `r` (whatever the calling thread records into)
+void record_copy(Recorder& r, void* dst, const void* src, size_t bytes);
 
 }  // namespace dev
 }  // namespace titus

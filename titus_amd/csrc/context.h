@@ -251,6 +251,7 @@ struct titus_ctx {
   LockstepGroup* group = nullptr;
   dev::Recorder rec;
   double *h_arena_dev = nullptr, *h_upl_dev = nullptr;
+  size_t upl_sent = 0;      // upload ring prefix the group has copied to the device
   double *partial = nullptr, *arena = nullptr, *upl = nullptr;
   int *bcells = nullptr, *cand_d = nullptr;
   unsigned char *bmask = nullptr, *restore = nullptr;
@@ -332,9 +333,8 @@ struct titus_ctx {
   // then wait for the stream
   void fetch() {
     if (group) {
-      copy_arena_out();
       lockstep_sync(*this);
-      upl_top = 0;
+      upl_top = upl_sent = 0;
       return;
     }
     if (arena_top > fetched_top)
@@ -344,14 +344,6 @@ struct titus_ctx {
     HostScope hs(timing, "host:wait");
     HIPCHK(hipStreamSynchronize(st));
     upl_top = 0;          // every queued upload has been consumed
-  }
-  // in a group: the arena beyond the prefix already copied goes to the pinned
-  // mirror by a recorded copy kernel
-  void copy_arena_out() {
-    if (arena_top > fetched_top)
-      dev::launch_copy(st, h_arena_dev + fetched_top, arena + fetched_top,
-                       (arena_top - fetched_top) * sizeof(double));
-    fetched_top = arena_top;
   }
   // device-to-device copy on the context's stream (a recorded copy kernel in
   // a group)
@@ -363,10 +355,8 @@ struct titus_ctx {
   // wait_part() waits for that copy only, so the host can use those results
   // while later work runs on the device
   void fetch_part() {
-    if (group) {
-      copy_arena_out();
-      return;
-    }
+    // in a group the whole arena reaches the host at the end of every flush
+    if (group) return;
     if (arena_top > fetched_top)
       HIPCHK(hipMemcpyAsync(h_arena + fetched_top, arena + fetched_top,
                             (arena_top - fetched_top) * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -390,10 +380,10 @@ struct titus_ctx {
     if (group) {
       // the ring is reset only at the group's full syncs: it must hold a phase
       if (upl_top + n > upl_n) throw TitusError(TITUS_E_ARG, "upload ring too small for a batched phase");
-      double* h = h_upl + upl_top;
+      // copied to the device with the group's other uploads before the
+      // flush's first kernel
       double* d = upl + upl_top;
-      std::memcpy(h, src, n * sizeof(double));
-      dev::launch_copy(st, d, h_upl_dev + upl_top, n * sizeof(double));
+      std::memcpy(h_upl + upl_top, src, n * sizeof(double));
       upl_top += (n + 31) & ~size_t(31);
       return d;
     }

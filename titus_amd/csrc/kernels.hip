@@ -1607,6 +1607,18 @@ void launch_copy(hipStream_t st, void* dst, const void* src, size_t bytes) {
   check_launch("copy");
 }
 
+void record_copy(Recorder& r, void* dst, const void* src, size_t bytes) {
+  Recorder* saved = t_recorder;
+  t_recorder = &r;
+  try {
+    launch_copy(nullptr, dst, src, bytes);
+  } catch (...) {
+    t_recorder = saved;
+    throw;
+  }
+  t_recorder = saved;
+}
+
 
 // ---------------------------------------------------------------------------
 // Adjoint kernels (FP64).  y = 1/rho throughout.

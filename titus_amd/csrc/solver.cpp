@@ -133,8 +133,46 @@ struct LockstepGroup {
     ++gen;
     cv.notify_all();
   }
+  struct Launch { dev::ManyFn fn; size_t off, isz; int count; unsigned blocks; size_t lds; };
+  // every op of `r` (independent copies) in launches by kernel
+  void plan_all(const dev::Recorder& r, std::vector<Launch>& plan) {
+    std::vector<bool> done(r.ops.size(), false);
+    for (size_t a = 0; a < r.ops.size(); ++a) {
+      if (done[a]) continue;
+      Launch L{r.ops[a].fn, (staging.size() + 15) & ~size_t(15), 0, 0, 0, 0};
+      staging.resize(L.off);
+      for (size_t b = a; b < r.ops.size(); ++b) {
+        const dev::RecOp& o = r.ops[b];
+        if (done[b] || o.fn != L.fn) continue;
+        done[b] = true;
+        const size_t at = staging.size();
+        staging.resize(at + o.size);
+        std::memcpy(staging.data() + at, r.bytes.data() + o.off, o.size);
+        ++L.count;
+        L.isz = o.size;
+        L.blocks = std::max(L.blocks, o.blocks);
+        L.lds = std::max(L.lds, o.lds);
+      }
+      plan.push_back(L);
+    }
+  }
   void flush_ops() {
     const auto t0 = std::chrono::steady_clock::now();
+    // the contexts' uploads since the last flush go to the device before the
+    // first kernel, their result arenas to the host after the last one
+    dev::Recorder pro, epi;
+    for (titus_ctx* c : members) {
+      if (c->upl_top > c->upl_sent) {
+        dev::record_copy(pro, c->upl + c->upl_sent, c->h_upl_dev + c->upl_sent,
+                         (c->upl_top - c->upl_sent) * sizeof(double));
+        c->upl_sent = c->upl_top;
+      }
+      if (c->arena_top > c->fetched_top) {
+        dev::record_copy(epi, c->h_arena_dev + c->fetched_top, c->arena + c->fetched_top,
+                         (c->arena_top - c->fetched_top) * sizeof(double));
+        c->fetched_top = c->arena_top;
+      }
+    }
     struct Head { dev::Recorder* r; size_t i; };
     std::vector<Head> heads;
     size_t lmax = 0;
@@ -144,10 +182,10 @@ struct LockstepGroup {
         lmax = std::max(lmax, c->rec.ops.size());
       }
     longest += (int64_t)lmax;
-    struct Launch { dev::ManyFn fn; size_t off, isz; int count; unsigned blocks; size_t lds; };
     std::vector<Launch> plan;
     std::vector<std::pair<dev::ManyFn, size_t>> next;
     staging.clear();
+    plan_all(pro, plan);
     for (;;) {
       next.clear();
       // the next kernel: the one whose contexts have the most launches left
@@ -180,6 +218,7 @@ struct LockstepGroup {
       }
       plan.push_back(L);
     }
+    plan_all(epi, plan);
     if (!plan.empty()) {
       if (staging.size() > cap) {
         // the stream is idle here: every flush ends with a sync
@@ -544,19 +583,29 @@ int titus_solve_batch_lockstep(titus_ctx** ctxs, int32_t n, int32_t width, int32
   // their first steps together and reach every sync point together (a
   // context that joined after the others had flushed would stay some phases
   // apart, and its launches would not share the others')
-  for (int i = 0; i < w; ++i) {
-    gof[i] = i % G;
-    grps[i % G]->join(ctxs[i]);
+  // groups are buckets of similar shape: the first w contexts sorted by
+  // moment count and rank capacity, dealt out in contiguous runs, so that a
+  // group's launches share kernel variants and tile counts
+  std::vector<int> start(w);
+  std::iota(start.begin(), start.end(), 0);
+  std::stable_sort(start.begin(), start.end(), [&](int a, int b) {
+    return std::make_pair(ctxs[a]->nm, ctxs[a]->capx) < std::make_pair(ctxs[b]->nm, ctxs[b]->capx);
+  });
+  std::vector<int> tgroup(w);
+  for (int t = 0; t < w; ++t) {
+    tgroup[t] = (int)((int64_t)t * G / w);
+    gof[start[t]] = tgroup[t];
+    grps[tgroup[t]]->join(ctxs[start[t]]);
   }
   std::atomic<int> next{w};
   auto worker = [&](int t) {
     (void)hipSetDevice(dev_id);
-    LockstepGroup& grp = *grps[t % G];
-    for (int i = t;;) {
+    LockstepGroup& grp = *grps[tgroup[t]];
+    for (int i = start[t];;) {
       if (i >= n) return;
       titus_ctx* c = ctxs[i];
       if (i >= w) {
-        gof[i] = t % G;
+        gof[i] = tgroup[t];
         grp.join(c);
       }
       c->group = &grp;
