@@ -269,8 +269,12 @@ def slab_bench(sync, cpu):
                        f"{P['Q']} ordinates), fixed rank 20, {P['nE'] - 1} steps"}
     doses = {}
     for name, prec, tol in (("fp64", _lib.PRECISION_FP64, RANK_TOL),
-                            ("mixed", _lib.PRECISION_MIXED, MIXED_RANK_TOL)):
+                            ("mixed", _lib.PRECISION_MIXED, MIXED_RANK_TOL),
+                            ("fp64_floor_1e-4", _lib.PRECISION_FP64, MIXED_RANK_TOL)):
         out[name], doses[name] = whole_solve(P, INTEGRATOR_FIXED, prec, tol, sync)
+    # the precision's share (same rank floor) and the floor's share
+    out["dose_rel_l2_mixed_vs_fp64_same_floor"] = rel(doses["mixed"], doses["fp64_floor_1e-4"])
+    out["dose_rel_l2_fp64_floor_1e-4_vs_1e-7"] = rel(doses["fp64_floor_1e-4"], doses["fp64"])
     out["dose_rel_l2_mixed_vs_fp64"] = rel(doses["mixed"], doses["fp64"])
     if cpu:
         from oracle import dlra

@@ -333,6 +333,12 @@ struct titus_ctx {
   // then wait for the stream
   void fetch() {
     if (group) {
+      // nothing recorded, uploaded or allocated since the last flush: the
+      // host already holds every result (a fetch right after wait_part)
+      if (!group_pending()) {
+        upl_top = upl_sent = 0;
+        return;
+      }
       lockstep_sync(*this);
       upl_top = upl_sent = 0;
       return;
@@ -344,6 +350,10 @@ struct titus_ctx {
     HostScope hs(timing, "host:wait");
     HIPCHK(hipStreamSynchronize(st));
     upl_top = 0;          // every queued upload has been consumed
+  }
+  // in a group: launches, uploads or results waiting for the group's flush
+  bool group_pending() const {
+    return !rec.ops.empty() || arena_top > fetched_top || upl_top > upl_sent;
   }
   // device-to-device copy on the context's stream (a recorded copy kernel in
   // a group)
@@ -365,7 +375,7 @@ struct titus_ctx {
   }
   void wait_part() {
     if (group) {
-      lockstep_sync(*this);
+      if (group_pending()) lockstep_sync(*this);
       return;
     }
     HostScope hs(timing, "host:wait");

@@ -1081,6 +1081,89 @@ __device__ __forceinline__ void d_gemm(Blk blk, int C, const TA* __restrict__ A,
   }
 }
 
+// The same product on the matrix cores, computed transposed as the K-step
+// is: the result tile out^T[b, c] of 16 columns b0.. by 16 cells c0.. is
+// sum_a T[a, b] A[c, a] -- A operand (lane l) T[a0 + l/16, b0 + l%16] from
+// LDS (staged as Ts[a*16 + b], zero padded to kp = k rounded up to 4), B
+// operand cell c0 + l%16 of column a0 + l/16 (16 consecutive cells per
+// column: coalesced loads), and a result lane holds one cell of four output
+// columns (coalesced stores).  Each wavefront walks `tpw` consecutive
+// 16-cell tiles, two at a time, the next four columns loaded while the
+// current four are multiplied.  out = alpha acc (+ beta out), rows with
+// mask[c] != 0 set to 0.
+template <class TA, class TC, class Src>
+__device__ __forceinline__ void d_gemm_mfma(Blk blk, int C, const TA* __restrict__ A, int lda, int k,
+                                            int kp, const Src& T, int n, double alpha, double beta,
+                                            double* __restrict__ out, int ldo,
+                                            const unsigned char* __restrict__ mask, int tpw) {
+  using MF = Mfma<TC>;
+  using acc_t = typename MF::acc_t;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  TC* Ts = reinterpret_cast<TC*>(lds_raw);
+  const int b0 = blk.y * 16;
+  const int nbv = min(16, n - b0);
+  for (int e = threadIdx.x; e < kp * 16; e += blockDim.x) {
+    const int b = e & 15, a = e >> 4;
+    Ts[e] = (b < nbv && a < k) ? (TC)T[(size_t)(b0 + b) * k + a] : TC(0);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cl = lane & 15, kg = lane >> 4;
+  const int L = xcd_block(blk.x, blk.nx);
+  constexpr int TT = 2;
+  const int tile0 = (L * 4 + wave) * tpw;            // tpw is a multiple of TT
+  for (int t = 0; t < tpw; t += TT) {
+    if ((tile0 + t) * 16 >= C) break;
+    int cc[TT];
+    bool live[TT];
+#pragma unroll
+    for (int u = 0; u < TT; ++u) {
+      const int c = (tile0 + t + u) * 16 + cl;
+      live[u] = c < C;
+      cc[u] = live[u] ? c : 0;                       // a readable cell for dead lanes
+    }
+    // unconditional loads (column 0 for the padding columns), zeros at use
+    TA cur[TT], nxt[TT];
+    auto fetch = [&](TA (&v)[TT], int ab) {
+      const int a = ab + kg;
+      const TA* xa = A + (size_t)(a < k ? a : 0) * lda;
+#pragma unroll
+      for (int u = 0; u < TT; ++u) v[u] = xa[cc[u]];
+    };
+    fetch(cur, 0);
+    acc_t acc[TT];
+#pragma unroll
+    for (int u = 0; u < TT; ++u) acc[u] = acc_t{};
+    for (int ab = 0; ab < kp; ab += 4) {
+      if (ab + 4 < kp) fetch(nxt, ab + 4);
+      const int a = ab + kg;
+      const TC ta = Ts[(size_t)a * 16 + cl];
+#pragma unroll
+      for (int u = 0; u < TT; ++u) {
+        const TC x = (live[u] && a < k) ? (TC)cur[u] : TC(0);
+        acc[u] = MF::step(ta, x, acc[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < TT; ++u) cur[u] = nxt[u];
+    }
+#pragma unroll
+    for (int u = 0; u < TT; ++u) {
+      if (!live[u]) continue;
+      const int c = cc[u];
+      const bool zero = mask && mask[c];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int b = MF::row(kg, v);
+        if (b >= nbv) continue;
+        double* o = out + (size_t)(b0 + b) * ldo + c;
+        double val = alpha * (double)acc[u][v];
+        if (beta != 0.0) val += beta * (*o);
+        *o = zero ? 0.0 : val;
+      }
+    }
+  }
+}
+
 __device__ __forceinline__ void d_gather_rows(Blk blk, const double* __restrict__ A, int lda, int ncols,
                               const int* __restrict__ rows, int nrows, double* __restrict__ out) {
   const size_t t = (size_t)blk.x * blockDim.x + threadIdx.x;
@@ -1188,6 +1271,19 @@ struct GemmBody {
   __device__ static void run(const P& p, Blk b) {
     d_gemm<TA, TC, NB, Src>(b, p.C, p.A, p.lda, p.k, p.T, p.n, p.alpha, p.beta, p.out, p.ldo,
                             p.mask);
+  }
+};
+template <class TA, class TC, class Src>
+struct GemmMfmaBody {
+  struct P {
+    int C; const TA* A; int lda, k, kp; Src T; int n; double alpha, beta; double* out; int ldo;
+    const unsigned char* mask; int tpw;
+  };
+  static constexpr int kThreads = kBlock;
+  static constexpr bool kBatch = !std::is_same<Src, ArgSrc>::value;
+  __device__ static void run(const P& p, Blk b) {
+    d_gemm_mfma<TA, TC, Src>(b, p.C, p.A, p.lda, p.k, p.kp, p.T, p.n, p.alpha, p.beta, p.out,
+                             p.ldo, p.mask, p.tpw);
   }
 };
 struct GatherBody {
@@ -1528,6 +1624,18 @@ template <class TA, class TC, class Src>
 void gemm_nb(hipStream_t st, int C, const void* A, int lda, int k, const Src& T, int n,
              double alpha, double beta, double* out, int ldo, const unsigned char* mask) {
   const TA* a = static_cast<const TA*>(A);
+  // the matrix-core form (every shape whose k x 16 slice of T fits 64 KiB of
+  // LDS): about 4096 workgroups per column tile at most, an even number of
+  // 16-cell tiles per wavefront
+  const int kp = (k + 3) / 4 * 4;
+  if (k > 0 && (size_t)kp * 16 * sizeof(TC) <= 65536) {
+    const int tiles = (C + 15) / 16;
+    const int tpw = std::max(2, (tiles + 16383) / 16384 + 1) / 2 * 2;
+    const int blocks = (tiles + 4 * tpw - 1) / (4 * tpw);
+    launch_body<GemmMfmaBody<TA, TC, Src>>(st, dim3(blocks, (n + 15) / 16), (size_t)kp * 16 * sizeof(TC),
+                                           {C, a, lda, k, kp, T, n, alpha, beta, out, ldo, mask, tpw});
+    return;
+  }
   // every width sums each output column over a in the same order, so a
   // batched launch takes one width for all items (one kernel, not three)
   if (t_recorder)
