@@ -74,17 +74,23 @@ struct RecOp {
   size_t size;
   unsigned blocks;  // workgroups of the item's own grid
   size_t lds;       // dynamic LDS bytes
+  // variants of a kernel with the same item layout: a batched launch of
+  // items with variant indices v runs alts[max v] (e.g. the narrowest
+  // template that covers every item's tile count); null: fn
+  int var = 0;
+  const ManyFn* alts = nullptr;
 };
 
 // Launches of one host thread, in issue order.
 struct Recorder {
   std::vector<RecOp> ops;
   std::vector<unsigned char> bytes;
-  void add(ManyFn fn, const void* item, size_t size, unsigned blocks, size_t lds) {
+  void add(ManyFn fn, const void* item, size_t size, unsigned blocks, size_t lds, int var = 0,
+           const ManyFn* alts = nullptr) {
     const size_t off = (bytes.size() + 15) & ~size_t(15);
     bytes.resize(off + size);
     std::memcpy(bytes.data() + off, item, size);
-    ops.push_back(RecOp{fn, off, size, blocks, lds});
+    ops.push_back(RecOp{fn, off, size, blocks, lds, var, alts});
   }
   void clear() {
     ops.clear();
@@ -116,6 +122,15 @@ void launch_body(hipStream_t st, dim3 grid, size_t lds, const typename Body::P& 
     }
   }
   hipLaunchKernelGGL(k_one<Body>, grid, dim3(Body::kThreads), lds, st, p);
+}
+
+// A batched launch with variants: the item is recorded under Body's kernel
+// (the grouping key) with variant index `var` into `alts` (kernels whose
+// items have Body's layout).
+template <class Body>
+void record_variant(dim3 grid, size_t lds, const typename Body::P& p, int var, const ManyFn* alts) {
+  const Item<typename Body::P> it{p, (int)grid.x, (int)grid.y};
+  t_recorder->add(&launch_many<Body>, &it, sizeof(it), grid.x * grid.y, lds, var, alts);
 }
 
 // launchers of kernels with no batched form refuse to run while recording

@@ -1317,10 +1317,22 @@ void tn_mfma_launch(hipStream_t st, int blocks, const void* A, int lda, int m, c
                     int ldb, int n, const Grid& g, int cpw, int ngroups, int groups_n,
                     int nbt, double* partial) {
   using Body = TnBody<NOPS, NBT, TB, TC>;
-  launch_body<Body>(st, dim3(blocks), 0,
-                    typename Body::P{static_cast<const double*>(A), lda, m, static_cast<const TB*>(B),
-                                     ldb, n, g.C, g.nx, g.ny, g.nz, 1.0 / g.dx, 1.0 / g.dy,
-                                     1.0 / g.dz, g.irho, cpw, ngroups, groups_n, nbt, partial});
+  const typename Body::P p{static_cast<const double*>(A), lda, m, static_cast<const TB*>(B),
+                           ldb, n, g.C, g.nx, g.ny, g.nz, 1.0 / g.dx, 1.0 / g.dy,
+                           1.0 / g.dz, g.irho, cpw, ngroups, groups_n, nbt, partial};
+  if (NOPS == 1 && NBT == kTnBTiles && t_recorder) {
+    // batched plain products: one grouping key for every B-tile count; the
+    // launch runs the narrowest template that covers its items' counts (the
+    // parameter structs of the four templates have one layout)
+    static_assert(sizeof(typename TnBody<1, 1, TB, TC>::P) == sizeof(typename Body::P), "layout");
+    static const ManyFn alts[kTnBTiles] = {&launch_many<TnBody<1, 1, TB, TC>>,
+                                           &launch_many<TnBody<1, 2, TB, TC>>,
+                                           &launch_many<TnBody<1, 3, TB, TC>>,
+                                           &launch_many<TnBody<1, 4, TB, TC>>};
+    record_variant<Body>(dim3(blocks), 0, p, nbt - 1, alts);
+    return;
+  }
+  launch_body<Body>(st, dim3(blocks), 0, p);
 }
 
 template <int NOPS, int NBT>

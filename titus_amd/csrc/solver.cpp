@@ -133,13 +133,16 @@ struct LockstepGroup {
     ++gen;
     cv.notify_all();
   }
-  struct Launch { dev::ManyFn fn; size_t off, isz; int count; unsigned blocks; size_t lds; };
+  struct Launch {
+    dev::ManyFn fn; size_t off, isz; int count; unsigned blocks; size_t lds;
+    int var = 0; const dev::ManyFn* alts = nullptr;
+  };
   // every op of `r` (independent copies) in launches by kernel
   void plan_all(const dev::Recorder& r, std::vector<Launch>& plan) {
     std::vector<bool> done(r.ops.size(), false);
     for (size_t a = 0; a < r.ops.size(); ++a) {
       if (done[a]) continue;
-      Launch L{r.ops[a].fn, (staging.size() + 15) & ~size_t(15), 0, 0, 0, 0};
+      Launch L{r.ops[a].fn, (staging.size() + 15) & ~size_t(15), 0, 0, 0, 0, 0, r.ops[a].alts};
       staging.resize(L.off);
       for (size_t b = a; b < r.ops.size(); ++b) {
         const dev::RecOp& o = r.ops[b];
@@ -152,6 +155,7 @@ struct LockstepGroup {
         L.isz = o.size;
         L.blocks = std::max(L.blocks, o.blocks);
         L.lds = std::max(L.lds, o.lds);
+        L.var = std::max(L.var, o.var);
       }
       plan.push_back(L);
     }
@@ -201,7 +205,7 @@ struct LockstepGroup {
       if (next.empty()) break;
       const auto best = *std::max_element(next.begin(), next.end(),
                                           [](const auto& a, const auto& b) { return a.second < b.second; });
-      Launch L{best.first, (staging.size() + 15) & ~size_t(15), 0, 0, 0, 0};
+      Launch L{best.first, (staging.size() + 15) & ~size_t(15), 0, 0, 0, 0, 0, nullptr};
       staging.resize(L.off);
       for (Head& h : heads) {
         if (h.i >= h.r->ops.size() || h.r->ops[h.i].fn != best.first) continue;
@@ -213,6 +217,8 @@ struct LockstepGroup {
         L.isz = o.size;
         L.blocks = std::max(L.blocks, o.blocks);
         L.lds = std::max(L.lds, o.lds);
+        L.var = std::max(L.var, o.var);
+        L.alts = o.alts;
         ++h.i;
         ++ops;
       }
@@ -236,7 +242,8 @@ struct LockstepGroup {
         // at most 65535 items per launch (the grid's y extent)
         for (int first = 0; first < L.count; first += 65535) {
           const int cnt = std::min(65535, L.count - first);
-          L.fn(st, d_items + L.off + (size_t)first * L.isz, cnt, L.blocks, L.lds);
+          const dev::ManyFn fn = L.alts ? L.alts[L.var] : L.fn;
+          fn(st, d_items + L.off + (size_t)first * L.isz, cnt, L.blocks, L.lds);
           const hipError_t e = hipGetLastError();
           if (e != hipSuccess)
             throw TitusError(TITUS_E_HIP, std::string("batched launch: ") + hipGetErrorString(e));
