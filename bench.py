@@ -178,7 +178,7 @@ def run_form(P, precision, W, K, period, sync, rank_tol):
                 ranks=ranks, dose=dose)
 
 
-def beamlet_batch(n_beamlets, nthreads, sync):
+def beamlet_batch(n_beamlets, nthreads, sync, profile_only=False, oracle_steps=40):
     """Config-4 plan: the plain FP64 GPU path (one beamlet after another)
     against the batched forms.
 
@@ -194,6 +194,9 @@ def beamlet_batch(n_beamlets, nthreads, sync):
     from titus_amd.solver import MIXED_RANK_TOL
     plan = beamlet_plan(n_beamlets=n_beamlets)
     total_steps = sum(P["nE"] - 2 for _, P in plan)
+
+    def rate(t):
+        return {"beamlets_per_s": n_beamlets / t, "energy_steps_per_s": total_steps / t, "seconds": t}
 
     def run(precision, mode):
         ctxs = [_lib.Context(P, _lib.INTEGRATOR_BUG, precision=precision, rank_tol=MIXED_RANK_TOL)
@@ -217,6 +220,9 @@ def beamlet_batch(n_beamlets, nthreads, sync):
                 c.close()
         return dt, doses, stats
 
+    if profile_only:
+        tlmx, _, stmx = run(_lib.PRECISION_MIXED, "lockstep")
+        return {"mixed_batched": dict(rate(tlmx), **stmx)}
     t64, d64, _ = run(_lib.PRECISION_FP64, "one_at_a_time")
     tl64, dl64, st64 = run(_lib.PRECISION_FP64, "lockstep")
     tlmx, dlmx, stmx = run(_lib.PRECISION_MIXED, "lockstep")
@@ -224,9 +230,6 @@ def beamlet_batch(n_beamlets, nthreads, sync):
     errs = np.linalg.norm(dlmx - d64, axis=0) / np.linalg.norm(d64, axis=0)
     w = np.ones(len(plan))
     plan_err = float(np.linalg.norm((dlmx - d64) @ w) / np.linalg.norm(d64 @ w))
-    rate = lambda t: {"beamlets_per_s": n_beamlets / t, "energy_steps_per_s": total_steps / t,
-                      "seconds": t}
-
     def shared(st):
         return {"groups": LOCKSTEP_GROUPS, "host_syncs": st["flushes"], "launches": st["launches"],
                 "recorded_launches": st["ops"],
@@ -250,9 +253,33 @@ def beamlet_batch(n_beamlets, nthreads, sync):
                                     "above_1e-4": int((errs > 1e-4).sum()),
                                     "worst": [[int(b), float(errs[b])] for b in np.argsort(errs)[::-1][:3]]},
         "plan_dose_rel_l2": plan_err,
+        "vs_oracle": beamlet_vs_oracle(plan[0][1], oracle_steps),
         "note": "per-beamlet errors are mixed vs FP64 at the same rank floor; the FP64 solves "
                 "match the oracle to ~5e-12 (GPU tests, two beamlets over 120 steps)",
     }
+
+
+def beamlet_vs_oracle(P, steps):
+    """Beamlet 0 of the plan over its first `steps` steps: FP64 and mixed
+    device doses against the FP64 oracle's (the oracle at the plain floor;
+    the mixed form at its own floor, and the oracle rerun at that floor)."""
+    from oracle import dlra
+    from titus_amd import _lib
+    from titus_amd.problem import INTEGRATOR_BUG
+    from titus_amd.solver import MIXED_RANK_TOL, RANK_TOL
+    ref = dlra.solve_bug(P, steps=steps)["dose"]
+    saved = dlra.RANK_TOL
+    try:
+        dlra.RANK_TOL = MIXED_RANK_TOL
+        ref4 = dlra.solve_bug(P, steps=steps)["dose"]
+    finally:
+        dlra.RANK_TOL = saved
+    return {"beamlet": 0, "steps": steps,
+            "dose_rel_l2_fp64": rel(device_dose(P, INTEGRATOR_BUG, _lib.PRECISION_FP64, steps, RANK_TOL), ref),
+            "dose_rel_l2_mixed_vs_oracle_same_floor": rel(
+                device_dose(P, INTEGRATOR_BUG, _lib.PRECISION_MIXED, steps, MIXED_RANK_TOL), ref4),
+            "dose_rel_l2_mixed_vs_oracle_plain_floor": rel(
+                device_dose(P, INTEGRATOR_BUG, _lib.PRECISION_MIXED, steps, MIXED_RANK_TOL), ref)}
 
 
 def slab_bench(sync, cpu):
@@ -682,6 +709,8 @@ def main():
                          "window timed), for profiler passes; prints its rate")
     ap.add_argument("--beamlets-only", action="store_true",
                     help="only the batched beamlet plan")
+    ap.add_argument("--profile-batch", action="store_true",
+                    help="only the mixed shared-launch batch of the beamlet plan, for profiler runs")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the timed FP64 and mixed phantom windows (for profiler runs)")
     args = ap.parse_args()
@@ -706,6 +735,10 @@ def main():
         return
     if args.full_pn_only:
         print(json.dumps({"full_pn": full_pn_bench(sync, 0 if args.no_cpu else 2)}), flush=True)
+        return
+    if args.profile_batch:
+        print(json.dumps({"beamlets": beamlet_batch(args.beamlets, args.threads, sync, profile_only=True)}),
+              flush=True)
         return
     if args.beamlets_only:
         print(json.dumps({"beamlets": beamlet_batch(args.beamlets, args.threads, sync)}), flush=True)
@@ -809,13 +842,22 @@ def main():
     else:
         rec, s1, ref = cpu_baseline(P, args.cpu_seconds, A + W)
         out["cpu_baseline"] = rec
+        from oracle import dlra
+        saved = dlra.RANK_TOL
+        try:
+            dlra.RANK_TOL = MIXED_RANK_TOL
+            ref4 = dlra.solve_bug(P, steps=s1)["dose"]
+        finally:
+            dlra.RANK_TOL = saved
+        mixed_dose = device_dose(P, INTEGRATOR_BUG, _lib.PRECISION_MIXED, s1, MIXED_RANK_TOL)
         out["phantom_vs_oracle"] = {
             "step": s1,
             "dose_rel_l2_fp64": rel(device_dose(P, INTEGRATOR_BUG, _lib.PRECISION_FP64, s1, RANK_TOL), ref),
-            "dose_rel_l2_mixed": rel(device_dose(P, INTEGRATOR_BUG, _lib.PRECISION_MIXED, s1,
-                                                 MIXED_RANK_TOL), ref),
-            "note": "the oracle runs at the plain rank floor 1e-7; the mixed form runs at 1e-4, so "
-                    "its difference is mostly the floor's (whole_solve.dose_rel_l2)",
+            "dose_rel_l2_mixed_vs_oracle_same_floor": rel(mixed_dose, ref4),
+            "dose_rel_l2_mixed_vs_oracle_plain_floor": rel(mixed_dose, ref),
+            "note": "the mixed form runs at the rank floor 1e-4; against the FP64 oracle at that "
+                    "floor its difference is the precision's, against the oracle at the plain "
+                    "floor 1e-7 it is mostly the floor's (whole_solve.dose_rel_l2)",
         }
     print(json.dumps(out), flush=True)
 

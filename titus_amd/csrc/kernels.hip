@@ -975,10 +975,13 @@ __device__ __forceinline__ void d_chol_inv(Blk blk, const double* __restrict__ G
   __shared__ double Dg[kMaxChol];
   __shared__ double tr;
   const int tid = threadIdx.x, nt = blockDim.x;
-  if (tid == 0) {
+  // the trace: wavefront 0 loads the diagonal in parallel (one load latency,
+  // not k of them in a row) and adds it in a fixed shuffle tree
+  if (tid < 64) {
     double t = 0.0;
-    for (int i = 0; i < k; ++i) t += G[(size_t)i * k + i];
-    tr = t;
+    for (int i = tid; i < k; i += 64) t += G[(size_t)i * k + i];
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+    if (tid == 0) tr = t;
   }
   __syncthreads();
   double s = allow_plain ? 0.0 : (tr > 0.0 ? 1e-14 * tr : 1e-300);
