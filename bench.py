@@ -234,7 +234,8 @@ def beamlet_batch(n_beamlets, nthreads, sync, profile_only=False, oracle_steps=4
         return {"groups": LOCKSTEP_GROUPS, "host_syncs": st["flushes"], "launches": st["launches"],
                 "recorded_launches": st["ops"],
                 "beamlets_per_launch": st["ops"] / max(st["launches"], 1),
-                "launches_per_beamlet_step": st["launches"] / total_steps}
+                "launches_per_beamlet_step": st["launches"] / total_steps,
+                "tuned_tiles_per_group": st["tiles"]}
     return {
         "workload": f"{n_beamlets} pencil beamlets on a 128x128 ellipse phantom (10x10 cm), "
                     "P_15 (136 moments), 6-20 MeV, +-15 deg, rank caps 5-40, seed 0",

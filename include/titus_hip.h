@@ -171,10 +171,18 @@ int titus_solve_batch(struct titus_ctx** ctxs, int32_t n, int32_t nthreads, int3
  * launches, the recorded launches they carried, nanoseconds spent in flushes
  * and, of those, waiting for the streams, and the sum over flushes of the
  * longest recorded list (the fewest launches those flushes could take).
+ * Groups are buckets of similar contexts (moment count, rank capacity); each
+ * tunes its tiles while it runs: over its first 40 flushes it times the
+ * matrix-core GEMM and K-step at 2, 4 and 8 16-cell tiles per wavefront and
+ * the sweep at strips of 2, 4, 8 rows (and non-temporal streaming), then
+ * keeps the fastest of each.  These choices regroup the same tiles into
+ * workgroups and change no result.  tiles (optional, 4 per group, groups
+ * clamped to [1, width]): the chosen GEMM and K-step tiles per wavefront,
+ * the sweep's rows per strip and non-temporal flag.
  * No reference counterpart: the reference solves one beam per call
  * (code/SolverCSD.jl:891, 2011). */
 int titus_solve_batch_lockstep(struct titus_ctx** ctxs, int32_t n, int32_t width, int32_t groups,
-                               int32_t* status, int64_t* stats);
+                               int32_t* status, int64_t* stats, int32_t* tiles);
 
 /* The pencil beam of the RK4 integrator's analytic uncollided flux
  * (SolverCSD.jl:1437-1440, 1549-1557):

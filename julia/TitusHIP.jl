@@ -444,17 +444,19 @@ end
 """Solve many 2D contexts with shared launches: every kernel of the energy
 step runs once for all contexts of a group (`groups` groups of about
 `width / groups` contexts; `width` <= 0: all at once).  Returns the status
-codes and the batch's counters (flushes, launches, recorded launches,
+codes, the batch's counters (flushes, launches, recorded launches,
 nanoseconds in flushes, of them waiting for the streams, and the fewest
-launches the flushes could take)."""
+launches the flushes could take) and each group's tuned tiles (4 values per
+group)."""
 function solve_batch_lockstep!(ctxs::Vector{TitusContext}; width::Integer=0, groups::Integer=4)
     ptrs = [c.ptr for c in ctxs]
     status = zeros(Int32, length(ctxs))
     stats = zeros(Int64, 6)
+    tiles = zeros(Int32, 4 * max(1, groups))
     check(ccall((:titus_solve_batch_lockstep, libtitus), Cint,
-                (Ptr{Ptr{Cvoid}}, Int32, Int32, Int32, Ptr{Int32}, Ptr{Int64}),
-                ptrs, Int32(length(ctxs)), Int32(width), Int32(groups), status, stats))
-    return status, stats
+                (Ptr{Ptr{Cvoid}}, Int32, Int32, Int32, Ptr{Int32}, Ptr{Int64}, Ptr{Int32}),
+                ptrs, Int32(length(ctxs)), Int32(width), Int32(groups), status, stats, tiles))
+    return status, stats, tiles
 end
 
 set_checkpoints!(ctx::TitusContext, every::Integer) =

@@ -115,7 +115,7 @@ def load():
     lib.titus_solve.argtypes = [vp, ctypes.c_int32]
     lib.titus_solve_batch.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32, _PI]
     lib.titus_solve_batch_lockstep.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32,
-                                               ctypes.c_int32, _PI, ctypes.POINTER(ctypes.c_int64)]
+                                               ctypes.c_int32, _PI, ctypes.POINTER(ctypes.c_int64), _PI]
     lib.titus_progress.argtypes = [vp, _PI, _PI]
     lib.titus_get_state.argtypes = [vp, _P, _P, _P, _P, _P]
     lib.titus_get_history.argtypes = [vp, _PI, _P, ctypes.c_int32]
@@ -276,18 +276,24 @@ def solve_batch_lockstep(contexts, width=0, groups=1):
     of its solves.
 
     Returns (per-context status codes, {"flushes", "launches", "ops", "flush_s",
-    "stream_wait_s", "launches_lower_bound"}); raises on the first failure.
+    "stream_wait_s", "launches_lower_bound", "tiles"}), "tiles" holding each
+    group's tuned tile choices; raises on the first failure.
     """
     lib = load()
     n = len(contexts)
     arr = (ctypes.c_void_p * n)(*[c.h for c in contexts])
     status = np.zeros(n, dtype=np.int32)
     stats = (ctypes.c_int64 * 6)()
-    code = lib.titus_solve_batch_lockstep(arr, n, width, groups, _ip(status), stats)
+    g = max(1, min(groups if groups > 0 else 1, width if width > 0 else n, max(n, 1)))
+    tiles = np.zeros(4 * g, dtype=np.int32)
+    code = lib.titus_solve_batch_lockstep(arr, n, width, groups, _ip(status), stats, _ip(tiles))
     check(code)
     return status, {"flushes": int(stats[0]), "launches": int(stats[1]), "ops": int(stats[2]),
                     "flush_s": stats[3] * 1e-9, "stream_wait_s": stats[4] * 1e-9,
-                    "launches_lower_bound": int(stats[5])}
+                    "launches_lower_bound": int(stats[5]),
+                    "tiles": [{"gemm_tiles_per_wave": int(t[0]), "kstep_tiles_per_wave": int(t[1]),
+                               "sweep_rows": int(t[2]), "sweep_nontemporal": int(t[3])}
+                              for t in tiles.reshape(g, 4)]}
 
 
 def adjoint_batch(contexts, dose_bars, nthreads=0):

@@ -79,19 +79,27 @@ struct RecOp {
   // template that covers every item's tile count); null: fn
   int var = 0;
   const ManyFn* alts = nullptr;
+  int family = 0;   // TuneFamily of the launch (0: not tuned)
+  double work = 0;  // its work in the family's unit (16-cell tiles, cell-ordinates)
 };
+
+// Tile choices of a lockstep group, set by the group's tuner between flushes
+// (0, or -1 for sweep_nt: the launcher's own choice).  None of them changes
+// a workgroup's arithmetic; they regroup the same tiles into workgroups.
+struct BatchTuning {
+  int gemm_tpw = 0;    // 16-cell tiles per wavefront, matrix-core GEMM
+  int kstep_tpw = 0;   // the same, matrix-core K-step
+  int sweep_rows = 0;  // rows per strip of the upwind sweep
+  int sweep_nt = -1;   // non-temporal psi streaming of the sweep (0 / 1)
+};
+enum TuneFamily { TF_NONE = 0, TF_GEMM = 1, TF_KSTEP = 2, TF_SWEEP = 3, TF_COUNT = 3 };
 
 // Launches of one host thread, in issue order.
 struct Recorder {
   std::vector<RecOp> ops;
   std::vector<unsigned char> bytes;
   void add(ManyFn fn, const void* item, size_t size, unsigned blocks, size_t lds, int var = 0,
-           const ManyFn* alts = nullptr) {
-    const size_t off = (bytes.size() + 15) & ~size_t(15);
-    bytes.resize(off + size);
-    std::memcpy(bytes.data() + off, item, size);
-    ops.push_back(RecOp{fn, off, size, blocks, lds, var, alts});
-  }
+           const ManyFn* alts = nullptr);
   void clear() {
     ops.clear();
     bytes.clear();
@@ -100,6 +108,22 @@ struct Recorder {
 
 // the calling thread's recorder (null: launches are issued directly)
 inline thread_local Recorder* t_recorder = nullptr;
+// the tile choices of the calling thread's group (null: the launchers' own)
+inline thread_local const BatchTuning* t_tune = nullptr;
+// family and work of the next launch the calling thread records (set by a
+// tuned launcher just before it records; consumed by Recorder::add)
+inline thread_local int t_family = TF_NONE;
+inline thread_local double t_work = 0.0;
+
+inline void Recorder::add(ManyFn fn, const void* item, size_t size, unsigned blocks, size_t lds,
+                          int var, const ManyFn* alts) {
+  const size_t off = (bytes.size() + 15) & ~size_t(15);
+  bytes.resize(off + size);
+  std::memcpy(bytes.data() + off, item, size);
+  ops.push_back(RecOp{fn, off, size, blocks, lds, var, alts, t_family, t_work});
+  t_family = TF_NONE;
+  t_work = 0.0;
+}
 
 template <class Body>
 void launch_many(hipStream_t st, const void* items, int count, unsigned blocks, size_t lds) {
@@ -111,7 +135,10 @@ void launch_many(hipStream_t st, const void* items, int count, unsigned blocks, 
 // coefficient blocks carried in the arguments) have no batched form.
 template <class Body>
 void launch_body(hipStream_t st, dim3 grid, size_t lds, const typename Body::P& p) {
-  if (grid.x == 0 || grid.y == 0) return;
+  if (grid.x == 0 || grid.y == 0) {
+    t_family = TF_NONE;
+    return;
+  }
   if (Recorder* r = t_recorder) {
     if constexpr (Body::kBatch) {
       const Item<typename Body::P> it{p, (int)grid.x, (int)grid.y};

@@ -1476,24 +1476,36 @@ void launch_sweep(hipStream_t st, const Grid& g, const void* pin, void* pout, DT
     throw std::runtime_error("sweep: ordinate groups out of range");
   if (plan.limited && (g.ndim() != 2 || plan.nontemporal))
     throw std::runtime_error("sweep: the limited flux is 2D and has no non-temporal form");
-  const SweepShape s = sweep_shape(g, nq, plan);
+  // a batched sweep may take its group's strip height and streaming mode;
+  // the ordinate groups (which set the moment partials) stay the context's
+  SweepPlan pl = plan;
+  if (t_recorder && t_tune && !plan.limited && g.ndim() == 2) {
+    if (t_tune->sweep_rows > 0) pl.rows = t_tune->sweep_rows;
+    if (t_tune->sweep_nt >= 0) pl.nontemporal = t_tune->sweep_nt != 0;
+  }
+  if (t_recorder) {
+    t_family = TF_SWEEP;
+    t_work = (double)g.C * nq;
+  }
+  const SweepShape s = sweep_shape(g, nq, pl);
   const bool f64 = tpsi == DT::F64;
-  switch (plan.rows) {
+  const SweepPlan& plan_ = pl;
+  switch (plan_.rows) {
     case 2:
-      if (f64) sweep_launch<double, 2>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal, plan.limited);
-      else sweep_launch<float, 2>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal, plan.limited);
+      if (f64) sweep_launch<double, 2>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan_.nontemporal, plan_.limited);
+      else sweep_launch<float, 2>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan_.nontemporal, plan_.limited);
       break;
     case 4:
-      if (f64) sweep_launch<double, 4>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal, plan.limited);
-      else sweep_launch<float, 4>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal, plan.limited);
+      if (f64) sweep_launch<double, 4>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan_.nontemporal, plan_.limited);
+      else sweep_launch<float, 4>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan_.nontemporal, plan_.limited);
       break;
     case 8:
-      if (f64) sweep_launch<double, 8>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal, plan.limited);
-      else sweep_launch<float, 8>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal, plan.limited);
+      if (f64) sweep_launch<double, 8>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan_.nontemporal, plan_.limited);
+      else sweep_launch<float, 8>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan_.nontemporal, plan_.limited);
       break;
     case 16:
-      if (f64) sweep_launch<double, 16>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal, plan.limited);
-      else sweep_launch<float, 16>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan.nontemporal, plan.limited);
+      if (f64) sweep_launch<double, 16>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan_.nontemporal, plan_.limited);
+      else sweep_launch<float, 16>(st, g, pin, pout, nq, omega, dE, denom, restore, m0, u0p, s, plan_.nontemporal, plan_.limited);
       break;
     default:
       throw std::runtime_error("sweep: rows per strip must be 2, 4, 8 or 16");
@@ -1527,8 +1539,13 @@ bool stream_k_mfma_nd(hipStream_t st, const Grid& g, const double* X, int ldx, i
   const size_t lds = (size_t)(2 * ND + 2) * rp * 16 * sizeof(TC);
   if (lds > 65536) return false;
   const int tiles = (g.C + 15) / 16;
-  const int tpw = std::max(2, (tiles + 16383) / 16384 + 1) / 2 * 2;   // even
+  int tpw = std::max(2, (tiles + 16383) / 16384 + 1) / 2 * 2;   // even
+  if (t_recorder && t_tune && t_tune->kstep_tpw > 0) tpw = t_tune->kstep_tpw;
   const int blocks = (tiles + 4 * tpw - 1) / (4 * tpw);
+  if (t_recorder) {
+    t_family = TF_KSTEP;
+    t_work = (double)tiles * ((n + 15) / 16);
+  }
   using Body = StreamKMfmaBody<TC, ND, Src>;
   launch_body<Body>(st, dim3(blocks, (n + 15) / 16), lds,
                     typename Body::P{X, ldx, r, rp, g.nx, g.ny, g.nz, g.rho, Z, n, dE, K, ldk, tpw});
@@ -1645,8 +1662,13 @@ void gemm_nb(hipStream_t st, int C, const void* A, int lda, int k, const Src& T,
   const int kp = (k + 3) / 4 * 4;
   if (k > 0 && (size_t)kp * 16 * sizeof(TC) <= 65536) {
     const int tiles = (C + 15) / 16;
-    const int tpw = std::max(2, (tiles + 16383) / 16384 + 1) / 2 * 2;
+    int tpw = std::max(2, (tiles + 16383) / 16384 + 1) / 2 * 2;
+    if (t_recorder && t_tune && t_tune->gemm_tpw > 0) tpw = t_tune->gemm_tpw;
     const int blocks = (tiles + 4 * tpw - 1) / (4 * tpw);
+    if (t_recorder) {
+      t_family = TF_GEMM;
+      t_work = (double)tiles * ((n + 15) / 16);
+    }
     launch_body<GemmMfmaBody<TA, TC, Src>>(st, dim3(blocks, (n + 15) / 16), (size_t)kp * 16 * sizeof(TC),
                                            {C, a, lda, k, kp, T, n, alpha, beta, out, ldo, mask, tpw});
     return;

@@ -90,9 +90,53 @@ struct LockstepGroup {
   std::vector<unsigned char> staging;
   int64_t flushes = 0, launches = 0, ops = 0;
   int64_t longest = 0;                    // sum over flushes of the longest recorded list
+  // Tile tuner of the group (a bucket of similar beamlets): for its first
+  // kRounds rounds of kRoundFlushes flushes, round i gives every tuned
+  // family its candidate i mod (count); every launch of a tuned family is
+  // timed with events and its time divided by its work (16-cell tiles or
+  // cell-ordinates); then each family keeps its fastest candidate.  The
+  // candidates regroup the same tiles into workgroups (batch.h,
+  // BatchTuning), so the tuner changes no result.
+  static constexpr int kRoundFlushes = 4, kRounds = 10;
+  dev::BatchTuning tune;
+  bool tuning = true;
+  int tuned_flushes = 0;
+  const std::vector<int> cand_gemm{2, 4, 8}, cand_kstep{2, 4, 8};
+  const std::vector<std::pair<int, int>> cand_sweep{{2, 0}, {4, 0}, {8, 0}, {2, 1}, {4, 1}};
+  double t_sum[dev::TF_COUNT][8] = {}, w_sum[dev::TF_COUNT][8] = {};
+  std::vector<hipEvent_t> evs;
+  int ncand(int f) const {
+    return f == 0 ? (int)cand_gemm.size() : f == 1 ? (int)cand_kstep.size() : (int)cand_sweep.size();
+  }
+  void set_round(int round) {
+    tune.gemm_tpw = cand_gemm[round % cand_gemm.size()];
+    tune.kstep_tpw = cand_kstep[round % cand_kstep.size()];
+    tune.sweep_rows = cand_sweep[round % cand_sweep.size()].first;
+    tune.sweep_nt = cand_sweep[round % cand_sweep.size()].second;
+  }
+  void finish_tuning() {
+    int best[dev::TF_COUNT] = {0, 0, 0};
+    for (int f = 0; f < dev::TF_COUNT; ++f) {
+      double bt = -1.0;
+      for (int c = 0; c < ncand(f); ++c)
+        if (w_sum[f][c] > 0.0 && (bt < 0.0 || t_sum[f][c] / w_sum[f][c] < bt)) {
+          bt = t_sum[f][c] / w_sum[f][c];
+          best[f] = c;
+        }
+    }
+    tune.gemm_tpw = cand_gemm[best[0]];
+    tune.kstep_tpw = cand_kstep[best[1]];
+    tune.sweep_rows = cand_sweep[best[2]].first;
+    tune.sweep_nt = cand_sweep[best[2]].second;
+    tuning = false;
+  }
   double flush_s = 0.0, sync_s = 0.0;     // wall time in flushes, of it waiting for the stream
+  std::vector<std::pair<int, size_t>> pending_timed;
+  std::vector<double> tuned_work;
 
+  LockstepGroup() { set_round(0); }
   ~LockstepGroup() {
+    for (hipEvent_t e : evs) (void)hipEventDestroy(e);
     if (h_items) (void)hipHostFree(h_items);
     if (d_items) (void)hipFree(d_items);
     if (st) (void)hipStreamDestroy(st);
@@ -136,6 +180,7 @@ struct LockstepGroup {
   struct Launch {
     dev::ManyFn fn; size_t off, isz; int count; unsigned blocks; size_t lds;
     int var = 0; const dev::ManyFn* alts = nullptr;
+    int family = 0; double work = 0.0;
   };
   // every op of `r` (independent copies) in launches by kernel
   void plan_all(const dev::Recorder& r, std::vector<Launch>& plan) {
@@ -219,6 +264,8 @@ struct LockstepGroup {
         L.lds = std::max(L.lds, o.lds);
         L.var = std::max(L.var, o.var);
         L.alts = o.alts;
+        L.family = o.family;
+        L.work += o.work;
         ++h.i;
         ++ops;
       }
@@ -238,7 +285,22 @@ struct LockstepGroup {
       }
       std::memcpy(h_items, staging.data(), staging.size());
       HIPCHK(hipMemcpyAsync(d_items, h_items, staging.size(), hipMemcpyHostToDevice, st));
+      // events around the tuned launches while the tuner runs
+      std::vector<std::pair<int, size_t>> timed;      // (family, first event)
+      if (tuning) {
+        size_t need = 0;
+        for (const Launch& L : plan)
+          if (L.family > 0) need += 2;
+        while (evs.size() < need) {
+          hipEvent_t e;
+          HIPCHK(hipEventCreate(&e));
+          evs.push_back(e);
+        }
+      }
+      size_t ev_next = 0;
       for (const Launch& L : plan) {
+        const bool time_it = tuning && L.family > 0;
+        if (time_it) HIPCHK(hipEventRecord(evs[ev_next], st));
         // at most 65535 items per launch (the grid's y extent)
         for (int first = 0; first < L.count; first += 65535) {
           const int cnt = std::min(65535, L.count - first);
@@ -249,11 +311,36 @@ struct LockstepGroup {
             throw TitusError(TITUS_E_HIP, std::string("batched launch: ") + hipGetErrorString(e));
           ++launches;
         }
+        if (time_it) {
+          HIPCHK(hipEventRecord(evs[ev_next + 1], st));
+          timed.emplace_back(L.family, ev_next);
+          tuned_work.push_back(L.work);
+          ev_next += 2;
+        }
       }
+      pending_timed = std::move(timed);
     }
     const auto t1 = std::chrono::steady_clock::now();
     HIPCHK(hipStreamSynchronize(st));
     const auto t2 = std::chrono::steady_clock::now();
+    if (tuning) {
+      const int round = tuned_flushes / kRoundFlushes;
+      for (size_t q = 0; q < pending_timed.size(); ++q) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, evs[pending_timed[q].second], evs[pending_timed[q].second + 1]));
+        const int f = pending_timed[q].first - 1;
+        const int c = round % ncand(f);
+        t_sum[f][c] += ms;
+        w_sum[f][c] += tuned_work[q];
+      }
+      pending_timed.clear();
+      tuned_work.clear();
+      ++tuned_flushes;
+      if (tuned_flushes % kRoundFlushes == 0) {
+        if (tuned_flushes / kRoundFlushes >= kRounds) finish_tuning();
+        else set_round(tuned_flushes / kRoundFlushes);
+      }
+    }
     for (Head& h : heads) h.r->clear();
     ++flushes;
     flush_s += std::chrono::duration<double>(t2 - t0).count();
@@ -537,7 +624,7 @@ int titus_solve_batch(titus_ctx** ctxs, int32_t n, int32_t nthreads, int32_t* st
 }
 
 int titus_solve_batch_lockstep(titus_ctx** ctxs, int32_t n, int32_t width, int32_t groups,
-                               int32_t* status, int64_t* stats) {
+                               int32_t* status, int64_t* stats, int32_t* tiles) {
   if (!ctxs || n < 0) {
     g_last_error = "bad argument";
     return TITUS_E_ARG;
@@ -617,12 +704,14 @@ int titus_solve_batch_lockstep(titus_ctx** ctxs, int32_t n, int32_t width, int32
       }
       c->group = &grp;
       dev::t_recorder = &c->rec;
+      dev::t_tune = &grp.tune;
       codes[i] = guarded([&] {
         const int last = c->integrator == TITUS_INTEGRATOR_BUG ? c->ne - 2 : c->ne - 1;
         for (int s = c->steps_done + 1; s <= last; ++s) c->step(s);
       });
       if (codes[i] != TITUS_OK) msgs[i] = g_last_error;
       dev::t_recorder = nullptr;
+      dev::t_tune = nullptr;
       c->group = nullptr;
       grp.leave();
       i = next.fetch_add(1);
@@ -644,6 +733,14 @@ int titus_solve_batch_lockstep(titus_ctx** ctxs, int32_t n, int32_t width, int32
       stats[5] += gp->longest;
     }
   }
+  if (tiles)
+    for (int k = 0; k < G; ++k) {
+      const dev::BatchTuning& t = grps[k]->tune;
+      tiles[4 * k] = t.gemm_tpw;
+      tiles[4 * k + 1] = t.kstep_tpw;
+      tiles[4 * k + 2] = t.sweep_rows;
+      tiles[4 * k + 3] = t.sweep_nt;
+    }
   int first = TITUS_OK;
   for (int i = 0; i < n; ++i) {
     const LockstepGroup& grp = *grps[gof[i]];
