@@ -227,7 +227,11 @@ def beamlet_batch(n_beamlets, nthreads, sync, profile_only=False, oracle_steps=4
     tl64, dl64, st64 = run(_lib.PRECISION_FP64, "lockstep")
     tlmx, dlmx, stmx = run(_lib.PRECISION_MIXED, "lockstep")
     tsmx, dsmx, _ = run(_lib.PRECISION_MIXED, "streams")
+    # the form with the streaming stencil, psi and Galerkin products in FP32
+    # as well (FP32 accumulation), batched the same way
+    tlf32, dlf32, stf32 = run(_lib.PRECISION_MIXED_FULL_FP32, "lockstep")
     errs = np.linalg.norm(dlmx - d64, axis=0) / np.linalg.norm(d64, axis=0)
+    errs32 = np.linalg.norm(dlf32 - d64, axis=0) / np.linalg.norm(d64, axis=0)
     w = np.ones(len(plan))
     plan_err = float(np.linalg.norm((dlmx - d64) @ w) / np.linalg.norm(d64 @ w))
     def shared(st):
@@ -243,6 +247,13 @@ def beamlet_batch(n_beamlets, nthreads, sync, profile_only=False, oracle_steps=4
         "mixed_batched": dict(rate(tlmx), form="shared launches", **shared(stmx)),
         "fp64_batched": dict(rate(tl64), form="shared launches", **shared(st64)),
         "mixed_one_stream_per_beamlet": dict(rate(tsmx), host_threads=nthreads),
+        "mixed_full_fp32_batched": dict(
+            rate(tlf32), form="shared launches",
+            speedup_vs_fp64_one_at_a_time=t64 / tlf32,
+            dose_rel_l2_per_beamlet={"max": float(errs32.max()), "median": float(np.median(errs32)),
+                                     "above_1e-4": int((errs32 > 1e-4).sum())},
+            plan_dose_rel_l2=float(np.linalg.norm((dlf32 - d64) @ np.ones(len(plan)))
+                                   / np.linalg.norm(d64 @ np.ones(len(plan))))),
         "fp64_one_at_a_time": rate(t64),
         "speedup_mixed_batched_vs_fp64_one_at_a_time": t64 / tlmx,
         "speedup_from_batching_fp64": t64 / tl64,

@@ -121,3 +121,25 @@ def test_lockstep_refuses_unbatched_contexts(small_plan):
     finally:
         for x in (a, b, c):
             x.close()
+
+
+@pytest.mark.parametrize("precision", [_lib.PRECISION_MIXED, _lib.PRECISION_FP64])
+def test_lockstep_tuned_tiles_equal_streams(precision):
+    """Eight beamlets of the 128 x 128 P_15 plan, large enough for the sweep
+    plan tuner at creation and for the batch's own tile tuner (strip heights,
+    non-temporal streaming, tiles per wavefront): the tuned batch equals the
+    one-stream-per-beamlet batch bit for bit."""
+    plan = beamlet_plan(n_beamlets=8)
+    print(f"\n[case] lockstep vs streams, 8 beamlets 128x128 P_15, precision {precision}", flush=True)
+    problems = [(P, INTEGRATOR_BUG) for _, P in plan]
+    status, stats, got = solve_lockstep(problems, 0, precision)
+    assert np.all(status == 0)
+    print(f"[lockstep tuned] tiles {stats['tiles']}", flush=True)
+    ctxs = [_lib.Context(P, INTEGRATOR_BUG, precision=precision) for P, _ in problems]
+    try:
+        _lib.solve_batch(ctxs, 8)
+        for b, c in enumerate(ctxs):
+            assert np.array_equal(got[b]["dose"], c.state()["dose"]), b
+    finally:
+        for c in ctxs:
+            c.close()

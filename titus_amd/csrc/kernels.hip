@@ -142,6 +142,10 @@ __device__ __forceinline__ void d_sweep(Blk blk,
     const double* __restrict__ omega, T dx, T dy, const double* __restrict__ rho, T dE, T denom,
     const unsigned char* __restrict__ restore, const double* __restrict__ m0,
     double* __restrict__ u0p, int ntj, int nsb) {
+  // every product and sum rounded on its own (no fused multiply-adds): the
+  // instantiations for each strip height and streaming mode then compute
+  // the same bits, so a batch may choose its own strip height
+#pragma clang fp contract(off)
   const int C = nx * ny;
   const int L = xcd_block(blk.x, blk.nx);
   const int jt = L % ntj, sb = (L / ntj) % nsb, gq = L / (ntj * nsb);
