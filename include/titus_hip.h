@@ -288,6 +288,18 @@ int titus_adjoint(struct titus_ctx* ctx, const double* dose_bar, double* grad_de
 int titus_adjoint_batch(struct titus_ctx** ctxs, int32_t n, int32_t nthreads,
                         const double* const* dose_bars, double* const* grads, int32_t* status);
 
+/* The same gradients with shared launches: each context's set-up (saved
+ * forward state, seed) and wrap-up (restored state, gradient) run on its own
+ * stream; the recomputation of its checkpoint segments and the reverse sweep
+ * run in a lockstep batch (as titus_solve_batch_lockstep: `width` at once in
+ * `groups` groups, every kernel one launch per group), bitwise the gradients
+ * of titus_adjoint.  A forward solve with checkpoints may itself run in
+ * titus_solve_batch_lockstep.  No reference counterpart (the reference has
+ * no gradients). */
+int titus_adjoint_batch_lockstep(struct titus_ctx** ctxs, int32_t n, int32_t width, int32_t groups,
+                                 const double* const* dose_bars, double* const* grads,
+                                 int32_t* status);
+
 void titus_destroy(struct titus_ctx* ctx);
 
 /* ---- full-rank P_N solver ---------------------------------------------------

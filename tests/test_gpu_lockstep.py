@@ -7,6 +7,8 @@ grids (solves end at different steps, so the group shrinks and, with a
 width below the batch size, takes the next context), FP64 and mixed forms,
 and the rank-adaptive, fixed-rank and parallel integrators side by side.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -104,16 +106,14 @@ def test_lockstep_matches_oracle(small_plan):
 
 
 def test_lockstep_refuses_unbatched_contexts(small_plan):
-    """3D grids, checkpointed contexts and the RK4 integrator are refused
-    before any solve runs."""
+    """3D grids and the limited flux are refused before any solve runs."""
     from titus_amd.problem3d import build_problem3d, proton_cube
     P3 = build_problem3d(proton_cube(n=8, N=3, r=4))
     P = small_plan[0][1]
     a = _lib.Context(P, INTEGRATOR_BUG)
     b = _lib.Context(P3, INTEGRATOR_FIXED)
-    c = _lib.Context(cases.small_fixed(n=24, N=5, r=6), INTEGRATOR_FIXED)
+    c = _lib.Context(dict(cases.small_fixed(n=24, N=5, r=6), flux_order=2), INTEGRATOR_FIXED)
     try:
-        c.set_checkpoints(4)
         for bad in (b, c):
             with pytest.raises(RuntimeError):
                 _lib.solve_batch_lockstep([a, bad], 0)
@@ -121,6 +121,53 @@ def test_lockstep_refuses_unbatched_contexts(small_plan):
     finally:
         for x in (a, b, c):
             x.close()
+
+
+def test_lockstep_forward_and_adjoint_equal_alone():
+    """Checkpointed fixed-rank solves and their density gradients with shared
+    launches equal the same solves and adjoints run alone, bit for bit; the
+    forward state is back afterwards."""
+    problems = [cases.small_fixed(n=24, N=5, r=6), cases.small_fixed(n=32, N=3, r=4),
+                cases.small_fixed(n=24, N=5, r=8, angular=1.0), cases.small_fixed(n=28, N=5, r=5)]
+    rng = np.random.default_rng(5)
+    bars = [rng.standard_normal(P["C"]) for P in problems]
+    print("\n[case] lockstep forward with checkpoints and adjoint, 4 fixed-rank solves", flush=True)
+
+    def alone(P, bar):
+        c = _lib.Context(P, INTEGRATOR_FIXED)
+        try:
+            c.set_checkpoints(7)
+            c.solve(0)
+            d = c.state()["dose"]
+            return d, c.adjoint(bar)
+        finally:
+            c.close()
+
+    ctxs = [_lib.Context(P, INTEGRATOR_FIXED) for P in problems]
+    try:
+        for c in ctxs:
+            c.set_checkpoints(7)
+        _lib.solve_batch_lockstep(ctxs, 0, 2)
+        doses = [c.state()["dose"] for c in ctxs]
+        # the grids differ in size: each context takes its own seed buffer
+        lib = _lib.load()
+        n = len(ctxs)
+        bufs = [np.ascontiguousarray(b) for b in bars]
+        outs = [np.zeros(P["C"]) for P in problems]
+        arr = (ctypes.c_void_p * n)(*[c.h for c in ctxs])
+        pb = (_lib._P * n)(*[_lib._dp(b) for b in bufs])
+        pg = (_lib._P * n)(*[_lib._dp(g) for g in outs])
+        status = np.zeros(n, dtype=np.int32)
+        _lib.check(lib.titus_adjoint_batch_lockstep(arr, n, 0, 2, pb, pg, _lib._ip(status)))
+        after = [c.state()["dose"] for c in ctxs]
+    finally:
+        for c in ctxs:
+            c.close()
+    for k, P in enumerate(problems):
+        d, g = alone(P, bars[k])
+        assert np.array_equal(doses[k], d), k
+        assert np.array_equal(outs[k], g), k
+        assert np.array_equal(after[k], d), k
 
 
 @pytest.mark.parametrize("precision", [_lib.PRECISION_MIXED, _lib.PRECISION_FP64])

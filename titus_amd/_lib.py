@@ -31,6 +31,7 @@ EXPORTED = ("titus_create", "titus_get_cfl", "titus_set_analytic_beam", "titus_s
             "titus_progress", "titus_get_state",
             "titus_get_history", "titus_set_timing", "titus_kernel_stats", "titus_get_tuning",
             "titus_get_decisions", "titus_set_checkpoints", "titus_adjoint", "titus_adjoint_batch",
+            "titus_adjoint_batch_lockstep",
             "titus_destroy", "titus_fcs_create", "titus_fcs_step", "titus_fcs_solve",
             "titus_fcs_get_state", "titus_fcs_set_timing", "titus_fcs_kernel_stats",
             "titus_fcs_get_plan", "titus_fcs_destroy", "titus_galerkin", "titus_galerkin_noalloc",
@@ -127,6 +128,9 @@ def load():
                                         ctypes.POINTER(ctypes.c_int64)]
     lib.titus_set_checkpoints.argtypes = [vp, ctypes.c_int32]
     lib.titus_adjoint.argtypes = [vp, _P, _P]
+    lib.titus_adjoint_batch_lockstep.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32,
+                                                 ctypes.c_int32, ctypes.POINTER(_P), ctypes.POINTER(_P),
+                                                 _PI]
     lib.titus_adjoint_batch.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32,
                                         ctypes.POINTER(_P), ctypes.POINTER(_P), _PI]
     lib.titus_destroy.argtypes = [vp]
@@ -153,7 +157,8 @@ def load():
                  "titus_progress", "titus_get_state", "titus_get_history", "titus_set_timing",
                  "titus_kernel_stats", "titus_get_tuning", "titus_get_decisions",
                  "titus_set_checkpoints",
-                 "titus_adjoint", "titus_adjoint_batch", "titus_fcs_create", "titus_fcs_step",
+                 "titus_adjoint", "titus_adjoint_batch", "titus_adjoint_batch_lockstep",
+                 "titus_fcs_create", "titus_fcs_step",
                  "titus_fcs_solve", "titus_fcs_get_state", "titus_fcs_set_timing",
                  "titus_fcs_kernel_stats", "titus_fcs_get_plan"):
         getattr(lib, name).restype = ctypes.c_int
@@ -296,9 +301,13 @@ def solve_batch_lockstep(contexts, width=0, groups=1):
                               for t in tiles.reshape(g, 4)]}
 
 
-def adjoint_batch(contexts, dose_bars, nthreads=0):
+def adjoint_batch(contexts, dose_bars, nthreads=0, lockstep=False, groups=4):
     """titus_adjoint over many contexts concurrently; dose_bars is C x n
-    (column i seeds context i).  Returns the density gradients, C x n."""
+    (column i seeds context i).  Returns the density gradients, C x n.
+    ``lockstep``: shared launches (titus_adjoint_batch_lockstep, ``nthreads``
+    > 0 limiting how many run at once), else one stream per context over
+    ``nthreads`` host workers (titus_adjoint_batch); the same gradients bit
+    for bit."""
     lib = load()
     n = len(contexts)
     bars = [np.ascontiguousarray(dose_bars[:, i], dtype=np.float64) for i in range(n)]
@@ -307,7 +316,10 @@ def adjoint_batch(contexts, dose_bars, nthreads=0):
     pb = (_P * n)(*[_dp(b) for b in bars])
     pg = (_P * n)(*[_dp(g) for g in grads])
     status = np.zeros(n, dtype=np.int32)
-    check(lib.titus_adjoint_batch(arr, n, nthreads, pb, pg, _ip(status)))
+    if lockstep:
+        check(lib.titus_adjoint_batch_lockstep(arr, n, nthreads, groups, pb, pg, _ip(status)))
+    else:
+        check(lib.titus_adjoint_batch(arr, n, nthreads, pb, pg, _ip(status)))
     return np.stack(grads, axis=1) if n else np.zeros((0, 0))
 
 

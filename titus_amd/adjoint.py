@@ -36,15 +36,21 @@ def piecewise_target(n_cells, ny, blocks=4, seed=0, scale=1.0):
     return levels[bi[:, None], bj[None, :]].reshape(-1)
 
 
-def solve_with_checkpoints(problems, every=16, nthreads=0, device=0):
+def solve_with_checkpoints(problems, every=16, nthreads=0, device=0, lockstep=True, groups=4):
     """FP64 fixed-rank solves of every problem dict, checkpointed for the
-    adjoint.  Returns (contexts, doses C x B); the caller closes the contexts."""
+    adjoint: with shared launches (``lockstep``; ``nthreads`` > 0 limits how
+    many run at once) or one stream per context over ``nthreads`` host
+    workers.  Returns (contexts, doses C x B); the caller closes the
+    contexts."""
     ctxs = [_lib.Context(P, P["integrator"], precision=_lib.PRECISION_FP64, device=device)
             for P in problems]
     try:
         for c in ctxs:
             c.set_checkpoints(every)
-        _lib.solve_batch(ctxs, nthreads)
+        if lockstep:
+            _lib.solve_batch_lockstep(ctxs, nthreads, groups)
+        else:
+            _lib.solve_batch(ctxs, nthreads)
         doses = np.stack([c.state()["dose"] for c in ctxs], axis=1)
     except Exception:
         for c in ctxs:
@@ -53,19 +59,22 @@ def solve_with_checkpoints(problems, every=16, nthreads=0, device=0):
     return ctxs, doses
 
 
-def plan_objective_rrule(problems, weights, scale, target, every=16, nthreads=0, device=0):
+def plan_objective_rrule(problems, weights, scale, target, every=16, nthreads=0, device=0,
+                         lockstep=True, groups=4):
     """J and its pullback for the plan objective above.
 
     ``problems``: fixed-rank problem dicts (``beamlet_plan(...,
     integrator=INTEGRATOR_FIXED)``) sharing one phantom; their "density" is
     taken as rho0.  Returns (J, pullback) with pullback(Jbar) -> (wbar, sbar).
-    The contexts stay alive until the pullback has run.
+    The contexts stay alive until the pullback has run.  ``lockstep``: the
+    forward solves and the adjoints with shared launches (the same values
+    bit for bit as one stream per beamlet).
     """
     w = np.asarray(weights, dtype=np.float64)
     s = np.asarray(scale, dtype=np.float64)
     rho0 = np.asarray(problems[0]["density"], dtype=np.float64)
     scaled = [dict(P, density=rho0 * s) for P in problems]
-    ctxs, doses = solve_with_checkpoints(scaled, every, nthreads, device)
+    ctxs, doses = solve_with_checkpoints(scaled, every, nthreads, device, lockstep, groups)
     resid = doses @ w - np.asarray(target, dtype=np.float64)
     J = float(resid @ resid)
 
@@ -75,7 +84,7 @@ def plan_objective_rrule(problems, weights, scale, target, every=16, nthreads=0,
         try:
             wbar = Jbar * 2.0 * (doses.T @ resid)
             bars = Jbar * 2.0 * resid[:, None] * w[None, :]
-            grads = _lib.adjoint_batch(ctxs, bars, nthreads)        # d/d rho, C x B
+            grads = _lib.adjoint_batch(ctxs, bars, nthreads, lockstep, groups)   # d/d rho, C x B
             sbar = rho0 * grads.sum(axis=1)
             if per_beamlet:
                 return wbar, sbar, rho0[:, None] * grads, [c.decisions()[0] for c in ctxs]
@@ -89,8 +98,10 @@ def plan_objective_rrule(problems, weights, scale, target, every=16, nthreads=0,
     return J, pullback
 
 
-def plan_objective_grad(problems, weights, scale, target, every=16, nthreads=0, device=0):
+def plan_objective_grad(problems, weights, scale, target, every=16, nthreads=0, device=0,
+                        lockstep=True, groups=4):
     """(J, dJ/dw, dJ/ds) in one call."""
-    J, pb = plan_objective_rrule(problems, weights, scale, target, every, nthreads, device)
+    J, pb = plan_objective_rrule(problems, weights, scale, target, every, nthreads, device,
+                                 lockstep, groups)
     wbar, sbar = pb(1.0)
     return J, wbar, sbar

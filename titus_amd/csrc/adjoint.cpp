@@ -88,8 +88,8 @@ void adjoint_checkpoint(titus_ctx& x, int s) {
   c.S = x.S;
   c.W = x.W;
   c.saved = true;
-  HIPCHK(hipMemcpyAsync(c.X, x.X, sizeof(double) * (size_t)x.C * x.rank, hipMemcpyDeviceToDevice, x.st));
-  HIPCHK(hipMemcpyAsync(c.psi, x.psi, sizeof(double) * (size_t)x.C * x.nq, hipMemcpyDeviceToDevice, x.st));
+  x.dcopy(c.X, x.X, sizeof(double) * (size_t)x.C * x.rank);
+  x.dcopy(c.psi, x.psi, sizeof(double) * (size_t)x.C * x.nq);
 }
 
 namespace {
@@ -175,7 +175,8 @@ struct Reverse {
   void orth_bar_dev(const SpatialOrthTape& o, const double* X1, const double* X1bar, double* Kout) {
     const int k = o.k, p = o.p;
     if (p == 0) {
-      HIPCHK(hipMemsetAsync(Kout, 0, sizeof(double) * (size_t)C * k, x.st));
+      // a zero block by a zero-coefficient product (no memset on the stream)
+      gemm(X1bar, r, HMat(r, k), 0.0, 0.0, Kout);
       return;
     }
     gemm(X1bar, r, titus::transpose(o.Tp), 1.0, 0.0, a.Ub);            // Z = X1bar Tp^T
@@ -353,7 +354,20 @@ struct Reverse {
   }
 };
 
-void run_adjoint(titus_ctx& x, const double* dose_bar, double* grad) {
+// The forward state an adjoint overwrites and puts back.
+struct AdjointRun {
+  int done = 0;
+  HMat S_end, W_end;
+  uint64_t hash_was = 0;
+  int64_t decisions_was = 0;
+  bool timing_was = false;
+  size_t cu = 0;
+  bool begun = false;
+};
+
+// Checks, buffers, the saved forward state, the seed and zeroed adjoint
+// blocks, on the context's own stream (before any shared launches).
+void adjoint_begin(titus_ctx& x, const double* dose_bar, AdjointRun& run) {
   if (!x.adj) throw TitusError(TITUS_E_STATE, "adjoint: checkpoints were not enabled before the solve");
   AdjointStore& a = *x.adj;
   if (x.steps_done < 1) throw TitusError(TITUS_E_STATE, "adjoint: nothing solved yet");
@@ -389,83 +403,112 @@ void run_adjoint(titus_ctx& x, const double* dose_bar, double* grad) {
   }
   hipStream_t st = x.st;
   // save the forward state the recomputation overwrites
-  const int done = x.steps_done;
-  const HMat S_end = x.S, W_end = x.W;
+  run.done = x.steps_done;
+  run.S_end = x.S;
+  run.W_end = x.W;
   HIPCHK(hipMemcpyAsync(a.Xsave, x.X, sizeof(double) * cr, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(a.psisave, x.psi, sizeof(double) * cq, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(a.dosesave, x.dose, sizeof(double) * C, hipMemcpyDeviceToDevice, st));
-  const size_t cu = (size_t)x.u0_groups * C;
-  HIPCHK(hipMemcpyAsync(a.u0save, x.u0p, sizeof(double) * cu, hipMemcpyDeviceToDevice, st));
+  run.cu = (size_t)x.u0_groups * C;
+  HIPCHK(hipMemcpyAsync(a.u0save, x.u0p, sizeof(double) * run.cu, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(a.Db, dose_bar, sizeof(double) * C, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(a.Xb, 0, sizeof(double) * cr, st));
   HIPCHK(hipMemsetAsync(a.psib, 0, sizeof(double) * cq, st));
   HIPCHK(hipMemsetAsync(a.ybar, 0, sizeof(double) * C, st));
-  const uint64_t hash_was = x.decision_hash;
-  const int64_t decisions_was = x.decisions;
-  const bool timing_was = x.timing.enabled;
+  run.hash_was = x.decision_hash;
+  run.decisions_was = x.decisions;
+  run.timing_was = x.timing.enabled;
   x.timing.enabled = false;
-  // puts the forward state back as it was before the call: device blocks
-  // from their copies, host factors, counters; runs on success and failure
-  auto restore = [&] {
-    x.tape = nullptr;
-    x.timing.enabled = timing_was;
-    x.decision_hash = hash_was;
-    x.decisions = decisions_was;
-    HIPCHK(hipMemcpyAsync(x.X, a.Xsave, sizeof(double) * cr, hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipMemcpyAsync(x.psi, a.psisave, sizeof(double) * cq, hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipMemcpyAsync(x.dose, a.dosesave, sizeof(double) * C, hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipMemcpyAsync(x.u0p, a.u0save, sizeof(double) * cu, hipMemcpyDeviceToDevice, st));
-    x.S = S_end;
-    x.W = W_end;
-    x.steps_done = done;
-  };
+  run.begun = true;
+}
+
+// The segments backwards: restore a checkpoint, recompute the segment with
+// the tape on, run the taped steps in reverse.  Its launches go through the
+// context (alone, or recorded in a lockstep group).
+void adjoint_segments(titus_ctx& x, const AdjointRun& run) {
+  AdjointStore& a = *x.adj;
+  const size_t cr = (size_t)x.C * x.rank, cq = (size_t)x.C * x.nq;
+  const int done = run.done;
   Reverse rev(x, a);
-  try {
-    for (int ci = (int)a.cps.size() - 1; ci >= 0; --ci) {
-      const Checkpoint& cp = a.cps[ci];
-      if (cp.s > done) continue;
-      if (!cp.saved) throw TitusError(TITUS_E_STATE, "adjoint: missing checkpoint");
-      const int s1 = (ci + 1 < (int)a.cps.size()) ? std::min(done, a.cps[ci + 1].s - 1) : done;
-      if (s1 - cp.s + 1 > (int)a.tapes.size()) throw TitusError(TITUS_E_STATE, "adjoint: checkpoint gap");
-      // restore and recompute the segment with the tape on
-      HIPCHK(hipMemcpyAsync(x.X, cp.X, sizeof(double) * cr, hipMemcpyDeviceToDevice, st));
-      HIPCHK(hipMemcpyAsync(x.psi, cp.psi, sizeof(double) * cq, hipMemcpyDeviceToDevice, st));
-      x.S = cp.S;
-      x.W = cp.W;
-      for (int s = cp.s; s <= s1; ++s) {
-        x.tape = &a.tapes[s - cp.s];
-        x.step_fixed(s);
-        x.tape = nullptr;
-      }
-      for (int s = s1; s >= cp.s; --s) {
-        rev.step(a.tapes[s - cp.s]);
-        // the host-side adjoint coefficients carry every reverse step
-        for (double v : rev.Sb.a)
-          if (!std::isfinite(v)) throw TitusError(TITUS_E_NONFINITE, "adjoint: non-finite coefficients");
-        for (double v : rev.Wb.a)
-          if (!std::isfinite(v)) throw TitusError(TITUS_E_NONFINITE, "adjoint: non-finite coefficients");
-      }
+  for (int ci = (int)a.cps.size() - 1; ci >= 0; --ci) {
+    const Checkpoint& cp = a.cps[ci];
+    if (cp.s > done) continue;
+    if (!cp.saved) throw TitusError(TITUS_E_STATE, "adjoint: missing checkpoint");
+    const int s1 = (ci + 1 < (int)a.cps.size()) ? std::min(done, a.cps[ci + 1].s - 1) : done;
+    if (s1 - cp.s + 1 > (int)a.tapes.size()) throw TitusError(TITUS_E_STATE, "adjoint: checkpoint gap");
+    // restore and recompute the segment with the tape on
+    x.dcopy(x.X, cp.X, sizeof(double) * cr);
+    x.dcopy(x.psi, cp.psi, sizeof(double) * cq);
+    x.S = cp.S;
+    x.W = cp.W;
+    for (int s = cp.s; s <= s1; ++s) {
+      x.tape = &a.tapes[s - cp.s];
+      x.step_fixed(s);
+      x.tape = nullptr;
     }
-  } catch (...) {
-    // a failed HIP call may leave the stream unusable: the restore is then
-    // best effort and the first error is the one reported
-    try {
-      restore();
-      (void)hipStreamSynchronize(st);
-    } catch (...) {
+    for (int s = s1; s >= cp.s; --s) {
+      rev.step(a.tapes[s - cp.s]);
+      // the host-side adjoint coefficients carry every reverse step
+      for (double v : rev.Sb.a)
+        if (!std::isfinite(v)) throw TitusError(TITUS_E_NONFINITE, "adjoint: non-finite coefficients");
+      for (double v : rev.Wb.a)
+        if (!std::isfinite(v)) throw TitusError(TITUS_E_NONFINITE, "adjoint: non-finite coefficients");
     }
-    throw;
   }
-  // d/drho = -y^2 d/dy
-  std::vector<double> yb(C), rho(C);
-  HIPCHK(hipMemcpyAsync(yb.data(), a.ybar, sizeof(double) * C, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(rho.data(), x.rho, sizeof(double) * C, hipMemcpyDeviceToHost, st));
-  restore();
+}
+
+// Puts the forward state back as it was before the adjoint (device blocks
+// from their copies, host factors, counters) and, when `grad` is not null,
+// writes d/drho = -y^2 d/dy.  On the context's own stream.
+void adjoint_end(titus_ctx& x, const AdjointRun& run, double* grad) {
+  if (!run.begun) return;
+  AdjointStore& a = *x.adj;
+  const int C = x.C;
+  const size_t cr = (size_t)C * x.rank, cq = (size_t)C * x.nq;
+  hipStream_t st = x.st;
+  x.tape = nullptr;
+  x.timing.enabled = run.timing_was;
+  x.decision_hash = run.hash_was;
+  x.decisions = run.decisions_was;
+  std::vector<double> yb(grad ? C : 0), rho(grad ? C : 0);
+  if (grad) {
+    HIPCHK(hipMemcpyAsync(yb.data(), a.ybar, sizeof(double) * C, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(rho.data(), x.rho, sizeof(double) * C, hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(hipMemcpyAsync(x.X, a.Xsave, sizeof(double) * cr, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(x.psi, a.psisave, sizeof(double) * cq, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(x.dose, a.dosesave, sizeof(double) * C, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(x.u0p, a.u0save, sizeof(double) * run.cu, hipMemcpyDeviceToDevice, st));
+  x.S = run.S_end;
+  x.W = run.W_end;
+  x.steps_done = run.done;
   HIPCHK(hipStreamSynchronize(st));
+  if (!grad) return;
   for (int c = 0; c < C; ++c) {
     if (!std::isfinite(yb[c])) throw TitusError(TITUS_E_NONFINITE, "adjoint: non-finite gradient");
     grad[c] = -yb[c] / (rho[c] * rho[c]);
   }
+}
+
+// a failed HIP call may leave the stream unusable: the restore is then best
+// effort and the first error is the one reported
+void adjoint_end_after_failure(titus_ctx& x, const AdjointRun& run) {
+  try {
+    adjoint_end(x, run, nullptr);
+  } catch (...) {
+  }
+}
+
+void run_adjoint(titus_ctx& x, const double* dose_bar, double* grad) {
+  AdjointRun run;
+  try {
+    adjoint_begin(x, dose_bar, run);
+    adjoint_segments(x, run);
+  } catch (...) {
+    adjoint_end_after_failure(x, run);
+    throw;
+  }
+  adjoint_end(x, run, grad);
 }
 
 }  // namespace
@@ -525,6 +568,78 @@ int titus_adjoint_batch(titus_ctx** ctxs, int32_t n, int32_t nthreads, const dou
   for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
   worker();
   for (auto& th : pool) th.join();
+  int first = TITUS_OK;
+  for (int i = 0; i < n; ++i) {
+    if (status) status[i] = codes[i];
+    if (first == TITUS_OK && codes[i] != TITUS_OK) {
+      first = codes[i];
+      g_last_error = "adjoint " + std::to_string(i) + ": " + msgs[i];
+    }
+  }
+  return first;
+}
+
+int titus_adjoint_batch_lockstep(titus_ctx** ctxs, int32_t n, int32_t width, int32_t groups,
+                                 const double* const* dose_bars, double* const* grads,
+                                 int32_t* status) {
+  if (!ctxs || n < 0 || !dose_bars || !grads) {
+    g_last_error = "bad argument";
+    return TITUS_E_ARG;
+  }
+  if (batch_has_duplicates(ctxs, n)) {
+    g_last_error = "a context appears twice in the batch";
+    return TITUS_E_ARG;
+  }
+  for (int i = 0; i < n; ++i)
+    if (!ctxs[i] || ctxs[i]->device != ctxs[0]->device || !dose_bars[i] || !grads[i]) {
+      g_last_error = "adjoint lockstep batch, context " + std::to_string(i) +
+                     ": null argument or contexts on different devices";
+      return TITUS_E_ARG;
+    }
+  std::vector<AdjointRun> runs(n);
+  std::vector<int> codes(n, TITUS_OK);
+  std::vector<std::string> msgs(n);
+  // set-up on each context's own stream (synchronised before the shared launches)
+  for (int i = 0; i < n; ++i) {
+    codes[i] = guarded([&] {
+      HIPCHK(hipSetDevice(ctxs[i]->device));
+      adjoint_begin(*ctxs[i], dose_bars[i], runs[i]);
+    });
+    if (codes[i] != TITUS_OK) msgs[i] = g_last_error;
+  }
+  std::vector<titus_ctx*> live;
+  std::vector<int> idx;
+  for (int i = 0; i < n; ++i)
+    if (codes[i] == TITUS_OK) {
+      live.push_back(ctxs[i]);
+      idx.push_back(i);
+    }
+  std::vector<int32_t> st(live.size(), TITUS_OK);
+  std::vector<std::string> lmsg(live.size());
+  const int rc = lockstep_run(live.data(), (int)live.size(), width, groups, st.data(), nullptr, nullptr,
+                              [&](titus_ctx& c, int j) {
+                                try {
+                                  adjoint_segments(c, runs[idx[j]]);
+                                } catch (const std::exception& e) {
+                                  lmsg[j] = e.what();
+                                  throw;
+                                }
+                              });
+  (void)rc;
+  for (size_t j = 0; j < live.size(); ++j)
+    if (st[j] != TITUS_OK) {
+      codes[idx[j]] = st[j];
+      msgs[idx[j]] = lmsg[j].empty() ? "lockstep batch failed" : lmsg[j];
+    }
+  // the forward states back and the gradients, on each context's stream
+  for (int i = 0; i < n; ++i) {
+    if (codes[i] == TITUS_OK) {
+      codes[i] = guarded([&] { adjoint_end(*ctxs[i], runs[i], grads[i]); });
+      if (codes[i] != TITUS_OK) msgs[i] = g_last_error;
+    } else {
+      adjoint_end_after_failure(*ctxs[i], runs[i]);
+    }
+  }
   int first = TITUS_OK;
   for (int i = 0; i < n; ++i) {
     if (status) status[i] = codes[i];

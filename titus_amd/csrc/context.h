@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -201,6 +202,12 @@ struct LockstepGroup;
 // solver.cpp: the calling context's sync point in its lockstep group (flushes
 // the recorded launches of every context of the group once all have arrived)
 void lockstep_sync(titus_ctx& x);
+// solver.cpp: run work(ctx, i) for every context of a lockstep batch (the
+// validated contexts of titus_solve_batch_lockstep's contract): up to
+// `width` at once in `groups` groups, their launches shared; per-context
+// codes in status, counters in stats (6), tuned tiles in tiles (4 per group)
+int lockstep_run(titus_ctx** ctxs, int n, int width, int groups, int32_t* status, int64_t* stats,
+                 int32_t* tiles, const std::function<void(titus_ctx&, int)>& work);
 // adjoint.cpp: checkpoint of the state before step s (when enabled)
 void adjoint_checkpoint(titus_ctx& x, int s);
 

@@ -332,22 +332,22 @@ __global__ __launch_bounds__(kBlock) void k_sweep_lim(
 // [d interior] X[d,:] Zd[:,b], with no 1/rho (the adjoint applies it to the
 // result); the centre term counts when c itself is interior.
 template <class TC, int NB, int ND, bool TR, class Src>
-__global__ __launch_bounds__(kBlock) void k_stream_k(
+__device__ __forceinline__ void d_stream_k(Blk blk, 
     const double* __restrict__ X, int ldx, int r, int nx, int ny, int nz,
-    const double* __restrict__ rho, const Src Z, int n, double dE,
+    const double* __restrict__ rho, const Src& Z, int n, double dE,
     double* __restrict__ K, int ldk) {
   constexpr int NZB = 2 * ND + 2;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   TC* Zs = reinterpret_cast<TC*>(lds_raw);
   const int C = nx * ny * nz;
-  const int b0 = blockIdx.y * NB;
+  const int b0 = blk.y * NB;
   const int nbv = min(NB, n - b0);
   for (int e = threadIdx.x; e < r * NZB * NB; e += blockDim.x) {
     const int k = e % NB, d = (e / NB) % NZB, a = e / (NZB * NB);
     Zs[e] = k < nbv ? (TC)Z[((size_t)d * n + b0 + k) * r + a] : TC(0);
   }
   __syncthreads();
-  const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int c = xcd_block(blk.x, blk.nx) * blockDim.x + threadIdx.x;
   if (c >= C) return;
   const int syz = ny * nz;
   const int i = c / syz, rem = c - (c / syz) * syz;
@@ -1518,14 +1518,26 @@ void launch_sweep(hipStream_t st, const Grid& g, const void* pin, void* pout, DT
 }
 
 namespace {
+template <class TC, int NB, int ND, bool TR, class Src>
+struct StreamKBody {
+  struct P {
+    const double* X; int ldx, r, nx, ny, nz; const double* rho; Src Z; int n; double dE;
+    double* K; int ldk;
+  };
+  static constexpr int kThreads = kBlock;
+  static constexpr bool kBatch = !std::is_same<Src, ArgSrc>::value;
+  __device__ static void run(const P& p, Blk b) {
+    d_stream_k<TC, NB, ND, TR, Src>(b, p.X, p.ldx, p.r, p.nx, p.ny, p.nz, p.rho, p.Z, p.n, p.dE,
+                                    p.K, p.ldk);
+  }
+};
 template <class TC, int NB, int ND, bool TR = false, class Src = GlobalSrc>
 void stream_k_nd(hipStream_t st, const Grid& g, const double* X, int ldx, int r, const Src& Z,
                  int n, double dE, double* K, int ldk) {
-  no_batch("stream_K (VALU form)");
   const dim3 grid(blocks_for(g.C), (n + NB - 1) / NB);
   const size_t lds = (size_t)r * (2 * ND + 2) * NB * sizeof(TC);
-  hipLaunchKernelGGL((k_stream_k<TC, NB, ND, TR, Src>), grid, dim3(kBlock), lds, st, X, ldx, r,
-                     g.nx, g.ny, g.nz, g.rho, Z, n, dE, K, ldk);
+  using Body = StreamKBody<TC, NB, ND, TR, Src>;
+  launch_body<Body>(st, grid, lds, typename Body::P{X, ldx, r, g.nx, g.ny, g.nz, g.rho, Z, n, dE, K, ldk});
 }
 template <class TC, int NB, class Src>
 void stream_k_nb(hipStream_t st, const Grid& g, const double* X, int ldx, int r, const Src& Z,
@@ -1789,11 +1801,11 @@ void launch_stream_kt(hipStream_t st, const Grid& g, const double* X, int ldx, i
 namespace {
 
 // out[c,:] += alpha y_c T[c,:];  ybar[c] += alpha sum_k V[c,k] T[c,k] (V may be null)
-__global__ void k_adj_combine(int C, int n, double* __restrict__ out, int ldo,
+__device__ __forceinline__ void d_adj_combine(Blk blk, int C, int n, double* __restrict__ out, int ldo,
                               const double* __restrict__ T, int ldt, const double* __restrict__ V,
                               int ldv, const double* __restrict__ rho, double alpha,
                               double* __restrict__ ybar) {
-  const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int c = xcd_block(blk.x, blk.nx) * blockDim.x + threadIdx.x;
   if (c >= C) return;
   const double y = 1.0 / rho[c];
   double acc = 0.0;
@@ -1806,15 +1818,15 @@ __global__ void k_adj_combine(int C, int n, double* __restrict__ out, int ldo,
 }
 
 // y += alpha x over n contiguous values
-__global__ void k_axpy(size_t n, double alpha, const double* __restrict__ x, double* __restrict__ y) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void d_axpy(Blk blk, size_t n, double alpha, const double* __restrict__ x, double* __restrict__ y) {
+  const size_t t = (size_t)blk.x * blockDim.x + threadIdx.x;
   if (t < n) y[t] += alpha * x[t];
 }
 
 // A[rows[t], :] = 0
-__global__ void k_zero_rows(double* __restrict__ A, int lda, int ncols, const int* __restrict__ rows,
+__device__ __forceinline__ void d_zero_rows(Blk blk, double* __restrict__ A, int lda, int ncols, const int* __restrict__ rows,
                             int nrows) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t t = (size_t)blk.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)nrows * ncols) return;
   const int r = (int)(t % nrows), k = (int)(t / nrows);
   A[(size_t)k * lda + rows[r]] = 0.0;
@@ -1822,13 +1834,13 @@ __global__ void k_zero_rows(double* __restrict__ A, int lda, int ncols, const in
 
 // Transpose of dose[c] += a y_c (X[c,:] v + psi[c,:] m0) for the seed Db:
 // g_c = a y_c Db_c; Xb += g v^T; psib += g m0^T; ybar += a Db (X v + psi m0).
-__global__ void k_dose_adj(int C, const double* __restrict__ Db, double a,
+__device__ __forceinline__ void d_dose_adj(Blk blk, int C, const double* __restrict__ Db, double a,
                            const double* __restrict__ rho, const double* __restrict__ X, int ldx,
                            int r, const double* __restrict__ v, const double* __restrict__ psi,
                            int nq, const double* __restrict__ m0, double* __restrict__ Xb,
                            double* __restrict__ psib, double* __restrict__ ybar,
                            double* __restrict__ gout) {
-  const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int c = xcd_block(blk.x, blk.nx) * blockDim.x + threadIdx.x;
   if (c >= C) return;
   const double db = Db[c];
   const double g = a * db / rho[c];
@@ -1852,13 +1864,13 @@ __global__ void k_dose_adj(int C, const double* __restrict__ Db, double a,
 //        (Omega_x <  0) (ox/dx) ([c-ex int] G'_{c-ex} - [c int] G'_c)
 // and the same along j with Omega_z / dy.  out = (G' - dE y t) / denom (+ G
 // at the kept entries); ybar -= dE / denom sum_q psi_a t.
-__global__ void k_sweep_adj(int nx, int ny, int nq, const double* __restrict__ G,
+__device__ __forceinline__ void d_sweep_adj(Blk blk, int nx, int ny, int nq, const double* __restrict__ G,
                             const double* __restrict__ psi_a, const double* __restrict__ omega,
                             double dx, double dy, const double* __restrict__ rho, double dE,
                             double denom, const unsigned char* __restrict__ restore,
                             double* __restrict__ out, double* __restrict__ ybar) {
   const int C = nx * ny;
-  const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int c = xcd_block(blk.x, blk.nx) * blockDim.x + threadIdx.x;
   if (c >= C) return;
   const int i = c / ny, j = c - (c / ny) * ny;
   auto inner = [&](int ii, int jj) { return ii >= 1 && ii <= nx - 2 && jj >= 1 && jj <= ny - 2; };
@@ -1885,30 +1897,75 @@ __global__ void k_sweep_adj(int nx, int ny, int nq, const double* __restrict__ G
   ybar[c] -= dE / denom * yb;
 }
 
+struct AdjCombineBody {
+  struct P {
+    int C, n; double* out; int ldo; const double* T; int ldt; const double* V; int ldv;
+    const double* rho; double alpha; double* ybar;
+  };
+  static constexpr int kThreads = kBlock;
+  static constexpr bool kBatch = true;
+  __device__ static void run(const P& p, Blk b) {
+    d_adj_combine(b, p.C, p.n, p.out, p.ldo, p.T, p.ldt, p.V, p.ldv, p.rho, p.alpha, p.ybar);
+  }
+};
+struct AxpyBody {
+  struct P { size_t n; double alpha; const double* x; double* y; };
+  static constexpr int kThreads = kBlock;
+  static constexpr bool kBatch = true;
+  __device__ static void run(const P& p, Blk b) { d_axpy(b, p.n, p.alpha, p.x, p.y); }
+};
+struct ZeroRowsBody {
+  struct P { double* A; int lda, ncols; const int* rows; int nrows; };
+  static constexpr int kThreads = kBlock;
+  static constexpr bool kBatch = true;
+  __device__ static void run(const P& p, Blk b) { d_zero_rows(b, p.A, p.lda, p.ncols, p.rows, p.nrows); }
+};
+struct DoseAdjBody {
+  struct P {
+    int C; const double* Db; double a; const double* rho; const double* X; int ldx, r;
+    const double* v; const double* psi; int nq; const double* m0; double* Xb; double* psib;
+    double* ybar; double* gout;
+  };
+  static constexpr int kThreads = kBlock;
+  static constexpr bool kBatch = true;
+  __device__ static void run(const P& p, Blk b) {
+    d_dose_adj(b, p.C, p.Db, p.a, p.rho, p.X, p.ldx, p.r, p.v, p.psi, p.nq, p.m0, p.Xb, p.psib,
+               p.ybar, p.gout);
+  }
+};
+struct SweepAdjBody {
+  struct P {
+    int nx, ny, nq; const double* G; const double* psi_a; const double* omega; double dx, dy;
+    const double* rho; double dE, denom; const unsigned char* restore; double* out; double* ybar;
+  };
+  static constexpr int kThreads = kBlock;
+  static constexpr bool kBatch = true;
+  __device__ static void run(const P& p, Blk b) {
+    d_sweep_adj(b, p.nx, p.ny, p.nq, p.G, p.psi_a, p.omega, p.dx, p.dy, p.rho, p.dE, p.denom,
+                p.restore, p.out, p.ybar);
+  }
+};
+
 }  // namespace
 
 void launch_adj_combine(hipStream_t st, int C, int n, double* out, int ldo, const double* T, int ldt,
                         const double* V, int ldv, const double* rho, double alpha, double* ybar) {
   if (C == 0) return;
-  no_batch("adj_combine");
-  hipLaunchKernelGGL(k_adj_combine, dim3(blocks_for(C)), dim3(kBlock), 0, st, C, n, out, ldo, T,
-                     ldt, V, ldv, rho, alpha, ybar);
+  launch_body<AdjCombineBody>(st, dim3(blocks_for(C)), 0,
+                              {C, n, out, ldo, T, ldt, V, ldv, rho, alpha, ybar});
   check_launch("adj_combine");
 }
 
 void launch_axpy(hipStream_t st, size_t n, double alpha, const double* x, double* y) {
   if (n == 0) return;
-  no_batch("axpy");
-  hipLaunchKernelGGL(k_axpy, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, alpha, x, y);
+  launch_body<AxpyBody>(st, dim3(blocks_for(n)), 0, {n, alpha, x, y});
   check_launch("axpy");
 }
 
 void launch_zero_rows(hipStream_t st, double* A, int lda, int ncols, const int* rows, int nrows) {
   const size_t n = (size_t)nrows * ncols;
   if (n == 0) return;
-  no_batch("zero_rows");
-  hipLaunchKernelGGL(k_zero_rows, dim3(blocks_for(n)), dim3(kBlock), 0, st, A, lda, ncols, rows,
-                     nrows);
+  launch_body<ZeroRowsBody>(st, dim3(blocks_for(n)), 0, {A, lda, ncols, rows, nrows});
   check_launch("zero_rows");
 }
 
@@ -1916,9 +1973,8 @@ void launch_dose_adj(hipStream_t st, const Grid& g, const double* Db, double a, 
                      int ldx, int r, const double* v, const double* psi, int nq, const double* m0,
                      double* Xb, double* psib, double* ybar, double* gout) {
   if (g.C == 0) return;
-  no_batch("dose_adj");
-  hipLaunchKernelGGL(k_dose_adj, dim3(blocks_for(g.C)), dim3(kBlock), 0, st, g.C, Db, a, g.rho, X,
-                     ldx, r, v, psi, nq, m0, Xb, psib, ybar, gout);
+  launch_body<DoseAdjBody>(st, dim3(blocks_for(g.C)), 0,
+                           {g.C, Db, a, g.rho, X, ldx, r, v, psi, nq, m0, Xb, psib, ybar, gout});
   check_launch("dose_adj");
 }
 
@@ -1927,9 +1983,9 @@ void launch_sweep_adj(hipStream_t st, const Grid& g, const double* G, const doub
                       double* out, double* ybar) {
   if (g.ndim() != 2) throw std::runtime_error("sweep adjoint: 2D grids only");
   if (g.C == 0) return;
-  no_batch("sweep_adj");
-  hipLaunchKernelGGL(k_sweep_adj, dim3(blocks_for(g.C)), dim3(kBlock), 0, st, g.nx, g.ny, nq, G,
-                     psi_a, omega, g.dx, g.dy, g.rho, dE, denom, restore, out, ybar);
+  launch_body<SweepAdjBody>(st, dim3(blocks_for(g.C)), 0,
+                            {g.nx, g.ny, nq, G, psi_a, omega, g.dx, g.dy, g.rho, dE, denom, restore,
+                             out, ybar});
   check_launch("sweep_adj");
 }
 

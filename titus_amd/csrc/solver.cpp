@@ -4,6 +4,7 @@
 #include "context.h"
 
 #include <condition_variable>
+#include <functional>
 
 // ---------------------------------------------------------------------------
 // Launch-plan autotuner of the uncollided sweep (the kernel that moves the
@@ -623,32 +624,10 @@ int titus_solve_batch(titus_ctx** ctxs, int32_t n, int32_t nthreads, int32_t* st
   return first;
 }
 
-int titus_solve_batch_lockstep(titus_ctx** ctxs, int32_t n, int32_t width, int32_t groups,
-                               int32_t* status, int64_t* stats, int32_t* tiles) {
-  if (!ctxs || n < 0) {
-    g_last_error = "bad argument";
-    return TITUS_E_ARG;
-  }
-  if (batch_has_duplicates(ctxs, n)) {
-    g_last_error = "a context appears twice in the batch";
-    return TITUS_E_ARG;
-  }
-  for (int i = 0; i < n; ++i) {
-    titus_ctx* c = ctxs[i];
-    std::string why;
-    if (!c) why = "null context";
-    else if (c->device != ctxs[0]->device) why = "contexts on different devices";
-    else if (c->integrator != TITUS_INTEGRATOR_FIXED && c->integrator != TITUS_INTEGRATOR_BUG &&
-             c->integrator != TITUS_INTEGRATOR_PARALLEL)
-      why = "integrator without a batched form (RK4)";
-    else if (c->nd != 2) why = "3D grid (no batched sweep)";
-    else if (c->flux_order != 1) why = "limited flux (no batched sweep)";
-    else if (c->adj) why = "checkpoints set (adjoint recording runs alone)";
-    if (!why.empty()) {
-      g_last_error = "lockstep batch, context " + std::to_string(i) + ": " + why;
-      return TITUS_E_ARG;
-    }
-  }
+} // extern "C"
+
+int lockstep_run(titus_ctx** ctxs, int n, int width, int groups, int32_t* status, int64_t* stats,
+                 int32_t* tiles, const std::function<void(titus_ctx&, int)>& work) {
   if (n == 0) return TITUS_OK;
   const int w = std::max(1, std::min<int>(width > 0 ? width : n, n));
   std::vector<int> codes(n, TITUS_OK);
@@ -705,10 +684,7 @@ int titus_solve_batch_lockstep(titus_ctx** ctxs, int32_t n, int32_t width, int32
       c->group = &grp;
       dev::t_recorder = &c->rec;
       dev::t_tune = &grp.tune;
-      codes[i] = guarded([&] {
-        const int last = c->integrator == TITUS_INTEGRATOR_BUG ? c->ne - 2 : c->ne - 1;
-        for (int s = c->steps_done + 1; s <= last; ++s) c->step(s);
-      });
+      codes[i] = guarded([&] { work(*c, i); });
       if (codes[i] != TITUS_OK) msgs[i] = g_last_error;
       dev::t_recorder = nullptr;
       dev::t_tune = nullptr;
@@ -755,6 +731,41 @@ int titus_solve_batch_lockstep(titus_ctx** ctxs, int32_t n, int32_t width, int32
     }
   }
   return first;
+}
+
+
+extern "C" {
+
+int titus_solve_batch_lockstep(titus_ctx** ctxs, int32_t n, int32_t width, int32_t groups,
+                               int32_t* status, int64_t* stats, int32_t* tiles) {
+  if (!ctxs || n < 0) {
+    g_last_error = "bad argument";
+    return TITUS_E_ARG;
+  }
+  if (batch_has_duplicates(ctxs, n)) {
+    g_last_error = "a context appears twice in the batch";
+    return TITUS_E_ARG;
+  }
+  for (int i = 0; i < n; ++i) {
+    titus_ctx* c = ctxs[i];
+    std::string why;
+    if (!c) why = "null context";
+    else if (c->device != ctxs[0]->device) why = "contexts on different devices";
+    else if (c->integrator != TITUS_INTEGRATOR_FIXED && c->integrator != TITUS_INTEGRATOR_BUG &&
+             c->integrator != TITUS_INTEGRATOR_PARALLEL)
+      why = "integrator without a batched form (RK4)";
+    else if (c->nd != 2) why = "3D grid (no batched sweep)";
+    else if (c->flux_order != 1) why = "limited flux (no batched sweep)";
+    else if (c->tape) why = "adjoint recomputation in progress";
+    if (!why.empty()) {
+      g_last_error = "lockstep batch, context " + std::to_string(i) + ": " + why;
+      return TITUS_E_ARG;
+    }
+  }
+  return lockstep_run(ctxs, n, width, groups, status, stats, tiles, [](titus_ctx& c, int) {
+    const int last = c.integrator == TITUS_INTEGRATOR_BUG ? c.ne - 2 : c.ne - 1;
+    for (int s = c.steps_done + 1; s <= last; ++s) c.step(s);
+  });
 }
 
 int titus_set_analytic_beam(titus_ctx* ctx, const titus_beam_v1* b) {
