@@ -622,7 +622,7 @@ def adjoint_bench(sync, nthreads, n_beamlets=64, every=16, fd_dirs=8, fd_beamlet
             "max_rel_err": max([r["rel_err"] for r in rows if r["rel_err"] is not None] or [None]),
             "beamlets_counted_per_direction": [r["counted"] for r in rows],
         },
-        "form": "forward and adjoint with shared launches (4 groups); difference solves one "
+        "form": "forward and adjoint with shared launches (8 groups); difference solves one "
                 f"stream per beamlet over {nthreads} host threads",
     }
 

@@ -36,7 +36,7 @@ def piecewise_target(n_cells, ny, blocks=4, seed=0, scale=1.0):
     return levels[bi[:, None], bj[None, :]].reshape(-1)
 
 
-def solve_with_checkpoints(problems, every=16, nthreads=0, device=0, lockstep=True, groups=4):
+def solve_with_checkpoints(problems, every=16, nthreads=0, device=0, lockstep=True, groups=8):
     """FP64 fixed-rank solves of every problem dict, checkpointed for the
     adjoint: with shared launches (``lockstep``; ``nthreads`` > 0 limits how
     many run at once) or one stream per context over ``nthreads`` host
@@ -60,7 +60,7 @@ def solve_with_checkpoints(problems, every=16, nthreads=0, device=0, lockstep=Tr
 
 
 def plan_objective_rrule(problems, weights, scale, target, every=16, nthreads=0, device=0,
-                         lockstep=True, groups=4):
+                         lockstep=True, groups=8):
     """J and its pullback for the plan objective above.
 
     ``problems``: fixed-rank problem dicts (``beamlet_plan(...,
@@ -99,7 +99,7 @@ def plan_objective_rrule(problems, weights, scale, target, every=16, nthreads=0,
 
 
 def plan_objective_grad(problems, weights, scale, target, every=16, nthreads=0, device=0,
-                        lockstep=True, groups=4):
+                        lockstep=True, groups=8):
     """(J, dJ/dw, dJ/ds) in one call."""
     J, pb = plan_objective_rrule(problems, weights, scale, target, every, nthreads, device,
                                  lockstep, groups)

@@ -79,6 +79,11 @@ void adjoint_reserve(titus_ctx& x) {
   }
 }
 
+int adjoint_phase_key(const titus_ctx& x) {
+  if (!x.adj || x.adj->every <= 0 || x.steps_done < 1) return 0;
+  return (x.steps_done - 1) % x.adj->every + 1;
+}
+
 void adjoint_checkpoint(titus_ctx& x, int s) {
   AdjointStore& a = *x.adj;
   if (x.tape || (s - 1) % a.every != 0) return;

@@ -208,6 +208,10 @@ void lockstep_sync(titus_ctx& x);
 // codes in status, counters in stats (6), tuned tiles in tiles (4 per group)
 int lockstep_run(titus_ctx** ctxs, int n, int width, int groups, int32_t* status, int64_t* stats,
                  int32_t* tiles, const std::function<void(titus_ctx&, int)>& work);
+// adjoint.cpp: for a context with checkpoints, the length of the last
+// checkpoint segment of its solve so far (the first one its adjoint
+// recomputes); 0 without checkpoints
+int adjoint_phase_key(const titus_ctx& x);
 // adjoint.cpp: checkpoint of the state before step s (when enabled)
 void adjoint_checkpoint(titus_ctx& x, int s);
 

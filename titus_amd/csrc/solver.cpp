@@ -661,9 +661,14 @@ int lockstep_run(titus_ctx** ctxs, int n, int width, int groups, int32_t* status
   // group's launches share kernel variants and tile counts
   std::vector<int> start(w);
   std::iota(start.begin(), start.end(), 0);
-  std::stable_sort(start.begin(), start.end(), [&](int a, int b) {
-    return std::make_pair(ctxs[a]->nm, ctxs[a]->capx) < std::make_pair(ctxs[b]->nm, ctxs[b]->capx);
-  });
+  // (an adjoint batch first sorts by the length of the last checkpoint
+  // segment, which it recomputes and reverses first: contexts with equal
+  // lengths run their recomputations and reverse sweeps in phase)
+  auto key = [&](int i) {
+    const titus_ctx* c = ctxs[i];
+    return std::make_tuple(adjoint_phase_key(*c), c->nm, c->capx);
+  };
+  std::stable_sort(start.begin(), start.end(), [&](int a, int b) { return key(a) < key(b); });
   std::vector<int> tgroup(w);
   for (int t = 0; t < w; ++t) {
     tgroup[t] = (int)((int64_t)t * G / w);
