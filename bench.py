@@ -379,6 +379,34 @@ def stencil_scale(sync, steps=4, n=1024, calibrate=False):
         torch.cuda.synchronize()
         gbs = 2 * a.numel() * 8 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
         out["copy_256MiB"] = {"GB/s": round(gbs, 1), "frac_of_8TB/s": round(gbs / HBM_PEAK_GBS, 3)}
+        if not calibrate:
+            out["hbm_ceiling"] = hbm_ceiling(n + 1, n + 1, P["Q"], out)
+    return out
+
+
+def hbm_ceiling(nx, ny, nq, scale):
+    """The measured ceilings (titus_amd/hbm_ceiling, a child process): best
+    read, fill and copy of 1 GiB, and the best copy with the sweep's access
+    pattern on the same psi planes; each sweep's rate as a share of them."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "titus_amd", "hbm_ceiling")
+    if not os.path.exists(exe):
+        return {"error": "titus_amd/hbm_ceiling is not built (python -m titus_amd.build)"}
+    res = subprocess.run([exe, str(nx), str(ny), str(nq)], capture_output=True, text=True, timeout=120)
+    if res.returncode != 0:
+        return {"error": res.stderr.strip()[-300:]}
+    raw = json.loads(res.stdout)
+    best = lambda runs: max(runs, key=lambda e: e["GB/s"])
+    out = {"read": best(raw["read"]), "fill": best(raw["fill"]), "copy": best(raw["copy"]),
+           "copy_hipMemcpy": next(e for e in raw["copy"] if e["shape"] == "hipMemcpyAsync")}
+    for form, prefix in (("fp64", "fp64"), ("mixed", "fp32")):
+        strip = best([e for e in raw["strip_copy"]["runs"] if e["shape"].startswith(prefix)])
+        out["sweep_pattern_copy_" + prefix] = strip
+        sw = scale.get(form, {}).get("sweep")
+        if sw:
+            out["sweep_" + form + "_share"] = {
+                "of_pattern_copy": round(sw["GB/s"] / strip["GB/s"], 3),
+                "of_best_copy": round(sw["GB/s"] / out["copy"]["GB/s"], 3)}
     return out
 
 

@@ -5,6 +5,7 @@
 Outputs (git-ignored, shipped with the tree):
     titus_amd/libtitus_hip.so   -- the C-ABI of include/titus_hip.h
     titus_amd/_smallla_host.so  -- host dense LA with a C test interface
+    titus_amd/hbm_ceiling       -- the measured HBM ceilings (bench.py runs it)
 """
 import os
 import shutil
@@ -16,6 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtitus_hip.so")
 SMALLLA = os.path.join(HERE, "_smallla_host.so")
+HBM_CEILING = os.path.join(HERE, "hbm_ceiling")
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -59,8 +61,11 @@ def build(verbose=False):
     _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared",
           os.path.join(CSRC, "smallla.cpp"), os.path.join(CSRC, "smallla_capi.cpp"), "-o", tmp])
     os.replace(tmp, SMALLLA)
+    tmp = HBM_CEILING + ".tmp"
+    _run([hipcc, *HIP_FLAGS, os.path.join(CSRC, "hbm_ceiling.hip"), "-o", tmp])
+    os.replace(tmp, HBM_CEILING)
     if verbose:
-        print("built", LIB, "and", SMALLLA)
+        print("built", LIB, SMALLLA, "and", HBM_CEILING)
     return LIB
 
 
