@@ -53,10 +53,10 @@ PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 # per-dispatch figures, the command and the window are in
 # profiles/dlra_step_phantom200_p21_pmc.md; the entry names them, and the
 # bench line repeats them beside the number.
-MEASURED_TRAFFIC = {"gram": {"bytes": 5.89e6, "kernels": "k_tn_mfma<1,...> + k_tn_reduce",
-                             "window": "steps 321..620, mixed form",
+MEASURED_TRAFFIC = {"gram": {"bytes": 5.90e6, "kernels": "TnBody<1,...> + TnReduceBody",
+                             "window": "steps 306..325, mixed form",
                              "command": "rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE) -- "
-                                        "python3 bench.py --profile-form mixed",
+                                        "python3 bench.py --profile-form mixed --steps 20 --warmup 5",
                              "profile": "profiles/dlra_step_phantom200_p21_pmc.md"}}
 # HBM bytes per launch of the full-rank solver's collided kernel on the
 # phantom (FETCH_SIZE x 2 + WRITE_SIZE x 1, profiles/fcs_collide_phantom200_p21.md)
