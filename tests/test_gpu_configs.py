@@ -1,8 +1,8 @@
 """The benchmark configurations at their stated sizes on the GPU.
 
 * 2D phantom, 200 x 200 cells, P_21 (253 moments), rank-adaptive BUG
-  (theta = 1e-2, cap 40): FP64 device vs the FP64 oracle over 200 energy
-  steps (dose relative L2 <= 1e-10, equal rank histories); the mixed form
+  (theta = 1e-2, cap 40): FP64 device vs the FP64 oracle at energy steps
+  200 and 558 (dose relative L2 <= 1e-10, equal rank histories); the mixed form
   against the FP64 solve at the mixed rank floor and at the plain one.
 * 3D proton cube, 64^3 cells of 0.1 cm, P_13 (196 moments), fixed rank 30:
   FP64 device vs the oracle over the first 8 steps; the whole 300-step solve
@@ -68,21 +68,43 @@ def phantom():
     return cases.phantom_bug()
 
 
-def test_phantom_200_p21_fp64_vs_oracle_200_steps(phantom):
+def test_phantom_200_p21_fp64_vs_oracle(phantom):
+    """FP64 device vs the oracle at step 200 and at step 558 (the end of the
+    bench's CPU-baseline window): dose <= 1e-10 at both, equal rank histories.
+    The truncation's SVD follows LAPACK's algorithm family (bidiagonalization,
+    then QR sweeps); with a one-sided Jacobi SVD there the device drifted from
+    the oracle to 1.2e-10 by step 558 while staying at 1e-12 of an oracle run
+    with the same Jacobi SVD."""
     P = phantom
-    announce(f"phantom 200x200 P_21 BUG, 200 steps: C={P['C']} Nm={P['Nm']} Q={P['Q']}")
+    announce(f"phantom 200x200 P_21 BUG, 558 steps: C={P['C']} Nm={P['Nm']} Q={P['Q']}")
     assert P["nx"] == 200 and P["ny"] == 200 and P["Nm"] == 253 and P["rank_cap"] == 40
     t = time.perf_counter()
-    g = run(P, INTEGRATOR_BUG, 200)
+    ctx = _lib.Context(P, INTEGRATOR_BUG)
+    try:
+        ctx.solve(200)
+        g200 = ctx.state()
+        ctx.solve(558)
+        g = ctx.state()
+        g["ranks"] = ctx.history(ctx.progress()[0])[0]
+    finally:
+        ctx.close()
     tg = time.perf_counter() - t
+    marks = {}
+
+    def keep(s, dose):
+        if s == 200:
+            marks[s] = dose.copy()
+
     t = time.perf_counter()
-    o = dlra.solve_bug(P, steps=200)
+    o = dlra.solve_bug(P, steps=558, on_step=keep)
     to = time.perf_counter() - t
+    e200 = rel(g200["dose"], marks[200])
     ed = rel(g["dose"], o["dose"])
     eu = rel(g["X"] @ g["S"] @ g["W"].T, o["X"] @ o["S"] @ o["W"].T)
-    print(f"[phantom fp64] device {tg:.2f} s oracle {to:.1f} s dose rel {ed:.3e} u rel {eu:.3e} "
-          f"ranks {int(min(o['ranks']))}..{int(max(o['ranks']))}", flush=True)
+    print(f"[phantom fp64] device {tg:.2f} s oracle {to:.1f} s dose rel at 200 {e200:.3e}, at 558 "
+          f"{ed:.3e}; u rel {eu:.3e} ranks {int(min(o['ranks']))}..{int(max(o['ranks']))}", flush=True)
     assert np.array_equal(g["ranks"], np.asarray(o["ranks"]))
+    assert e200 <= 1e-10
     assert ed <= 1e-10
     assert eu <= 1e-8
 

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <numeric>
 #include <stdexcept>
 
@@ -557,11 +558,226 @@ HMat orth_bar(const OrthRecord& rec, const HMat& X1, const HMat& X1bar) {
   return Ab;
 }
 
+// ---------------------------------------------------------------------------
+// Golub-Kahan-Reinsch SVD (Golub and Reinsch, Numer. Math. 14, 1970; Golub
+// and Van Loan, Matrix Computations, sec. 8.6): Householder bidiagonalization
+// A = Q B P^T (column reflector, then row reflector, per column), then
+// implicit Wilkinson-shift QR sweeps on the bidiagonal B with deflation, the
+// rotations accumulated into U = Q and V = P.  The same algorithm family as
+// LAPACK's gesvd/gesdd (reflectors, then a bidiagonal SVD).
+
+namespace {
+
+// Householder reflector H = I - tau v v^T with H [alpha; x] = [beta; 0],
+// v = [1; x / (alpha - beta)]; x is overwritten by v's tail.  tau = 0 (H = I,
+// beta = alpha) when x is zero.
+double make_reflector(double& alpha, double* x, int nx, int stride) {
+  double xn2 = 0.0;
+  for (int i = 0; i < nx; ++i) xn2 += x[(size_t)i * stride] * x[(size_t)i * stride];
+  if (xn2 == 0.0) return 0.0;
+  const double nrm = std::sqrt(alpha * alpha + xn2);
+  const double beta = alpha >= 0.0 ? -nrm : nrm;
+  const double tau = (beta - alpha) / beta;
+  const double scal = 1.0 / (alpha - beta);
+  for (int i = 0; i < nx; ++i) x[(size_t)i * stride] *= scal;
+  alpha = beta;
+  return tau;
+}
+
+// B = diag(d) + superdiag(e) (e[i] couples rows/columns i and i + 1) with
+// A = U B V^T; on return d >= 0 holds the singular values (unsorted) and U,
+// V the accumulated rotations.
+void bidiag_qr(std::vector<double>& d, std::vector<double>& e, HMat& U, HMat& V) {
+  const int n = (int)d.size();
+  const double eps = std::numeric_limits<double>::epsilon();
+  double anorm = 0.0;
+  for (int i = 0; i < n; ++i)
+    anorm = std::max(anorm, std::fabs(d[i]) + (i + 1 < n ? std::fabs(e[i]) : 0.0));
+  const double small = eps * anorm;
+  // columns a, b of M <- (c a + s b, -s a + c b)
+  auto rot = [](HMat& M, int a, int b, double c, double s) {
+    double* x = M.col(a);
+    double* y = M.col(b);
+    for (int i = 0; i < M.m; ++i) {
+      const double xa = x[i], yb = y[i];
+      x[i] = c * xa + s * yb;
+      y[i] = -s * xa + c * yb;
+    }
+  };
+  for (int hi = n - 1; hi >= 0; --hi) {
+    for (int iter = 0;; ++iter) {
+      if (iter > 100) throw std::runtime_error("svd: bidiagonal QR did not converge");
+      // the unreduced block lo..hi: split where a coupling is negligible
+      // against its two diagonal neighbours (or the matrix); a negligible
+      // diagonal entry above the block is rotated out first
+      int lo = hi;
+      bool zero_diag = false;
+      while (lo > 0) {
+        const double el = std::fabs(e[lo - 1]);
+        if (el <= eps * (std::fabs(d[lo - 1]) + std::fabs(d[lo])) || el <= small) {
+          e[lo - 1] = 0.0;
+          break;
+        }
+        if (std::fabs(d[lo - 1]) <= small) {
+          zero_diag = true;
+          break;
+        }
+        --lo;
+      }
+      if (zero_diag) {
+        // d[k] ~ 0 with e[k] != 0 (k = lo - 1): left rotations of row k
+        // against rows lo..hi move e[k] along the row and out of the block
+        const int k = lo - 1;
+        double f = e[k];
+        e[k] = 0.0;
+        d[k] = 0.0;
+        for (int i = lo; i <= hi; ++i) {
+          const double g = d[i];
+          const double h = std::hypot(f, g);
+          const double c = g / h, s = f / h;
+          d[i] = h;
+          // rows (i, k): row_i <- c row_i + s row_k, row_k <- -s row_i + c row_k
+          rot(U, i, k, c, s);
+          if (i < hi) {
+            f = -s * e[i];
+            e[i] = c * e[i];
+            if (std::fabs(f) <= small) break;
+          }
+        }
+        continue;
+      }
+      if (lo == hi) {
+        if (d[hi] < 0.0) {
+          d[hi] = -d[hi];
+          double* v = V.col(hi);
+          for (int i = 0; i < V.m; ++i) v[i] = -v[i];
+        }
+        break;
+      }
+      // Wilkinson shift: the eigenvalue of the trailing 2 x 2 of B^T B
+      // nearer its last diagonal entry
+      const double t11 = d[hi - 1] * d[hi - 1] + (hi - 1 > lo ? e[hi - 2] * e[hi - 2] : 0.0);
+      const double t12 = d[hi - 1] * e[hi - 1];
+      const double t22 = d[hi] * d[hi] + e[hi - 1] * e[hi - 1];
+      const double del = 0.5 * (t11 - t22);
+      const double den = del + (del >= 0.0 ? 1.0 : -1.0) * std::hypot(del, t12);
+      const double mu = den != 0.0 ? t22 - t12 * t12 / den : t22;
+      // chase the bulge from lo to hi
+      double y = d[lo] * d[lo] - mu;
+      double z = d[lo] * e[lo];
+      for (int k = lo; k < hi; ++k) {
+        // right rotation of columns k, k + 1: [y z] -> [r 0]
+        double r = std::hypot(y, z);
+        double c = r != 0.0 ? y / r : 1.0, s = r != 0.0 ? z / r : 0.0;
+        if (k > lo) e[k - 1] = r;
+        const double dk = c * d[k] + s * e[k];
+        const double ek = -s * d[k] + c * e[k];
+        const double bulge_lo = s * d[k + 1];           // at (k + 1, k)
+        const double dk1 = c * d[k + 1];
+        rot(V, k, k + 1, c, s);
+        // left rotation of rows k, k + 1: [dk; bulge_lo] -> [r; 0]
+        r = std::hypot(dk, bulge_lo);
+        c = r != 0.0 ? dk / r : 1.0;
+        s = r != 0.0 ? bulge_lo / r : 0.0;
+        d[k] = r;
+        e[k] = c * ek + s * dk1;
+        d[k + 1] = -s * ek + c * dk1;
+        rot(U, k, k + 1, c, s);
+        if (k + 1 < hi) {
+          y = e[k];
+          z = s * e[k + 1];                              // bulge at (k, k + 2)
+          e[k + 1] = c * e[k + 1];
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+void svd_gk(const HMat& A0, HMat& U, std::vector<double>& s, HMat& V) {
+  if (A0.m < A0.n) {
+    HMat Ut, Vt;
+    svd_gk(transpose(A0), Ut, s, Vt);
+    U = Vt;
+    V = Ut;
+    return;
+  }
+  const int m = A0.m, n = A0.n;
+  HMat A = A0;
+  std::vector<double> tq(n, 0.0), tp(n, 0.0), d(n, 0.0), e(std::max(0, n - 1), 0.0);
+  for (int j = 0; j < n; ++j) {
+    // column reflector: zero A(j+1:m, j)
+    double* a = A.col(j);
+    tq[j] = make_reflector(a[j], a + j + 1, m - j - 1, 1);
+    d[j] = a[j];
+    if (tq[j] != 0.0)
+      for (int k = j + 1; k < n; ++k) {
+        double* b = A.col(k);
+        const double w = tq[j] * (b[j] + dot(a + j + 1, b + j + 1, m - j - 1));
+        b[j] -= w;
+        for (int i = j + 1; i < m; ++i) b[i] -= w * a[i];
+      }
+    if (j + 1 < n) {
+      // row reflector: zero A(j, j+2:n)
+      double& alpha = A(j, j + 1);
+      tp[j] = make_reflector(alpha, &A(j, std::min(j + 2, n - 1)), n - j - 2, m);
+      e[j] = alpha;
+      if (tp[j] != 0.0)
+        for (int i = j + 1; i < m; ++i) {
+          double w = A(i, j + 1);
+          for (int k = j + 2; k < n; ++k) w += A(i, k) * A(j, k);
+          w *= tp[j];
+          A(i, j + 1) -= w;
+          for (int k = j + 2; k < n; ++k) A(i, k) -= w * A(j, k);
+        }
+    }
+  }
+  // U = H_0 ... H_{n-1} I(m x n), V = G_0 ... G_{n-2} (G_j acts on j+1..n-1)
+  HMat Uq(m, n), Vp = HMat::eye(n, n);
+  for (int j = 0; j < n; ++j) Uq(j, j) = 1.0;
+  for (int j = n - 1; j >= 0; --j) {
+    if (tq[j] == 0.0) continue;
+    const double* v = A.col(j);
+    for (int k = j; k < n; ++k) {
+      double* b = Uq.col(k);
+      const double w = tq[j] * (b[j] + dot(v + j + 1, b + j + 1, m - j - 1));
+      b[j] -= w;
+      for (int i = j + 1; i < m; ++i) b[i] -= w * v[i];
+    }
+  }
+  for (int j = n - 3; j >= 0; --j) {
+    if (tp[j] == 0.0) continue;
+    // v = [1; A(j, j+2:n)] on rows j+1..n-1
+    for (int k = j + 1; k < n; ++k) {
+      double* b = Vp.col(k);
+      double w = b[j + 1];
+      for (int i = j + 2; i < n; ++i) w += A(j, i) * b[i];
+      w *= tp[j];
+      b[j + 1] -= w;
+      for (int i = j + 2; i < n; ++i) b[i] -= w * A(j, i);
+    }
+  }
+  bidiag_qr(d, e, Uq, Vp);
+  // descending order (stable: equal values keep their positions)
+  std::vector<int> ord(n);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return d[a] > d[b]; });
+  U = HMat(m, n);
+  V = HMat(n, n);
+  s.assign(n, 0.0);
+  for (int k = 0; k < n; ++k) {
+    s[k] = d[ord[k]];
+    std::copy(Uq.col(ord[k]), Uq.col(ord[k]) + m, U.col(k));
+    std::copy(Vp.col(ord[k]), Vp.col(ord[k]) + n, V.col(k));
+  }
+}
+
 void svd_completed(const HMat& S, double tol, HMat& U, std::vector<double>& d, HMat& V) {
   const int n = S.n;
   HMat Us, Vs;
   std::vector<double> sv;
-  svd(S, Us, sv, Vs);
+  svd_gk(S, Us, sv, Vs);
   int p = 0;
   if (!sv.empty() && sv[0] > 0.0)
     for (double x : sv) if (x > tol * sv[0]) ++p;

@@ -53,6 +53,14 @@ double frob(const HMat& A);
 // with and without want_v; V is left empty without it (m >= n only).
 void svd(const HMat& A, HMat& U, std::vector<double>& s, HMat& V, bool want_v = true);
 
+// Thin SVD by Golub-Kahan-Reinsch: Householder bidiagonalization, then
+// implicit-shift QR on the bidiagonal (the algorithm family of LAPACK's
+// gesvd).  Same shapes and order as svd(); always forms V.  Its small singular
+// directions carry the errors of a bidiagonalization (order eps * s_1), where
+// the preconditioned Jacobi of svd() resolves them to high relative accuracy;
+// truncate! follows the reference's LAPACK SVD, so it uses this one.
+void svd_gk(const HMat& A, HMat& U, std::vector<double>& s, HMat& V);
+
 // Cholesky A = R^T R of a symmetric positive definite matrix; returns false
 // (and leaves R undefined) when a pivot is not positive.
 bool cholesky_upper(const HMat& A, HMat& R);
