@@ -974,11 +974,12 @@ __device__ __forceinline__ void d_chol_inv(Blk blk, const double* __restrict__ G
                            double* __restrict__ status) {
   // A (k x k, column-major, LDS): the factor in the upper triangle; its
   // diagonal is kept apart in Dg, and the strict lower triangle receives
-  // R^-1 transposed during the back substitution
+  // R^-1 transposed during the back substitution.  kBlock threads: a 16 x 16
+  // grid over the trailing block, 64 teams of four lanes in the inverse.
   extern __shared__ double A[];
   __shared__ double Dg[kMaxChol];
   __shared__ double tr;
-  const int tid = threadIdx.x, nt = blockDim.x;
+  const int tid = threadIdx.x, nt = kBlock;
   // the trace: wavefront 0 loads the diagonal in parallel (one load latency,
   // not k of them in a row) and adds it in a fixed shuffle tree
   if (tid < 64) {
@@ -1006,10 +1007,11 @@ __device__ __forceinline__ void d_chol_inv(Blk blk, const double* __restrict__ G
       if (tid == 0) Dg[j] = piv;
       for (int c = j + 1 + tid; c < k; c += nt) A[(size_t)c * k + j] /= piv;
       __syncthreads();
-      const int w = k - j - 1;                // trailing block (j, k) x (j, k)
-      for (int e = tid; e < w * w; e += nt) {
-        const int i = j + 1 + e % w, c = j + 1 + e / w;
-        if (i <= c) A[(size_t)c * k + i] -= A[(size_t)i * k + j] * A[(size_t)c * k + j];
+      // trailing block (j, k) x (j, k), upper triangle: thread (ti, tc) of
+      // the 16 x 16 grid takes rows j+1+ti (+16) of columns j+1+tc (+16)
+      for (int c = j + 1 + (tid >> 4); c < k; c += 16) {
+        const double acj = A[(size_t)c * k + j];
+        for (int i = j + 1 + (tid & 15); i <= c; i += 16) A[(size_t)c * k + i] -= A[(size_t)i * k + j] * acj;
       }
       __syncthreads();
     }
@@ -1023,14 +1025,27 @@ __device__ __forceinline__ void d_chol_inv(Blk blk, const double* __restrict__ G
     R[e] = (i < j) ? A[e] : (i == j ? Dg[j] : 0.0);
   }
   __syncthreads();
-  // column c of R^-1 by back substitution, one thread per column; entry
-  // (i, c), i < c, is kept in the lower triangle at A[i*k + c]
-  for (int c = tid; c < k; c += nt) {
-    const double dc = 1.0 / Dg[c];
-    for (int i = c - 1; i >= 0; --i) {
-      double acc = A[(size_t)c * k + i] * dc;
-      for (int m = i + 1; m < c; ++m) acc += A[(size_t)m * k + i] * A[(size_t)m * k + c];
-      A[(size_t)i * k + c] = -acc / Dg[i];
+  // column c of R^-1 = R^-1 e_c by a team of four lanes (column-oriented
+  // substitution): x = e_c; for m = c down to 0: x_m /= R(m, m), then
+  // x_i -= R(i, m) x_m for i < m.  Lane tl of the team owns x_i for
+  // i = tl (mod 4), kept in the lower triangle at A[i*k + c] (only its owner
+  // reads or writes it); the pivot x_m reaches the team by a shuffle.  x_c =
+  // 1 / R(c, c) is not stored (the output writes the diagonal from Dg).  The
+  // loops stay compact: a one-workgroup kernel starts on a cold instruction
+  // cache, and unrolled register-blocked forms of both loops ran slower.
+  {
+    const int team = tid >> 2, tl = tid & 3, base = (tid & 63) & ~3;
+    for (int c = team; c < k; c += nt / 4) {
+      const double xc = 1.0 / Dg[c];
+      for (int i = tl; i < c; i += 4) A[(size_t)i * k + c] = -(A[(size_t)c * k + i] * xc);
+      for (int m = c - 1; m >= 0; --m) {
+        const bool own = (m & 3) == tl;
+        double v = own ? A[(size_t)m * k + c] : 0.0;
+        v = __shfl(v, base | (m & 3), 64);
+        const double xm = v / Dg[m];
+        if (own) A[(size_t)m * k + c] = xm;
+        for (int i = tl; i < m; i += 4) A[(size_t)i * k + c] -= A[(size_t)m * k + i] * xm;
+      }
     }
   }
   __syncthreads();
