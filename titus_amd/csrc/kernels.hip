@@ -978,6 +978,7 @@ __device__ __forceinline__ void d_chol_inv(Blk blk, const double* __restrict__ G
   // grid over the trailing block, 64 teams of four lanes in the inverse.
   extern __shared__ double A[];
   __shared__ double Dg[kMaxChol];
+  __shared__ double Rw0[kMaxChol], Rw1[kMaxChol];
   __shared__ double tr;
   const int tid = threadIdx.x, nt = kBlock;
   // the trace: wavefront 0 loads the diagonal in parallel (one load latency,
@@ -999,19 +1000,42 @@ __device__ __forceinline__ void d_chol_inv(Blk blk, const double* __restrict__ G
     }
     __syncthreads();
     good = true;
-    for (int j = 0; j < k; ++j) {
-      // every thread reads the same pivot, so the exit below is uniform
-      const double d = A[(size_t)j * k + j];
-      if (!(d > 0.0)) { good = false; break; }
-      const double piv = sqrt(d);
-      if (tid == 0) Dg[j] = piv;
-      for (int c = j + 1 + tid; c < k; c += nt) A[(size_t)c * k + j] /= piv;
+    // two columns per phase: the scaled rows j and j + 1 of R go to Rw0 /
+    // Rw1 (and A), then one rank-2 update of the trailing block; each entry
+    // takes the same operations in the same order as one column at a time
+    for (int j = 0; j < k; j += 2) {
+      // every thread reads the same pivots, so the exits below are uniform
+      const double d0 = A[(size_t)j * k + j];
+      if (!(d0 > 0.0)) { good = false; break; }
+      const double piv0 = sqrt(d0);
+      if (j + 1 == k) {
+        if (tid == 0) Dg[j] = piv0;
+        break;
+      }
+      const double r01 = A[(size_t)(j + 1) * k + j] / piv0;
+      const double d1 = A[(size_t)(j + 1) * k + j + 1] - r01 * r01;
+      if (!(d1 > 0.0)) { good = false; break; }
+      const double piv1 = sqrt(d1);
+      for (int c = j + 2 + tid; c < k; c += nt) {
+        const double a0 = A[(size_t)c * k + j] / piv0;
+        const double a1 = (A[(size_t)c * k + j + 1] - r01 * a0) / piv1;
+        Rw0[c] = a0;
+        Rw1[c] = a1;
+        A[(size_t)c * k + j] = a0;
+        A[(size_t)c * k + j + 1] = a1;
+      }
       __syncthreads();
-      // trailing block (j, k) x (j, k), upper triangle: thread (ti, tc) of
-      // the 16 x 16 grid takes rows j+1+ti (+16) of columns j+1+tc (+16)
-      for (int c = j + 1 + (tid >> 4); c < k; c += 16) {
-        const double acj = A[(size_t)c * k + j];
-        for (int i = j + 1 + (tid & 15); i <= c; i += 16) A[(size_t)c * k + i] -= A[(size_t)i * k + j] * acj;
+      if (tid == 0) {
+        Dg[j] = piv0;
+        Dg[j + 1] = piv1;
+        A[(size_t)(j + 1) * k + j] = r01;
+      }
+      // trailing block (j+1, k) x (j+1, k), upper triangle: thread (ti, tc)
+      // of the 16 x 16 grid takes rows j+2+ti (+16) of columns j+2+tc (+16)
+      for (int c = j + 2 + (tid >> 4); c < k; c += 16) {
+        const double s0 = Rw0[c], s1 = Rw1[c];
+        for (int i = j + 2 + (tid & 15); i <= c; i += 16)
+          A[(size_t)c * k + i] = (A[(size_t)c * k + i] - Rw0[i] * s0) - Rw1[i] * s1;
       }
       __syncthreads();
     }
@@ -1032,19 +1056,32 @@ __device__ __forceinline__ void d_chol_inv(Blk blk, const double* __restrict__ G
   // reads or writes it); the pivot x_m reaches the team by a shuffle.  x_c =
   // 1 / R(c, c) is not stored (the output writes the diagonal from Dg).  The
   // loops stay compact: a one-workgroup kernel starts on a cold instruction
-  // cache, and unrolled register-blocked forms of both loops ran slower.
+  // cache, and unrolled register-blocked forms of both phases ran slower.
   {
     const int team = tid >> 2, tl = tid & 3, base = (tid & 63) & ~3;
     for (int c = team; c < k; c += nt / 4) {
       const double xc = 1.0 / Dg[c];
       for (int i = tl; i < c; i += 4) A[(size_t)i * k + c] = -(A[(size_t)c * k + i] * xc);
-      for (int m = c - 1; m >= 0; --m) {
-        const bool own = (m & 3) == tl;
-        double v = own ? A[(size_t)m * k + c] : 0.0;
-        v = __shfl(v, base | (m & 3), 64);
-        const double xm = v / Dg[m];
-        if (own) A[(size_t)m * k + c] = xm;
-        for (int i = tl; i < m; i += 4) A[(size_t)i * k + c] -= A[(size_t)m * k + i] * xm;
+      // pivots m and m - 1 per pass (the same operations per entry as one
+      // pivot at a time), then a last single pivot when one is left
+      int m = c - 1;
+      for (; m >= 1; m -= 2) {
+        const bool own0 = (m & 3) == tl, own1 = ((m - 1) & 3) == tl;
+        double v0 = own0 ? A[(size_t)m * k + c] : 0.0;
+        double v1 = own1 ? A[(size_t)(m - 1) * k + c] : 0.0;
+        v0 = __shfl(v0, base | (m & 3), 64);
+        v1 = __shfl(v1, base | ((m - 1) & 3), 64);
+        const double x0 = v0 / Dg[m];
+        const double x1 = (v1 - A[(size_t)m * k + m - 1] * x0) / Dg[m - 1];
+        if (own0) A[(size_t)m * k + c] = x0;
+        if (own1) A[(size_t)(m - 1) * k + c] = x1;
+        for (int i = tl; i < m - 1; i += 4)
+          A[(size_t)i * k + c] = (A[(size_t)i * k + c] - A[(size_t)m * k + i] * x0) - A[(size_t)(m - 1) * k + i] * x1;
+      }
+      if (m == 0) {
+        double v = tl == 0 ? A[c] : 0.0;
+        v = __shfl(v, base, 64);
+        if (tl == 0) A[c] = v / Dg[0];
       }
     }
   }
