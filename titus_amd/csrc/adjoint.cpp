@@ -60,6 +60,23 @@ struct AdjointStore {
     owned.push_back(p);
     return p;
   }
+  // one device block carved into buffers of the given sizes, each rounded
+  // up to 64 doubles (512-byte alignment, as a block of its own has): a
+  // context with checkpoints and tapes holds a few blocks instead of
+  // hundreds, which keeps its creation and destruction cheap (every device
+  // free waits for the device)
+  std::vector<double*> alloc_many(const std::vector<size_t>& sizes) {
+    size_t total = 0;
+    for (size_t n : sizes) total += (n + 63) & ~size_t(63);
+    double* base = alloc(total);
+    std::vector<double*> out;
+    size_t off = 0;
+    for (size_t n : sizes) {
+      out.push_back(base + off);
+      off += (n + 63) & ~size_t(63);
+    }
+    return out;
+  }
   ~AdjointStore() {
     for (double* p : owned) (void)hipFree(p);
   }
@@ -72,10 +89,16 @@ void adjoint_reserve(titus_ctx& x) {
   AdjointStore& a = *x.adj;
   const int n = (x.ne - 2) / a.every + 1;                 // steps 1 .. ne-1
   a.cps.resize(n);
+  std::vector<size_t> sizes;
+  for (int i = 0; i < n; ++i) {
+    sizes.push_back((size_t)x.C * x.rank);
+    sizes.push_back((size_t)x.C * x.nq);
+  }
+  const std::vector<double*> b = a.alloc_many(sizes);
   for (int i = 0; i < n; ++i) {
     a.cps[i].s = 1 + i * a.every;
-    a.cps[i].X = a.alloc((size_t)x.C * x.rank);
-    a.cps[i].psi = a.alloc((size_t)x.C * x.nq);
+    a.cps[i].X = b[2 * i];
+    a.cps[i].psi = b[2 * i + 1];
   }
 }
 
@@ -384,12 +407,14 @@ void adjoint_begin(titus_ctx& x, const double* dose_bar, AdjointRun& run) {
   const size_t cr = (size_t)C * r, cq = (size_t)C * nq;
   // buffers (first call)
   if (!a.Xb) {
-    a.Xb = a.alloc(cr); a.XmB = a.alloc(cr); a.Kb = a.alloc(cr); a.T1 = a.alloc(cr);
-    a.T2 = a.alloc(cr); a.K0 = a.alloc(cr); a.Ub = a.alloc(cr);
-    a.psib = a.alloc(cq); a.psib2 = a.alloc(cq);
-    a.ybar = a.alloc(C); a.Db = a.alloc(C); a.gbuf = a.alloc(C);
-    a.Xsave = a.alloc(cr); a.psisave = a.alloc(cq); a.dosesave = a.alloc(C);
-    a.u0save = a.alloc((size_t)dev::kMaxSweepGroups * C);
+    const std::vector<double*> b = a.alloc_many(
+        {cr, cr, cr, cr, cr, cr, cr, cq, cq, (size_t)C, (size_t)C, (size_t)C, cr, cq, (size_t)C,
+         (size_t)dev::kMaxSweepGroups * C});
+    a.Xb = b[0]; a.XmB = b[1]; a.Kb = b[2]; a.T1 = b[3]; a.T2 = b[4]; a.K0 = b[5]; a.Ub = b[6];
+    a.psib = b[7]; a.psib2 = b[8];
+    a.ybar = b[9]; a.Db = b[10]; a.gbuf = b[11];
+    a.Xsave = b[12]; a.psisave = b[13]; a.dosesave = b[14];
+    a.u0save = b[15];
     for (const titus::SpMat& A : x.sA) {
       HMat D(A.m, A.n);
       for (int i = 0; i < A.m; ++i)
@@ -399,12 +424,20 @@ void adjoint_begin(titus_ctx& x, const double* dose_bar, AdjointRun& run) {
   }
   // step tapes for the longest segment
   const size_t seg = (size_t)std::min(a.every, x.steps_done);
-  while (a.tapes.size() < seg) {
-    a.tapes.emplace_back();
-    StepTape& t = a.tapes.back();
-    t.X = a.alloc(cr); t.X1 = a.alloc(cr); t.X3 = a.alloc(cr);
-    t.psi_a = a.alloc(cq); t.psi_b = a.alloc(cq);
-    t.Rx1.Q2 = a.alloc(cr); t.Rx3.Q2 = a.alloc(cr);
+  if (a.tapes.size() < seg) {
+    const size_t have = a.tapes.size(), add = seg - have;
+    std::vector<size_t> sizes;
+    for (size_t i = 0; i < add; ++i)
+      for (size_t n : {cr, cr, cr, cq, cq, cr, cr}) sizes.push_back(n);
+    const std::vector<double*> b = a.alloc_many(sizes);
+    a.tapes.resize(seg);
+    for (size_t i = 0; i < add; ++i) {
+      StepTape& t = a.tapes[have + i];
+      double* const* q = b.data() + 7 * i;
+      t.X = q[0]; t.X1 = q[1]; t.X3 = q[2];
+      t.psi_a = q[3]; t.psi_b = q[4];
+      t.Rx1.Q2 = q[5]; t.Rx3.Q2 = q[6];
+    }
   }
   hipStream_t st = x.st;
   // save the forward state the recomputation overwrites
