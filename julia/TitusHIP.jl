@@ -462,18 +462,29 @@ end
 set_checkpoints!(ctx::TitusContext, every::Integer) =
     check(ccall((:titus_set_checkpoints, libtitus), Cint, (Ptr{Cvoid}, Int32), ctx.ptr, Int32(every)))
 
-"""d <dose_bar, dose> / d density for each context (C values each)."""
+"""d <dose_bar, dose> / d density for each context (C values each): with
+`lockstep`, the recomputations and reverse sweeps of all contexts share their
+launches (in `groups` groups); otherwise one stream per context on up to
+`nthreads` host threads.  Both give the gradients of a lone adjoint bit for
+bit."""
 function adjoint_batch(ctxs::Vector{TitusContext}, dose_bars::Vector{Vector{Float64}};
-                       nthreads::Integer=0)
+                       nthreads::Integer=0, lockstep::Bool=false, groups::Integer=8)
     grads = [zeros(c.C) for c in ctxs]
     ptrs = [c.ptr for c in ctxs]
     status = zeros(Int32, length(ctxs))
     GC.@preserve dose_bars grads begin
         bars = [pointer(b) for b in dose_bars]
         gps = [pointer(g) for g in grads]
-        check(ccall((:titus_adjoint_batch, libtitus), Cint,
-                    (Ptr{Ptr{Cvoid}}, Int32, Int32, Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}, Ptr{Int32}),
-                    ptrs, Int32(length(ctxs)), Int32(nthreads), bars, gps, status))
+        if lockstep
+            check(ccall((:titus_adjoint_batch_lockstep, libtitus), Cint,
+                        (Ptr{Ptr{Cvoid}}, Int32, Int32, Int32, Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}},
+                         Ptr{Int32}),
+                        ptrs, Int32(length(ctxs)), Int32(0), Int32(groups), bars, gps, status))
+        else
+            check(ccall((:titus_adjoint_batch, libtitus), Cint,
+                        (Ptr{Ptr{Cvoid}}, Int32, Int32, Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}, Ptr{Int32}),
+                        ptrs, Int32(length(ctxs)), Int32(nthreads), bars, gps, status))
+        end
     end
     return grads
 end
@@ -486,7 +497,9 @@ J(w, s) = || sum_b w_b D_b(rho_0 .* s) - target ||^2 over the beamlet weights w
 and the voxel density scalings s, where D_b is the fixed-rank DLRA dose of
 beamlet solver `objs[b]` (SolveFirstCollisionSourceDLR) and rho_0 the density
 of `objs[1]` (the beamlets share the phantom).  Differentiable through the
-`rrule` below (ChainRulesCore 1.15, in the reference's Manifest).
+`rrule` below (ChainRulesCore 1.15, in the reference's Manifest).  The
+forward solves and the adjoints run with shared launches (8 groups); with
+`nthreads > 0` they run one stream per beamlet on that many host threads.
 """
 function plan_objective(objs, w::AbstractVector, s::AbstractVector, target::AbstractVector;
                         every::Integer=16, nthreads::Integer=0)
@@ -504,7 +517,11 @@ function plan_forward(objs, w, s, target; every::Integer=16, nthreads::Integer=0
             push!(ctxs, create(obj, TITUS_INTEGRATOR_FIXED; density=rho))
         end
         foreach(c -> set_checkpoints!(c, every), ctxs)
-        solve_batch!(ctxs; nthreads=nthreads)
+        if nthreads > 0
+            solve_batch!(ctxs; nthreads=nthreads)
+        else
+            solve_batch_lockstep!(ctxs; groups=8)
+        end
     catch
         foreach(destroy!, ctxs)
         rethrow()
@@ -516,7 +533,7 @@ function plan_forward(objs, w, s, target; every::Integer=16, nthreads::Integer=0
         try
             wbar = Jbar .* 2 .* (D' * resid)
             bars = [Jbar .* 2 .* Float64(w[b]) .* resid for b in eachindex(ctxs)]
-            grads = adjoint_batch(ctxs, bars; nthreads=nthreads)   # d/d rho per beamlet
+            grads = adjoint_batch(ctxs, bars; nthreads=nthreads, lockstep=nthreads <= 0)
             sbar = rho0 .* reduce(+, grads)
             return wbar, sbar
         finally

@@ -100,7 +100,8 @@ def test_every_ccall_matches_its_prototype():
     for need in ("titus_create", "titus_solve", "titus_get_state", "titus_progress",
                  "titus_get_history", "titus_destroy", "titus_solve_batch",
                  "titus_solve_batch_lockstep",
-                 "titus_set_checkpoints", "titus_adjoint_batch", "titus_last_error",
+                 "titus_set_checkpoints", "titus_adjoint_batch", "titus_adjoint_batch_lockstep",
+                 "titus_last_error",
                  "titus_get_cfl", "titus_set_rejection", "titus_get_rejections"):
         assert need in seen, need
 
