@@ -308,6 +308,7 @@ struct titus_ctx {
   int r_start = 0;
 
   ~titus_ctx() {
+    for (const CapturedRange& g : range_graphs) (void)hipGraphExecDestroy(g.exec);
     void* ptrs[] = {rho, irho, omega, bc_base, m0, psi, psi2, dose, u0p, X, Xn, K, Kc, Q1, Q2, partial, arena,
                     upl, bcells, cand_d, bmask, restore, X_start, dose_start, u0_start, psi_start,
                     xmid_d, ymid_d, ang_rowptr, ang_col, ang_opnz, ang_order, ang_val, AWd, Zd};
@@ -649,10 +650,62 @@ struct titus_ctx {
   }
 
   // ---- device range basis --------------------------------------------------------
+  // The device part of a range basis is the same ~20 launches whenever its
+  // (K block, basis block, widths, arena position) recur, which is every step
+  // at a steady rank: the sequence is captured once into a graph and then
+  // replayed by one launch.  Every argument of those kernels is a device
+  // address or a constant (the coefficient blocks are arena results), so a
+  // replay computes what the launches compute.  Not in a lockstep group (its
+  // launches are recorded), on timed steps (events around the launches) or
+  // while the adjoint tapes a step.
+  struct CapturedRange {
+    double* Kw;
+    const double* Xa;
+    int k, q;
+    size_t top0, top1;     // arena position before and after the job
+    RangeJob job;
+    hipGraphExec_t exec;
+  };
+  std::vector<CapturedRange> range_graphs;   // oldest first
+  static constexpr size_t kMaxRangeGraphs = 8;
+
+  RangeJob range_launch(double* Kw, int k, const double* Xa, int q) {
+    if (group || dev::t_recorder || timing.enabled || tape) return range_launch_direct(Kw, k, Xa, q);
+    const size_t top0 = arena_top;
+    for (const CapturedRange& g : range_graphs)
+      if (g.Kw == Kw && g.Xa == Xa && g.k == k && g.q == q && g.top0 == top0) {
+        HIPCHK(hipGraphLaunch(g.exec, st));
+        arena_top = g.top1;
+        return g.job;
+      }
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    RangeJob j;
+    try {
+      j = range_launch_direct(Kw, k, Xa, q);
+    } catch (...) {
+      (void)hipStreamEndCapture(st, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(st, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(e);
+    if (range_graphs.size() >= kMaxRangeGraphs) {
+      (void)hipGraphExecDestroy(range_graphs.front().exec);
+      range_graphs.erase(range_graphs.begin());
+    }
+    range_graphs.push_back(CapturedRange{Kw, Xa, k, q, top0, arena_top, j, exec});
+    HIPCHK(hipGraphLaunch(exec, st));
+    return j;
+  }
+
   // Launch the device part of the basis of range(Kw) minus span(Xa): the
   // projection (twice), three Gram + shifted-Cholesky passes and the candidate
   // rows; everything the host needs lands in the arena.  Kw is overwritten.
-  RangeJob range_launch(double* Kw, int k, const double* Xa, int q) {
+  RangeJob range_launch_direct(double* Kw, int k, const double* Xa, int q) {
     RangeJob j;
     j.Kw = Kw; j.k = k; j.Xa = (Xa && q > 0) ? Xa : nullptr; j.q = j.Xa ? q : 0;
     if (j.Xa) {
