@@ -143,3 +143,71 @@ def test_svd_completed_matches_oracle(lib):
     np.testing.assert_allclose(U @ np.diag(d) @ V.T, S, atol=1e-12)
     np.testing.assert_allclose(U[:, 3:] @ U[:, 3:].T, Uo[:, 3:] @ Uo[:, 3:].T, atol=1e-12)
     np.testing.assert_allclose(U.T @ U, np.eye(n), atol=1e-13)
+
+
+def _csvd_completed(lib, S, tol=1e-7):
+    n = S.shape[0]
+    U, d, V = f(np.zeros((n, n))), np.zeros(n), f(np.zeros((n, n)))
+    assert lib.smallla_svd_completed(n, dp(f(S)), tol, dp(U), dp(d), dp(V)) == 0
+    return U, d, V
+
+
+def _orthogonal(rng, n):
+    q, r = np.linalg.qr(rng.standard_normal((n, n)))
+    return q * np.sign(np.diag(r))
+
+
+@pytest.mark.parametrize("kind", ["random", "graded", "rank_deficient", "zero_rows_cols", "repeated",
+                                  "zero", "tiny_sizes"])
+def test_truncation_svd_against_numpy(lib, kind):
+    """The truncation's SVD (bidiagonalization and QR sweeps) on the shapes a
+    rank-adaptive solve hands it: kept singular values to 1e-13 of sigma_1,
+    U D V^T equal to numpy's truncation of S at the same kept count,
+    orthonormal U and V, descending values, zeros beyond the kept count, and
+    the null directions completed as the oracle completes them."""
+    rng = np.random.default_rng(["random", "graded", "rank_deficient", "zero_rows_cols", "repeated",
+                                 "zero", "tiny_sizes"].index(kind) + 40)
+    sizes = [1, 2, 3] if kind == "tiny_sizes" else [4, 9, 18, 31, 40]
+    for n in sizes:
+        if kind in ("random", "tiny_sizes"):
+            S = rng.standard_normal((n, n))
+        elif kind == "graded":
+            # singular values 1 .. 1e-11 over n, never within 1 % of the 1e-7 cut
+            sv = 10.0 ** -np.linspace(0.0, 11.0, n)
+            sv = np.where(np.abs(np.log10(sv) + 7.0) < 0.005, sv * 1.5, sv)
+            S = _orthogonal(rng, n) @ np.diag(sv) @ _orthogonal(rng, n).T
+        elif kind == "rank_deficient":
+            k = max(1, n // 3)
+            S = rng.standard_normal((n, k)) @ rng.standard_normal((k, n))
+        elif kind == "zero_rows_cols":
+            S = rng.standard_normal((n, n))
+            S[:, : n // 3] = 0.0
+            S[n // 2 + 1:, :] = 0.0
+        elif kind == "repeated":
+            sv = np.repeat([3.0, 1.0, 0.25], -(-n // 3))[:n]
+            S = _orthogonal(rng, n) @ np.diag(sv) @ _orthogonal(rng, n).T
+        else:
+            S = np.zeros((n, n))
+        U, d, V = _csvd_completed(lib, S)
+        Uo, so, Vto = np.linalg.svd(S)
+        p = int(np.count_nonzero(so > 1e-7 * so[0])) if so[0] > 0 else 0
+        assert int(np.count_nonzero(d)) == p, (kind, n)
+        assert np.all(np.diff(d) <= 0.0), (kind, n)
+        np.testing.assert_allclose(U.T @ U, np.eye(n), atol=1e-13)
+        np.testing.assert_allclose(V.T @ V, np.eye(n), atol=1e-13)
+        if p:
+            np.testing.assert_allclose(d[:p], so[:p], rtol=0, atol=1e-13 * so[0])
+            St = (Uo[:, :p] * so[:p]) @ Vto[:p]
+            np.testing.assert_allclose(U @ np.diag(d) @ V.T, St, atol=1e-13 * so[0])
+        Uc, dc, Vc = dlra.svd_completed(S)
+        # the completion columns span what the oracle's span, to what the
+        # SVD determines them (sin of the subspace angle <= eps ||S|| / gap
+        # at the cut, Wedin's bound); for a zero block they are the oracle's
+        # identity columns
+        gap = (so[p - 1] - (so[p] if p < n else 0.0)) if p else 1.0
+        tol = max(1e-11, 50.0 * np.finfo(float).eps * so[0] / gap) if p else 1e-11
+        np.testing.assert_allclose(U[:, p:] @ U[:, p:].T, Uc[:, p:] @ Uc[:, p:].T, atol=tol)
+        np.testing.assert_allclose(V[:, p:] @ V[:, p:].T, Vc[:, p:] @ Vc[:, p:].T, atol=tol)
+        if kind == "zero":
+            np.testing.assert_array_equal(U, np.eye(n))
+            np.testing.assert_array_equal(V, np.eye(n))
