@@ -721,8 +721,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-events", action="store_true",
                     help="time without per-kernel HIP events (no roofline)")
-    ap.add_argument("--event-period", type=int, default=10,
-                    help="HIP events on every n-th step of the timed region")
+    ap.add_argument("--event-period", type=int, default=20,
+                    help="HIP events on every n-th step of the timed region (a sampled step "
+                         "brackets every launch with events and replays no captured graph; "
+                         "one in 20 gives the roofline kernel 16 timed launches per sample)")
     ap.add_argument("--beamlets", type=int, default=64,
                     help="beamlets of the batched plan (0: skip)")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1),
