@@ -85,7 +85,13 @@ def oracle(nx, ny, nz, h, rho, X):
 
 
 @pytest.mark.parametrize("shape,r", [((40, 37, 1), 9), ((200, 200, 1), 18), ((12, 10, 11), 30),
-                                     ((64, 64, 64), 30), ((5, 4, 1), 1)])
+                                     ((64, 64, 64), 30), ((5, 4, 1), 1),
+                                     # 2D grids of at least 2^17 cells: strips walked row by
+                                     # row (vector loads; ragged strips with per-cell loads;
+                                     # partial tiles; 7 interior rows, one band of four per
+                                     # workgroup); more strips than workgroup slots: cell runs
+                                     ((512, 512, 1), 16), ((400, 330, 1), 18), ((366, 365, 1), 9),
+                                     ((9, 16000, 1), 5), ((3, 43700, 1), 5)])
 def test_galerkin_products_match_the_oracle(shape, r):
     nx, ny, nz = shape
     print(f"\n[case] titus_galerkin {nx}x{ny}x{nz} r={r}", flush=True)
@@ -102,7 +108,7 @@ def test_galerkin_products_match_the_oracle(shape, r):
 
 
 @pytest.mark.parametrize("shape,r", [((200, 200, 1), 18), ((64, 64, 64), 12), ((9, 8, 1), 5),
-                                     ((6, 7, 8), 7)])
+                                     ((6, 7, 8), 7), ((512, 512, 1), 20)])
 def test_vector_and_per_cell_stencil_paths_agree_bitwise(shape, r):
     """The stencil products read B/rho with vector loads when the leading
     dimension and the unit-stride axis allow it, and cell by cell otherwise;

@@ -17,6 +17,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kMaxTnSlots = 1024;    // workgroups (partial slots) of one TN reduction
 constexpr int kTnBTiles = 4;         // B tiles of one A^T B wavefront (shared A loads)
+constexpr int kStripMinCells = 1 << 17;  // 2D stencil products walk strips from this size
 constexpr int kXcds = 8;             // accelerator complex dies of an MI355X
 constexpr int kSweepCols = 62;       // output columns of one sweep wavefront
 constexpr int kRowsKT = 256;         // rows of T staged in LDS at a time (k_gemm)
@@ -803,6 +804,172 @@ __device__ __forceinline__ void d_tn_mfma(Blk blk,
   }
 }
 
+// The 2D stencil products of d_tn_mfma (NOPS = 4: L1x, L1y, L2x, L2y of
+// B/rho) for large grids, with every value of A, B and 1/rho loaded once.
+// Cells c = i*ny + j.  A wavefront owns a strip of 16 cells along j (lane
+// (col, kg) holds cells j0 + 4kg .. j0 + 4kg + 3 of its column) and walks a
+// band of `rows` interior rows along i.  The rows i - 1, i and i + 1 of its
+// strip stay in registers, so each step loads one new row, one row ahead of
+// its use; the in-row neighbours of the strip's edge cells are two more
+// loads of the same row.  The four wavefronts of a workgroup take four
+// consecutive bands of one strip, and neighbouring bands walk in opposite
+// directions, so a band's first and last halo rows are read while the
+// neighbouring band reads them as its own.  Workgroups: tile group fastest,
+// then strip, then band group, XCD-contiguous (the strips of a row, whose
+// edge cells are neighbours, run on one XCD).  K-slot k of MFMA step t is
+// cell j0 + 4k + t of the current row.  Each interior cell's stencil values
+// are formed as d_tn_mfma forms them; the sums run over the cells in another
+// order.
+// partial[chunk][o*mn + a + b*m], chunk = band group * ns + strip.
+template <class TB, class TC>
+__device__ __forceinline__ void d_tn_strip(Blk blk,
+    const double* __restrict__ A, int lda, int m, const TB* __restrict__ B, int ldb, int n,
+    int nx, int ny, double idx_, double idy_, const double* __restrict__ irho, int ns, int rows,
+    int ngroups, int groups_n, double* __restrict__ partial) {
+  using MF = Mfma<TC>;
+  using acc_t = typename MF::acc_t;
+  __shared__ acc_t red[3][4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int L = xcd_block(blk.x, blk.nx);
+  const int grp = L % ngroups, chunk = L / ngroups;
+  const int tm = grp / groups_n, tn = grp % groups_n;
+  const int strip = chunk % ns, band = (chunk / ns) * 4 + wave;
+  const int col = lane & 15, kg = lane >> 4;
+  const int ca = tm * 16 + col, cb = tn * 16 + col;
+  const bool va = ca < m, vb = cb < n;
+  const double* Acol = A + (size_t)(va ? ca : 0) * lda;
+  const TB* Bcol = B + (size_t)(vb ? cb : 0) * ldb;
+  const bool same = sizeof(TB) == 8 && static_cast<const void*>(A) == static_cast<const void*>(B) &&
+                    lda == ldb && tm == tn;
+  // the lane's cells in a row (clamped into the row: every address is read)
+  // and their outer neighbours along j.  A cell off the interior enters the
+  // products with a zero A value (its stencil values are finite), so no
+  // stencil value needs a mask.
+  const int j0 = strip * 16 + 4 * kg;
+  int jc[4];
+  bool inj[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int j = j0 + t;
+    inj[t] = (j >= 1) & (j <= ny - 2);
+    jc[t] = min(j, ny - 1);
+  }
+  const int jlo = min(max(j0 - 1, 0), ny - 1), jhi = min(j0 + 4, ny - 1);
+  // vector loads when a lane's four cells are one aligned vector: then they
+  // lie all in the row or all beyond it (those read the row's first four)
+  const bool vec = ny % 4 == 0 && lda % 4 == 0 && ldb % 4 == 0 &&
+                   reinterpret_cast<uintptr_t>(A) % 32 == 0 &&
+                   reinterpret_cast<uintptr_t>(B) % (4 * sizeof(TB)) == 0;
+  const int jv = j0 + 3 < ny ? j0 : 0;
+  // 1/rho table of a row (a 16-lane row shares its cells): value q of
+  // [cells 0..3, lo, hi] is loaded by lane q of the row and broadcast
+  const int jt = col < 4 ? min(j0 + col, ny - 1) : col == 4 ? jlo : col == 5 ? jhi : jc[0];
+  // centre rows of the band, walked up (even bands) or down (odd bands)
+  const int ib = 1 + band * rows, ie = min(nx - 1, ib + rows);
+  const int cnt = max(0, ie - ib);
+  const TC h = TC(0.5);
+  const TC ih0 = (TC)idx_, ih1 = (TC)idy_;
+  acc_t acc[4];
+#pragma unroll
+  for (int o = 0; o < 4; ++o) acc[o] = acc_t{};
+
+  struct Raw {
+    TC b[4], a[4], lo, hi;
+    double ir;
+  };
+  struct Row {
+    TC y[4], a[4], lo, hi;
+  };
+  auto load = [&](int i, Raw& r) {
+    const size_t base = (size_t)i * ny;
+    if (vec) {
+      load4(Bcol + base + jv, true, r.b);
+      if (!same) load4(Acol + base + jv, va, r.a);
+    } else {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        r.b[t] = (TC)Bcol[base + jc[t]];
+        if (!same) r.a[t] = va ? (TC)Acol[base + jc[t]] : TC(0);
+      }
+    }
+    if (same)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) r.a[t] = va ? r.b[t] : TC(0);
+    r.lo = (TC)Bcol[base + jlo];
+    r.hi = (TC)Bcol[base + jhi];
+    r.ir = irho[base + jt];
+  };
+  auto convert = [&](const Raw& r, Row& w) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      w.y[t] = r.b[t] * (TC)row_bcast(r.ir, t);
+      w.a[t] = inj[t] ? r.a[t] : TC(0);
+    }
+    w.lo = r.lo * (TC)row_bcast(r.ir, 4);
+    w.hi = r.hi * (TC)row_bcast(r.ir, 5);
+  };
+  // the walk in direction D (+1: increasing i); `behind` is row i - D,
+  // `ahead` row i + D, `r` the loads of row i + 2D in flight
+  auto walk = [&](auto dc) {
+    constexpr int D = decltype(dc)::value;
+    int i = D > 0 ? ib : ie - 1;
+    Raw r;
+    Row behind, cur, ahead;
+    load(i - D, r);
+    convert(r, behind);
+    load(i, r);
+    convert(r, cur);
+    load(i + D, r);
+    convert(r, ahead);
+    load(min(nx - 1, max(0, i + 2 * D)), r);
+    for (int k = 0; k < cnt; ++k) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        // neighbours along i (rows i - 1, i + 1) and along j
+        const TC vm0 = D > 0 ? behind.y[t] : ahead.y[t];
+        const TC vp0 = D > 0 ? ahead.y[t] : behind.y[t];
+        const TC vm1 = t == 0 ? cur.lo : cur.y[t - 1 < 0 ? 0 : t - 1];
+        const TC vp1 = t == 3 ? cur.hi : cur.y[t + 1 > 3 ? 3 : t + 1];
+        const TC vc = cur.y[t];
+        TC bv[4];
+        bv[0] = (vc - h * vm0 - h * vp0) * ih0;
+        bv[1] = (vc - h * vm1 - h * vp1) * ih1;
+        bv[2] = (vp0 - vm0) * (h * ih0);
+        bv[3] = (vp1 - vm1) * (h * ih1);
+#pragma unroll
+        for (int o = 0; o < 4; ++o) acc[o] = MF::step(cur.a[t], bv[o], acc[o]);
+      }
+      behind = cur;
+      cur = ahead;
+      convert(r, ahead);
+      i += D;
+      load(min(nx - 1, max(0, i + 2 * D)), r);
+    }
+  };
+  if (cnt > 0) {
+    if (band & 1) walk(std::integral_constant<int, -1>{});
+    else walk(std::integral_constant<int, 1>{});
+  }
+  if (wave > 0)
+#pragma unroll
+    for (int o = 0; o < 4; ++o) red[wave - 1][o][lane] = acc[o];
+  __syncthreads();
+  if (wave != 0) return;
+#pragma unroll
+  for (int w = 0; w < 3; ++w)
+#pragma unroll
+    for (int o = 0; o < 4; ++o) acc[o] += red[w][o][lane];
+  if (!vb) return;
+  const size_t mn = (size_t)m * n;
+#pragma unroll
+  for (int o = 0; o < 4; ++o)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int a = tm * 16 + MF::row(kg, v);
+      if (a < m) partial[((size_t)chunk * 4 + o) * mn + a + (size_t)cb * m] = (double)acc[o][v];
+    }
+}
+
 // One wavefront per output: lane l sums chunks l, l+64, ...; then a fixed
 // shuffle tree (deterministic order).
 __device__ __forceinline__ void d_tn_reduce(Blk blk, const double* __restrict__ partial, int nchunks, int mn,
@@ -1292,6 +1459,19 @@ struct TnBody {
                                  p.nbt, p.partial);
   }
 };
+template <class TB, class TC>
+struct TnStripBody {
+  struct P {
+    const double* A; int lda, m; const TB* B; int ldb, n, nx, ny; double idx, idy;
+    const double* irho; int ns, rows, ngroups, groups_n; double* partial;
+  };
+  static constexpr int kThreads = kBlock;
+  static constexpr bool kBatch = true;
+  __device__ static void run(const P& p, Blk b) {
+    d_tn_strip<TB, TC>(b, p.A, p.lda, p.m, p.B, p.ldb, p.n, p.nx, p.ny, p.idx, p.idy, p.irho,
+                       p.ns, p.rows, p.ngroups, p.groups_n, p.partial);
+  }
+};
 struct TnReduceBody {
   struct P { const double* partial; int nchunks, mn; double* out; };
   static constexpr int kThreads = kBlock;
@@ -1391,6 +1571,17 @@ void tn_mfma_launch(hipStream_t st, int blocks, const void* A, int lda, int m, c
     record_variant<Body>(dim3(blocks), 0, p, nbt - 1, alts);
     return;
   }
+  launch_body<Body>(st, dim3(blocks), 0, p);
+}
+
+template <class TB, class TC>
+void tn_strip_launch(hipStream_t st, int blocks, const void* A, int lda, int m, const void* B,
+                     int ldb, int n, const Grid& g, int ns, int rows, int ngroups, int groups_n,
+                     double* partial) {
+  using Body = TnStripBody<TB, TC>;
+  const typename Body::P p{static_cast<const double*>(A), lda, m, static_cast<const TB*>(B),
+                           ldb, n, g.nx, g.ny, 1.0 / g.dx, 1.0 / g.dy, g.irho, ns, rows,
+                           ngroups, groups_n, partial};
   launch_body<Body>(st, dim3(blocks), 0, p);
 }
 
@@ -1675,9 +1866,39 @@ void launch_tn(hipStream_t st, const Grid& g, const void* A, DT ta, int lda, int
   const int nbt = nops > 1 ? 1 : std::min(tiles_n, kTnBTiles);
   const int groups_n = (tiles_n + nbt - 1) / nbt;
   const int ngroups = tiles_m * groups_n;
+  const size_t per_chunk = (size_t)nops * m * n;
+  const int mn = m * n;
+  const int per_block = kBlock / 64;
+  const int outs = nops * mn;
+  // 2D stencil products of a large grid: strips walked row by row
+  // (d_tn_strip), about kMaxTnSlots workgroups over the tile groups
+  if (nops == 4 && g.C >= kStripMinCells && g.nx >= 3 && g.ny >= 3) {
+    if (ta != DT::F64) throw std::runtime_error("tn: the left block is always FP64");
+    const int ns = (g.ny + 15) / 16, interior = g.nx - 2;
+    int nbg = std::min(kMaxTnSlots / (ns * ngroups), (interior + 3) / 4);
+    nbg = (int)std::min<size_t>(nbg, tn_partial_size() / ((size_t)ns * per_chunk));
+    if (nbg >= 1) {
+      // bands of at least 8 rows (three rows of a band are loaded before its first)
+      const int rows = std::max(8, (interior + 4 * nbg - 1) / (4 * nbg));
+      nbg = ((interior + rows - 1) / rows + 3) / 4;
+      const int nchunks = ns * nbg;
+      const int blocks = nchunks * ngroups;
+      if (tb == DT::F64) {
+        if (fp32) tn_strip_launch<double, float>(st, blocks, A, lda, m, B, ldb, n, g, ns, rows, ngroups, groups_n, partial);
+        else tn_strip_launch<double, double>(st, blocks, A, lda, m, B, ldb, n, g, ns, rows, ngroups, groups_n, partial);
+      } else {
+        if (fp32) tn_strip_launch<float, float>(st, blocks, A, lda, m, B, ldb, n, g, ns, rows, ngroups, groups_n, partial);
+        else tn_strip_launch<float, double>(st, blocks, A, lda, m, B, ldb, n, g, ns, rows, ngroups, groups_n, partial);
+      }
+      check_launch("tn_strip");
+      launch_body<TnReduceBody>(st, dim3((outs + per_block - 1) / per_block), 0,
+                                {partial, nchunks, outs, out});
+      check_launch("tn_reduce");
+      return;
+    }
+  }
   // cell runs of at least 64 cells (a multiple of 16) per wavefront; at most
   // kMaxTnSlots workgroups, and partials within tn_partial_size()
-  const size_t per_chunk = (size_t)nops * m * n;
   int nchunks = std::max(1, std::min((g.C + 255) / 256, kMaxTnSlots / ngroups));
   nchunks = std::max(1, (int)std::min<size_t>(nchunks, tn_partial_size() / per_chunk));
   int cpw = (g.C + 4 * nchunks - 1) / (4 * nchunks);
@@ -1701,9 +1922,6 @@ void launch_tn(hipStream_t st, const Grid& g, const void* A, DT ta, int lda, int
     }
   }
   check_launch("tn_partial");
-  const int mn = m * n;
-  const int per_block = kBlock / 64;
-  const int outs = nops * mn;
   launch_body<TnReduceBody>(st, dim3((outs + per_block - 1) / per_block), 0,
                             {partial, nchunks, outs, out});
   check_launch("tn_reduce");
