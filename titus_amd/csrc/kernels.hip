@@ -572,6 +572,23 @@ __device__ inline void load4(const TS* p, bool ok, TC* v) {
   }
 }
 
+// Four consecutive values at p (32-byte aligned for double, 16 for float) in
+// one vector load, as loaded.
+template <class TS>
+struct Quad {
+  TS v[4];
+};
+template <class TS>
+__device__ inline Quad<TS> load4raw(const TS* p) {
+  if constexpr (sizeof(TS) == 8) {
+    const double4 q = *reinterpret_cast<const double4*>(p);
+    return {{q.x, q.y, q.z, q.w}};
+  } else {
+    const float4 q = *reinterpret_cast<const float4*>(p);
+    return {{q.x, q.y, q.z, q.w}};
+  }
+}
+
 template <int NOPS, int NBT, class TB, class TC>
 __device__ __forceinline__ void d_tn_mfma(Blk blk, 
     const double* __restrict__ A, int lda, int m, const TB* __restrict__ B, int ldb, int n,
@@ -821,7 +838,7 @@ __device__ __forceinline__ void d_tn_mfma(Blk blk,
 // are formed as d_tn_mfma forms them; the sums run over the cells in another
 // order.
 // partial[chunk][o*mn + a + b*m], chunk = band group * ns + strip.
-template <class TB, class TC>
+template <class TB, class TC, bool VEC, bool SAME>
 __device__ __forceinline__ void d_tn_strip(Blk blk,
     const double* __restrict__ A, int lda, int m, const TB* __restrict__ B, int ldb, int n,
     int nx, int ny, double idx_, double idy_, const double* __restrict__ irho, int ns, int rows,
@@ -839,8 +856,6 @@ __device__ __forceinline__ void d_tn_strip(Blk blk,
   const bool va = ca < m, vb = cb < n;
   const double* Acol = A + (size_t)(va ? ca : 0) * lda;
   const TB* Bcol = B + (size_t)(vb ? cb : 0) * ldb;
-  const bool same = sizeof(TB) == 8 && static_cast<const void*>(A) == static_cast<const void*>(B) &&
-                    lda == ldb && tm == tn;
   // the lane's cells in a row (clamped into the row: every address is read)
   // and their outer neighbours along j.  A cell off the interior enters the
   // products with a zero A value (its stencil values are finite), so no
@@ -855,11 +870,8 @@ __device__ __forceinline__ void d_tn_strip(Blk blk,
     jc[t] = min(j, ny - 1);
   }
   const int jlo = min(max(j0 - 1, 0), ny - 1), jhi = min(j0 + 4, ny - 1);
-  // vector loads when a lane's four cells are one aligned vector: then they
-  // lie all in the row or all beyond it (those read the row's first four)
-  const bool vec = ny % 4 == 0 && lda % 4 == 0 && ldb % 4 == 0 &&
-                   reinterpret_cast<uintptr_t>(A) % 32 == 0 &&
-                   reinterpret_cast<uintptr_t>(B) % (4 * sizeof(TB)) == 0;
+  // VEC: a lane's four cells are one aligned vector, all in the row or all
+  // beyond it (those read the row's first four)
   const int jv = j0 + 3 < ny ? j0 : 0;
   // 1/rho table of a row (a 16-lane row shares its cells): value q of
   // [cells 0..3, lo, hi] is loaded by lane q of the row and broadcast
@@ -873,8 +885,12 @@ __device__ __forceinline__ void d_tn_strip(Blk blk,
 #pragma unroll
   for (int o = 0; o < 4; ++o) acc[o] = acc_t{};
 
+  // raw values of a row, untouched until convert() (one row later), so that
+  // no wait for the loads splits the walk
   struct Raw {
-    TC b[4], a[4], lo, hi;
+    Quad<TB> b;
+    Quad<double> a;  // (SAME: unused)
+    TB lo, hi;
     double ir;
   };
   struct Row {
@@ -882,31 +898,29 @@ __device__ __forceinline__ void d_tn_strip(Blk blk,
   };
   auto load = [&](int i, Raw& r) {
     const size_t base = (size_t)i * ny;
-    if (vec) {
-      load4(Bcol + base + jv, true, r.b);
-      if (!same) load4(Acol + base + jv, va, r.a);
+    if constexpr (VEC) {
+      r.b = load4raw(Bcol + base + jv);
+      if constexpr (!SAME) r.a = load4raw(Acol + base + jv);
     } else {
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        r.b[t] = (TC)Bcol[base + jc[t]];
-        if (!same) r.a[t] = va ? (TC)Acol[base + jc[t]] : TC(0);
+        r.b.v[t] = Bcol[base + jc[t]];
+        if constexpr (!SAME) r.a.v[t] = Acol[base + jc[t]];
       }
     }
-    if (same)
-#pragma unroll
-      for (int t = 0; t < 4; ++t) r.a[t] = va ? r.b[t] : TC(0);
-    r.lo = (TC)Bcol[base + jlo];
-    r.hi = (TC)Bcol[base + jhi];
+    r.lo = Bcol[base + jlo];
+    r.hi = Bcol[base + jhi];
     r.ir = irho[base + jt];
   };
   auto convert = [&](const Raw& r, Row& w) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      w.y[t] = r.b[t] * (TC)row_bcast(r.ir, t);
-      w.a[t] = inj[t] ? r.a[t] : TC(0);
+      w.y[t] = (TC)r.b.v[t] * (TC)row_bcast(r.ir, t);
+      const TC a = SAME ? (TC)r.b.v[t] : (TC)r.a.v[t];
+      w.a[t] = (inj[t] & va) ? a : TC(0);
     }
-    w.lo = r.lo * (TC)row_bcast(r.ir, 4);
-    w.hi = r.hi * (TC)row_bcast(r.ir, 5);
+    w.lo = (TC)r.lo * (TC)row_bcast(r.ir, 4);
+    w.hi = (TC)r.hi * (TC)row_bcast(r.ir, 5);
   };
   // the walk in direction D (+1: increasing i); `behind` is row i - D,
   // `ahead` row i + D, `r` the loads of row i + 2D in flight
@@ -1459,7 +1473,7 @@ struct TnBody {
                                  p.nbt, p.partial);
   }
 };
-template <class TB, class TC>
+template <class TB, class TC, bool VEC, bool SAME>
 struct TnStripBody {
   struct P {
     const double* A; int lda, m; const TB* B; int ldb, n, nx, ny; double idx, idy;
@@ -1468,7 +1482,7 @@ struct TnStripBody {
   static constexpr int kThreads = kBlock;
   static constexpr bool kBatch = true;
   __device__ static void run(const P& p, Blk b) {
-    d_tn_strip<TB, TC>(b, p.A, p.lda, p.m, p.B, p.ldb, p.n, p.nx, p.ny, p.idx, p.idy, p.irho,
+    d_tn_strip<TB, TC, VEC, SAME>(b, p.A, p.lda, p.m, p.B, p.ldb, p.n, p.nx, p.ny, p.idx, p.idy, p.irho,
                        p.ns, p.rows, p.ngroups, p.groups_n, p.partial);
   }
 };
@@ -1574,15 +1588,37 @@ void tn_mfma_launch(hipStream_t st, int blocks, const void* A, int lda, int m, c
   launch_body<Body>(st, dim3(blocks), 0, p);
 }
 
-template <class TB, class TC>
-void tn_strip_launch(hipStream_t st, int blocks, const void* A, int lda, int m, const void* B,
-                     int ldb, int n, const Grid& g, int ns, int rows, int ngroups, int groups_n,
-                     double* partial) {
-  using Body = TnStripBody<TB, TC>;
+template <class TB, class TC, bool VEC, bool SAME>
+void tn_strip_body(hipStream_t st, int blocks, const void* A, int lda, int m, const void* B,
+                   int ldb, int n, const Grid& g, int ns, int rows, int ngroups, int groups_n,
+                   double* partial) {
+  using Body = TnStripBody<TB, TC, VEC, SAME>;
   const typename Body::P p{static_cast<const double*>(A), lda, m, static_cast<const TB*>(B),
                            ldb, n, g.nx, g.ny, 1.0 / g.dx, 1.0 / g.dy, g.irho, ns, rows,
                            ngroups, groups_n, partial};
   launch_body<Body>(st, dim3(blocks), 0, p);
+}
+
+// Vector loads when every lane's four cells are one aligned vector (rows of
+// a multiple of four cells, aligned columns); A not loaded when it is B and
+// the product is one tile group.
+template <class TB, class TC>
+void tn_strip_launch(hipStream_t st, int blocks, const void* A, int lda, int m, const void* B,
+                     int ldb, int n, const Grid& g, int ns, int rows, int ngroups, int groups_n,
+                     double* partial) {
+  const bool vec = g.ny % 4 == 0 && lda % 4 == 0 && ldb % 4 == 0 &&
+                   reinterpret_cast<uintptr_t>(A) % 32 == 0 &&
+                   reinterpret_cast<uintptr_t>(B) % (4 * sizeof(TB)) == 0;
+  const bool same = sizeof(TB) == 8 && A == B && lda == ldb && ngroups == 1;
+  if constexpr (sizeof(TB) == 8) {
+    if (same) {
+      if (vec) tn_strip_body<TB, TC, true, true>(st, blocks, A, lda, m, B, ldb, n, g, ns, rows, ngroups, groups_n, partial);
+      else tn_strip_body<TB, TC, false, true>(st, blocks, A, lda, m, B, ldb, n, g, ns, rows, ngroups, groups_n, partial);
+      return;
+    }
+  }
+  if (vec) tn_strip_body<TB, TC, true, false>(st, blocks, A, lda, m, B, ldb, n, g, ns, rows, ngroups, groups_n, partial);
+  else tn_strip_body<TB, TC, false, false>(st, blocks, A, lda, m, B, ldb, n, g, ns, rows, ngroups, groups_n, partial);
 }
 
 template <int NOPS, int NBT>
