@@ -18,6 +18,7 @@ constexpr int kBlock = 256;
 constexpr int kMaxTnSlots = 1024;    // workgroups (partial slots) of one TN reduction
 constexpr int kTnBTiles = 4;         // B tiles of one A^T B wavefront (shared A loads)
 constexpr int kStripMinCells = 1 << 17;  // 2D stencil products walk strips from this size
+constexpr int kStripTargetWaves = 4096;   // wavefronts of one strip launch (4 per SIMD)
 constexpr int kXcds = 8;             // accelerator complex dies of an MI355X
 constexpr int kSweepCols = 62;       // output columns of one sweep wavefront
 constexpr int kRowsKT = 256;         // rows of T staged in LDS at a time (k_gemm)
@@ -828,7 +829,7 @@ __device__ __forceinline__ void d_tn_mfma(Blk blk,
 // band of `rows` interior rows along i.  The rows i - 1, i and i + 1 of its
 // strip stay in registers, so each step loads one new row, one row ahead of
 // its use; the in-row neighbours of the strip's edge cells are two more
-// loads of the same row.  The four wavefronts of a workgroup take four
+// loads of the same row.  The kStripWaves wavefronts of a workgroup take
 // consecutive bands of one strip, and neighbouring bands walk in opposite
 // directions, so a band's first and last halo rows are read while the
 // neighbouring band reads them as its own.  Workgroups: tile group fastest,
@@ -838,6 +839,7 @@ __device__ __forceinline__ void d_tn_mfma(Blk blk,
 // are formed as d_tn_mfma forms them; the sums run over the cells in another
 // order.
 // partial[chunk][o*mn + a + b*m], chunk = band group * ns + strip.
+constexpr int kStripWaves = 8;
 template <class TB, class TC, bool VEC, bool SAME>
 __device__ __forceinline__ void d_tn_strip(Blk blk,
     const double* __restrict__ A, int lda, int m, const TB* __restrict__ B, int ldb, int n,
@@ -845,12 +847,12 @@ __device__ __forceinline__ void d_tn_strip(Blk blk,
     int ngroups, int groups_n, double* __restrict__ partial) {
   using MF = Mfma<TC>;
   using acc_t = typename MF::acc_t;
-  __shared__ acc_t red[3][4][64];
+  __shared__ acc_t red[kStripWaves - 1][4][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int L = xcd_block(blk.x, blk.nx);
   const int grp = L % ngroups, chunk = L / ngroups;
   const int tm = grp / groups_n, tn = grp % groups_n;
-  const int strip = chunk % ns, band = (chunk / ns) * 4 + wave;
+  const int strip = chunk % ns, band = (chunk / ns) * kStripWaves + wave;
   const int col = lane & 15, kg = lane >> 4;
   const int ca = tm * 16 + col, cb = tn * 16 + col;
   const bool va = ca < m, vb = cb < n;
@@ -969,8 +971,10 @@ __device__ __forceinline__ void d_tn_strip(Blk blk,
     for (int o = 0; o < 4; ++o) red[wave - 1][o][lane] = acc[o];
   __syncthreads();
   if (wave != 0) return;
-#pragma unroll
-  for (int w = 0; w < 3; ++w)
+  // one wavefront's sums at a time (all of them at once would hold
+  // 7 x 4 accumulators in registers)
+#pragma unroll 1
+  for (int w = 0; w < kStripWaves - 1; ++w)
 #pragma unroll
     for (int o = 0; o < 4; ++o) acc[o] += red[w][o][lane];
   if (!vb) return;
@@ -1479,7 +1483,7 @@ struct TnStripBody {
     const double* A; int lda, m; const TB* B; int ldb, n, nx, ny; double idx, idy;
     const double* irho; int ns, rows, ngroups, groups_n; double* partial;
   };
-  static constexpr int kThreads = kBlock;
+  static constexpr int kThreads = 64 * kStripWaves;
   static constexpr bool kBatch = true;
   __device__ static void run(const P& p, Blk b) {
     d_tn_strip<TB, TC, VEC, SAME>(b, p.A, p.lda, p.m, p.B, p.ldb, p.n, p.nx, p.ny, p.idx, p.idy, p.irho,
@@ -1907,16 +1911,18 @@ void launch_tn(hipStream_t st, const Grid& g, const void* A, DT ta, int lda, int
   const int per_block = kBlock / 64;
   const int outs = nops * mn;
   // 2D stencil products of a large grid: strips walked row by row
-  // (d_tn_strip), about kMaxTnSlots workgroups over the tile groups
+  // (d_tn_strip): about kStripTargetWaves wavefronts over the tile groups,
+  // at most kMaxTnSlots workgroups, bands of at least 8 rows (three rows of a
+  // band are loaded before its first)
   if (nops == 4 && g.C >= kStripMinCells && g.nx >= 3 && g.ny >= 3) {
     if (ta != DT::F64) throw std::runtime_error("tn: the left block is always FP64");
     const int ns = (g.ny + 15) / 16, interior = g.nx - 2;
-    int nbg = std::min(kMaxTnSlots / (ns * ngroups), (interior + 3) / 4);
+    int nbg = std::max(1, kStripTargetWaves / (kStripWaves * ns * ngroups));
+    nbg = std::min({nbg, kMaxTnSlots / (ns * ngroups), (interior + kStripWaves - 1) / kStripWaves});
     nbg = (int)std::min<size_t>(nbg, tn_partial_size() / ((size_t)ns * per_chunk));
     if (nbg >= 1) {
-      // bands of at least 8 rows (three rows of a band are loaded before its first)
-      const int rows = std::max(8, (interior + 4 * nbg - 1) / (4 * nbg));
-      nbg = ((interior + rows - 1) / rows + 3) / 4;
+      const int rows = std::max(8, (interior + kStripWaves * nbg - 1) / (kStripWaves * nbg));
+      nbg = ((interior + rows - 1) / rows + kStripWaves - 1) / kStripWaves;
       const int nchunks = ns * nbg;
       const int blocks = nchunks * ngroups;
       if (tb == DT::F64) {
