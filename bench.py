@@ -53,7 +53,7 @@ PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 # per-dispatch figures, the command and the window are in
 # profiles/dlra_step_phantom200_p21_pmc.md; the entry names them, and the
 # bench line repeats them beside the number.
-MEASURED_TRAFFIC = {"gram": {"bytes": 5.90e6, "kernels": "TnBody<1,...> + TnReduceBody",
+MEASURED_TRAFFIC = {"gram": {"bytes": 5.89e6, "kernels": "TnBody<1,...> + TnReduceBody",
                              "window": "steps 306..325, mixed form",
                              "command": "rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE) -- "
                                         "python3 bench.py --profile-form mixed --steps 20 --warmup 5",
