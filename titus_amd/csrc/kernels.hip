@@ -642,7 +642,60 @@ __device__ __forceinline__ void d_tn_mfma(Blk blk,
   // values (same column, same conversion to TC)
   const bool same = sizeof(TB) == 8 && static_cast<const void*>(A) == static_cast<const void*>(B) &&
                     lda == ldb && tm == tn0;
-  for (int c0 = cbeg; c0 < cend; c0 += 16) {
+  int cfirst = cbeg;  // first cell of the general loop below
+  if constexpr (NOPS == 1 && sizeof(TB) == 8) {
+    // plain products of FP64 operands over whole 16-cell blocks with vector
+    // loads, software pipelined (with FP32 B the extra multiply costs more
+    // than the overlap gains): block k + 1's values are loaded while block k's products
+    // issue, and converted after them.  The conversion multiplies by `one`,
+    // 1 at run time: exact for every value, and it keeps the compiler from
+    // turning the loop-carried loads back into a load next to each block's
+    // products.  The two loads past the run reload its last block (in
+    // bounds, unused).  No value is masked: output (a, b) of a tile takes
+    // only A column a and B column b, so a lane of a column past m or n (it
+    // reads column 0) feeds only outputs that are not stored.  Products and
+    // their order are those of the loop below.
+    const int nfull = vec ? max(0, (cend - cbeg) / 16) : 0;
+    if (nfull > 0) {
+      // two run-time ones (both 1: ngroups, groups_n >= 1): with one factor
+      // the compiler would merge the two conversions into one before the
+      // products and wait for the loads there
+      const TC one0 = TC(ngroups > 0 ? 1 : 0), one = TC(groups_n > 0 ? 1 : 0);
+      Quad<double> ra;
+      Quad<TB> rb[NBT];
+      TC av[4], bv[NBT][4];
+      auto load = [&](int k) {
+        const int c = cbeg + 16 * min(k, nfull - 1) + 4 * kg;
+        ra = load4raw(Acol + c);
+#pragma unroll
+        for (int t = 0; t < NBT; ++t) rb[t] = load4raw(Bcol[t] + c);
+      };
+      auto convert = [&](TC u) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          av[s] = (TC)ra.v[s] * u;
+#pragma unroll
+          for (int t = 0; t < NBT; ++t) bv[t][s] = (TC)rb[t].v[s] * u;
+        }
+      };
+      load(0);
+      convert(one0);
+      load(1);
+      for (int k = 0; k < nfull; ++k) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+          for (int o = 0; o < NACC; ++o) acc[o] = MF::step(av[s], bv[o][s], acc[o]);
+        // every product of the block issues before the wait for the next
+        // block's loads (the scheduler would otherwise interleave them)
+        __builtin_amdgcn_sched_barrier(0);
+        convert(one);
+        load(k + 2);
+      }
+      cfirst = cbeg + 16 * nfull;
+    }
+  }
+  for (int c0 = cfirst; c0 < cend; c0 += 16) {
     const int cs = c0 + 4 * kg;
     TC av[4], bv[NACC][4];
     if (NOPS == 1 && vec && c0 + 16 <= cend) {
