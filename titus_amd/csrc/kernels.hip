@@ -1146,24 +1146,54 @@ __device__ __forceinline__ void d_stream_k_mfma(Blk blk,
       ic[u] = (TC)(1.0 / rc) * m;
     }
     // unconditional loads from readable addresses (column 0 for the padding
-    // columns), zeros selected at use: no branch between the loads
-    double cur[TT][NV], nxt[TT][NV];
-    auto fetch = [&](double (&v)[TT][NV], int ab) {
+    // columns and past the last group), zeros selected in the conversion: no
+    // branch between the loads.  Software pipelined: the values of the next
+    // four columns load while the current four are multiplied, and become
+    // operands after those products (the scheduling barrier keeps the
+    // products ahead of the wait).  Each conversion ends with a multiply by
+    // a run-time one (exact), by different ones before and inside the loop,
+    // so the compiler cannot merge the loop-carried operands back into loads
+    // beside the products.  Operands and products are those of the plain
+    // per-column form.
+    constexpr int NOP = 2 + 2 * ND;
+    double raw[TT][NV];
+    TC op[TT][NOP];
+    auto fetch = [&](int ab) {
       const int a = ab + kg;
       const double* xa = X + (size_t)(a < r ? a : 0) * ldx;
 #pragma unroll
       for (int u = 0; u < TT; ++u) {
-        v[u][0] = xa[cc[u]];
+        raw[u][0] = xa[cc[u]];
 #pragma unroll
-        for (int d = 0; d < 2 * ND; ++d) v[u][1 + d] = xa[nb[u][d]];
+        for (int d = 0; d < 2 * ND; ++d) raw[u][1 + d] = xa[nb[u][d]];
       }
     };
-    fetch(cur, 0);
+    auto convert = [&](int ab, TC one_) {
+      const int a = ab + kg;
+#pragma unroll
+      for (int u = 0; u < TT; ++u) {
+        const bool ok = live[u] && a < r;
+        const TC x0 = ok ? (TC)raw[u][0] : TC(0);
+        TC xn[2 * ND];
+#pragma unroll
+        for (int d = 0; d < 2 * ND; ++d) xn[d] = ok ? (TC)raw[u][1 + d] * inv[u][d] : TC(0);
+        op[u][0] = x0 * one_;
+        op[u][1] = (x0 * ic[u]) * one_;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+          op[u][2 + 2 * d] = (xn[2 * d] + xn[2 * d + 1]) * one_;
+          op[u][3 + 2 * d] = (xn[2 * d] - xn[2 * d + 1]) * one_;
+        }
+      }
+    };
+    const TC one0 = TC(r > 0 ? 1 : 0), one1 = TC(n > 0 ? 1 : 0);
+    fetch(0);
+    convert(0, one0);
+    fetch(4);
     acc_t a0[TT], as[TT];
 #pragma unroll
     for (int u = 0; u < TT; ++u) { a0[u] = acc_t{}; as[u] = acc_t{}; }
     for (int ab = 0; ab < rp; ab += 4) {
-      if (ab + 4 < rp) fetch(nxt, ab + 4);
       const int a = ab + kg;
       const TC* z = Zs + (size_t)a * 16 + cl;
       TC za[NZB];
@@ -1171,24 +1201,17 @@ __device__ __forceinline__ void d_stream_k_mfma(Blk blk,
       for (int d = 0; d < NZB; ++d) za[d] = z[(size_t)d * rp * 16];
 #pragma unroll
       for (int u = 0; u < TT; ++u) {
-        const bool ok = live[u] && a < r;
-        const TC x0 = ok ? (TC)cur[u][0] : TC(0);
-        const TC xc = x0 * ic[u];
-        TC xn[2 * ND];
-#pragma unroll
-        for (int d = 0; d < 2 * ND; ++d) xn[d] = ok ? (TC)cur[u][1 + d] * inv[u][d] : TC(0);
-        a0[u] = MF::step(za[0], x0, a0[u]);
-        as[u] = MF::step(za[1], xc, as[u]);
+        a0[u] = MF::step(za[0], op[u][0], a0[u]);
+        as[u] = MF::step(za[1], op[u][1], as[u]);
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
-          as[u] = MF::step(za[2 + 2 * d], xn[2 * d] + xn[2 * d + 1], as[u]);
-          as[u] = MF::step(za[3 + 2 * d], xn[2 * d] - xn[2 * d + 1], as[u]);
+          as[u] = MF::step(za[2 + 2 * d], op[u][2 + 2 * d], as[u]);
+          as[u] = MF::step(za[3 + 2 * d], op[u][3 + 2 * d], as[u]);
         }
       }
-#pragma unroll
-      for (int u = 0; u < TT; ++u)
-#pragma unroll
-        for (int e = 0; e < NV; ++e) cur[u][e] = nxt[u][e];
+      __builtin_amdgcn_sched_barrier(0);
+      convert(ab + 4, one1);
+      fetch(ab + 8);
     }
 #pragma unroll
     for (int u = 0; u < TT; ++u) {
